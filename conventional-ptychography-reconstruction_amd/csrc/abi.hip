@@ -1,0 +1,186 @@
+// Error reporting, fill, residual and Adam entry points of the C ABI.
+#include "common.h"
+
+#include <string>
+
+namespace pty {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(PTY_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return PTY_OK;
+}
+
+__global__ void k_fill(float* __restrict__ p, int64_t n, float v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// out = k * x + initial  (k is a device scalar; complex x, initial, out)
+__global__ void k_residual_fwd(const float* __restrict__ k, const float2* __restrict__ x,
+                               const float2* __restrict__ init, int64_t n, float2* __restrict__ out) {
+  const float kk = *k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 a = x[i], b = init[i];
+    out[i] = make_float2(kk * a.x + b.x, kk * a.y + b.y);
+  }
+}
+
+// gx = k * gout; partial[block] = sum Re(gout * conj(x))
+__global__ __launch_bounds__(256) void k_residual_bwd(const float* __restrict__ k, const float2* __restrict__ x,
+                                                      const float2* __restrict__ g, int64_t n,
+                                                      float2* __restrict__ gx, float* __restrict__ partial) {
+  const float kk = *k;
+  KSum acc;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 a = x[i], b = g[i];
+    gx[i] = make_float2(kk * b.x, kk * b.y);
+    acc.add(b.x * a.x + b.y * a.y);
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = acc.value();
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// out[0] += sum(partial[0..m)) in a fixed order (deterministic).
+__global__ void k_sum_into(const float* __restrict__ partial, int m, float* __restrict__ out) {
+  __shared__ float red[256];
+  KSum acc;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) acc.add(partial[i]);
+  red[threadIdx.x] = acc.value();
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] += red[0];
+}
+
+// torch.optim.Adam (amsgrad=False, maximize=False, weight_decay=0) on real
+// views: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+// p -= lr/bc1 * m / (sqrt(v)/sqrt(bc2) + eps), bc_i = 1 - b_i^step.
+// hyper = [lr, step] on the device; pty_adam_tick advances step.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, const float* __restrict__ hyper, float b1, float b2,
+                       float eps) {
+  const float lr = hyper[0], t = hyper[1];
+  const float bc1 = 1.f - powf(b1, t);
+  const float sbc2 = sqrtf(1.f - powf(b2, t));
+  const float step = lr / bc1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    float mi = m[i], vi = v[i];
+    mi = b1 * mi + (1.f - b1) * gi;
+    vi = b2 * vi + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float denom = sqrtf(vi) / sbc2 + eps;
+    p[i] = p[i] - step * (mi / denom);
+  }
+}
+
+// [M, N, F] -> [F, M, N] with ifftshift over (M, N): out[f, k0, k1] =
+// in[(k0 + M/2) % M, (k1 + N/2) % N, f]  (natural FFT order of the shifted
+// measurement, so that fftshift(fft2(.)) compares with it pixel by pixel).
+__global__ void k_stack_prepare(const float* __restrict__ in, int M, int N, int F, float* __restrict__ out) {
+  const int64_t total = (int64_t)M * N * F;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    // t indexes the input contiguously: t = (y*N + x)*F + f
+    const int f = (int)(t % F);
+    const int64_t yx = t / F;
+    const int y = (int)(yx / N), x = (int)(yx % N);
+    const int k0 = (y + M - M / 2) % M;
+    const int k1 = (x + N - N / 2) % N;
+    out[((int64_t)f * M + k0) * N + k1] = in[t];
+  }
+}
+
+__global__ void k_adam_tick(float* hyper) { hyper[1] += 1.f; }
+
+static unsigned grid_for(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace pty
+
+using namespace pty;
+
+extern "C" int pty_abi_version(void) { return PTY_ABI_VERSION; }
+extern "C" const char* pty_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int pty_fill(float* p, int64_t n, float value, hipStream_t stream) {
+  PTY_REQUIRE(n >= 0 && (n == 0 || p), PTY_EINVAL, "fill: null pointer");
+  if (n == 0) return PTY_OK;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, n, value);
+  return check_launch("fill");
+}
+
+extern "C" int pty_residual_fwd(const float* k, const float* x, const float* initial, int64_t n, float* out,
+                                hipStream_t stream) {
+  PTY_REQUIRE(k && x && initial && out && n > 0, PTY_EINVAL, "residual_fwd: bad arguments");
+  hipLaunchKernelGGL(k_residual_fwd, dim3(grid_for(n, 256)), dim3(256), 0, stream, k, (const float2*)x,
+                     (const float2*)initial, n, (float2*)out);
+  return check_launch("residual_fwd");
+}
+
+static int residual_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (int)(b > 1024 ? 1024 : (b < 1 ? 1 : b));
+}
+
+extern "C" size_t pty_residual_workspace(int64_t n) { return sizeof(float) * (size_t)residual_blocks(n); }
+
+extern "C" int pty_residual_bwd(const float* k, const float* x, const float* gout, int64_t n, float* gx,
+                                float* gk, void* work, size_t work_bytes, hipStream_t stream) {
+  PTY_REQUIRE(k && x && gout && gx && gk && n > 0, PTY_EINVAL, "residual_bwd: bad arguments");
+  const int blocks = residual_blocks(n);
+  PTY_REQUIRE(work && work_bytes >= pty_residual_workspace(n), PTY_EWORK, "residual_bwd: workspace too small");
+  float* partial = (float*)work;
+  hipLaunchKernelGGL(k_residual_bwd, dim3(blocks), dim3(256), 0, stream, k, (const float2*)x,
+                     (const float2*)gout, n, (float2*)gx, partial);
+  hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, stream, partial, blocks, gk);
+  return check_launch("residual_bwd");
+}
+
+extern "C" int pty_adam(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                        const float* hyper, float beta1, float beta2, float eps, hipStream_t stream) {
+  PTY_REQUIRE(param && grad && exp_avg && exp_avg_sq && hyper && n >= 0, PTY_EINVAL, "adam: bad arguments");
+  if (n == 0) return PTY_OK;
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n, 256)), dim3(256), 0, stream, param, grad, exp_avg, exp_avg_sq, n,
+                     hyper, beta1, beta2, eps);
+  return check_launch("adam");
+}
+
+extern "C" int pty_adam_tick(float* hyper, hipStream_t stream) {
+  PTY_REQUIRE(hyper, PTY_EINVAL, "adam_tick: null pointer");
+  hipLaunchKernelGGL(k_adam_tick, dim3(1), dim3(1), 0, stream, hyper);
+  return check_launch("adam_tick");
+}
+
+extern "C" int pty_stack_prepare(const float* stack_mnf, int32_t M, int32_t N, int32_t frames, float* out,
+                                 hipStream_t stream) {
+  PTY_REQUIRE(stack_mnf && out && M > 0 && N > 0 && frames > 0, PTY_EINVAL, "stack_prepare: bad arguments");
+  const int64_t total = (int64_t)M * N * frames;
+  hipLaunchKernelGGL(k_stack_prepare, dim3(grid_for(total, 256)), dim3(256), 0, stream, stack_mnf, M, N, frames, out);
+  return check_launch("stack_prepare");
+}

@@ -1,0 +1,420 @@
+// Far-field forward model, the three losses and their adjoint for one batch
+// of scan positions (reference: Recovery.py:287-307; same math as
+// utils/optics.py:102-112).
+//
+// For position p = batch_idx[b] with window (tlY, tlX):
+//   psi = probe_n * object[tlY:tlY+M, tlX:tlX+N]
+//   Psi = fft2(psi)   (natural order; the measured stack is stored
+//                      ifftshifted, so fftshift never has to move data)
+//   I = |Psi|^2,  loss_b = mean over pixels of l(I, meas_p)
+// Adjoint of sum_b loss_b:
+//   dPsi = 2 dI Psi,  dpsi = conj-DFT(dPsi) (= M*N*ifft2),
+//   dprobe += dpsi * conj(patch),  dobject[window] += dpsi * conj(probe_n)
+// Pass structure (LDS Stockham FFTs):
+//   F1 rows  : Psi1[b, r, :] = FFT_N(probe_n[r,:] * patch[r,:])
+//   F2 cols  : FFT_M; smooth-L1 / Poisson: loss, dI, dPsi, IFFT_M in place
+//              FD: store Psi and I, then FD1 (loss, gx, gy), FD2 (dPsi), F2i (IFFT_M)
+//   F3 rows  : per row r, for every b: IFFT_N, dprobe row accumulation and
+//              atomic scatter of dpatch into dobject
+#include "common.h"
+
+namespace pty {
+
+struct FarArgs {
+  int M, N, lgM, lgN, H, W, B, F, loss;
+  const float2* obj;
+  const float2* probe;
+  const float* stack;     // [F, M, N] natural order
+  const int* bidx;        // [B]
+  const int* tl;          // [F, 2] (tlY, tlX)
+  float2* psi;            // [B, M, N] workspace
+  float* inten;           // [B, M, N] (FD / intensity output)
+  float* gx;              // [B, M, N] (FD)
+  float* gy;              // [B, M, N] (FD)
+  float* partial;         // [B, P]
+  int P;                  // partial sums per pattern
+  float* loss_out;        // [B]
+  float2* dobj;
+  float2* dprobe;
+};
+
+// Window of batch entry b; returns false when the entry is invalid (index or
+// window out of range): such entries contribute nothing.
+__device__ __forceinline__ bool window_of(const FarArgs& a, int b, int& ty, int& tx, int& p) {
+  p = a.bidx[b];
+  if (p < 0 || p >= a.F) return false;
+  ty = a.tl[2 * p];
+  tx = a.tl[2 * p + 1];
+  return ty >= 0 && tx >= 0 && ty + a.M <= a.H && tx + a.N <= a.W;
+}
+
+// F1: rows.  lines rows of the flattened (b, r) index per workgroup.
+__global__ __launch_bounds__(256) void k_far_rows_fwd(FarArgs a, int lines) {
+  extern __shared__ float2 smem[];
+  const int N = a.N;
+  float2* tw = smem;
+  float2* x = tw + N;
+  float2* y = x + lines * N;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  fill_twiddles(tw, N, tid, nt);
+  const int row0 = blockIdx.x * lines;
+  const int total_rows = a.B * a.M;
+  for (int t = tid; t < lines * N; t += nt) {
+    const int li = t / N, c = t - li * N;
+    const int row = row0 + li;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < total_rows) {
+      const int b = row / a.M, r = row - b * a.M;
+      int ty, tx, p;
+      if (window_of(a, b, ty, tx, p))
+        v = cmul(a.probe[r * N + c], a.obj[(int64_t)(ty + r) * a.W + tx + c]);
+    }
+    x[t] = v;
+  }
+  __syncthreads();
+  float2* res = fft_lines<-1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
+  for (int t = tid; t < lines * N; t += nt) {
+    const int row = row0 + t / N;
+    if (row < total_rows) a.psi[(int64_t)row * N + (t % N)] = res[t];
+  }
+}
+
+__device__ __forceinline__ float smooth_l1(float d, float& g) {
+  const float ad = fabsf(d);
+  if (ad < 1.f) { g = d; return 0.5f * d * d; }
+  g = d > 0.f ? 1.f : -1.f;
+  return ad - 0.5f;
+}
+
+template <int NT>
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  float r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// F2: columns of pattern b = blockIdx.y, C columns per workgroup.
+// MODE 0: pointwise loss (smooth-L1 / Poisson) fused with dPsi and IFFT_M.
+// MODE 1: forward only, store natural-order Psi (in place) and I.
+// MODE 2: inverse only (FD path): IFFT_M of the stored dPsi, in place.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_far_cols(FarArgs a, int C) {
+  extern __shared__ float2 smem[];
+  __shared__ float red[256];
+  const int M = a.M, N = a.N, S = M + 1;
+  float2* tw = smem;
+  float2* x = tw + M;
+  float2* y = x + C * S;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  fill_twiddles(tw, M, tid, nt);
+  const int b = blockIdx.y;
+  const int c0 = blockIdx.x * C;
+  int ty, tx, p;
+  const bool valid = window_of(a, b, ty, tx, p);
+  float2* base = a.psi + (int64_t)b * M * N;
+  for (int t = tid; t < M * C; t += nt) {
+    const int r = t / C, c = t - r * C;
+    x[c * S + r] = base[(int64_t)r * N + c0 + c];
+  }
+  __syncthreads();
+  if (MODE == 2) {
+    float2* res = fft_lines<+1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
+    for (int t = tid; t < M * C; t += nt) {
+      const int r = t / C, c = t - r * C;
+      base[(int64_t)r * N + c0 + c] = res[c * S + r];
+    }
+    return;
+  }
+  float2* f = fft_lines<-1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
+  if (MODE == 1) {
+    for (int t = tid; t < M * C; t += nt) {
+      const int r = t / C, c = t - r * C;
+      const float2 v = f[c * S + r];
+      base[(int64_t)r * N + c0 + c] = v;
+      a.inten[(int64_t)b * M * N + (int64_t)r * N + c0 + c] = v.x * v.x + v.y * v.y;
+    }
+    return;
+  }
+  // MODE 0
+  const float inv_mn = 1.f / ((float)M * (float)N);
+  const float* meas = a.stack + (int64_t)(valid ? p : 0) * M * N;
+  KSum acc;
+  for (int t = tid; t < M * C; t += nt) {
+    const int r = t / C, c = t - r * C;   // consecutive threads -> consecutive columns
+    float2 v = f[c * S + r];
+    float2 gv = make_float2(0.f, 0.f);
+    if (valid) {
+      const float I = v.x * v.x + v.y * v.y;
+      const float m = meas[(int64_t)r * N + c0 + c];
+      float g;
+      if (a.loss == PTY_LOSS_SMOOTH_L1) {
+        acc.add(smooth_l1(I - m, g));
+      } else {  // Poisson
+        const float e = I + 1e-8f;
+        acc.add(I - m * logf(e));
+        g = 1.f - m / e;
+      }
+      gv = cscale(v, 2.f * g * inv_mn);
+    }
+    f[c * S + r] = gv;
+  }
+  __syncthreads();
+  const float s = block_sum<256>(acc.value(), red);
+  if (tid == 0) a.partial[b * a.P + blockIdx.x] = s;
+  float2* g2 = (f == x) ? y : x;
+  float2* res = fft_lines<+1>(f, g2, M, a.lgM, C, S, tw, 1, tid, nt);
+  for (int t = tid; t < M * C; t += nt) {
+    const int r = t / C, c = t - r * C;
+    base[(int64_t)r * N + c0 + c] = res[c * S + r];
+  }
+}
+
+// FD loss (Recovery.py:296-302) on natural-order pixels: the circular
+// neighbours of the shifted array are the circular neighbours here too.
+// FD1: dW = sqrt(m+eps) - sqrt(I+eps); Rx = dW[., x+1] - dW; Ry = dW[y+1, .] - dW;
+//      G = sqrt(Rx^2 + Ry^2 + eps); loss += G; gx = Rx/(G MN); gy = Ry/(G MN).
+__global__ __launch_bounds__(256) void k_far_fd1(FarArgs a) {
+  __shared__ float red[256];
+  const int M = a.M, N = a.N, b = blockIdx.y;
+  int ty, tx, p;
+  const bool valid = window_of(a, b, ty, tx, p);
+  const float eps = 1e-8f;
+  const float inv_mn = 1.f / ((float)M * (float)N);
+  const float* I = a.inten + (int64_t)b * M * N;
+  const float* meas = a.stack + (int64_t)(valid ? p : 0) * M * N;
+  const int per = (M * N + a.P - 1) / a.P;
+  const int q0 = blockIdx.x * per, q1 = min(M * N, q0 + per);
+  KSum acc;
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int r = q / N, c = q - r * N;
+    float gxv = 0.f, gyv = 0.f;
+    if (valid) {
+      const int qx = r * N + ((c + 1) & (N - 1));
+      const int qy = ((r + 1) & (M - 1)) * N + c;
+      const float w0 = sqrtf(meas[q] + eps) - sqrtf(I[q] + eps);
+      const float wx = sqrtf(meas[qx] + eps) - sqrtf(I[qx] + eps);
+      const float wy = sqrtf(meas[qy] + eps) - sqrtf(I[qy] + eps);
+      const float rx = wx - w0, ry = wy - w0;
+      const float G = sqrtf(rx * rx + ry * ry + eps);
+      acc.add(G);
+      gxv = rx / G * inv_mn;
+      gyv = ry / G * inv_mn;
+    }
+    a.gx[(int64_t)b * M * N + q] = gxv;
+    a.gy[(int64_t)b * M * N + q] = gyv;
+  }
+  const float s = block_sum<256>(acc.value(), red);
+  if (threadIdx.x == 0) a.partial[b * a.P + blockIdx.x] = s;
+}
+
+// FD2: d(dW)_q = -gx_q - gy_q + gx_{q-x} + gy_{q-y}; dI = -d(dW) / (2 sqrt(I+eps));
+//      dPsi = 2 dI Psi (in place over Psi).
+__global__ __launch_bounds__(256) void k_far_fd2(FarArgs a) {
+  const int M = a.M, N = a.N, b = blockIdx.y;
+  const float eps = 1e-8f;
+  const int64_t off = (int64_t)b * M * N;
+  const int per = (M * N + a.P - 1) / a.P;
+  const int q0 = blockIdx.x * per, q1 = min(M * N, q0 + per);
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int r = q / N, c = q - r * N;
+    const int qxm = r * N + ((c - 1) & (N - 1));
+    const int qym = ((r - 1) & (M - 1)) * N + c;
+    const float ddw = -a.gx[off + q] - a.gy[off + q] + a.gx[off + qxm] + a.gy[off + qym];
+    const float dI = -ddw / (2.f * sqrtf(a.inten[off + q] + eps));
+    a.psi[off + q] = cscale(a.psi[off + q], 2.f * dI);
+  }
+}
+
+// per-pattern loss = sum of partials / (M N)
+__global__ void k_far_loss_final(FarArgs a) {
+  const int b = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  KSum acc;
+  for (int i = 0; i < a.P; ++i) acc.add(a.partial[b * a.P + i]);
+  a.loss_out[b] = acc.value() / ((float)a.M * (float)a.N);
+}
+
+// F3: rows, inverse.  One wave per workgroup, `lines` rows r; loops over the
+// batch so that each dprobe row has a single owner.
+__global__ __launch_bounds__(64) void k_far_rows_bwd(FarArgs a, int lines) {
+  extern __shared__ float2 smem[];
+  const int N = a.N;
+  float2* tw = smem;
+  float2* x = tw + N;
+  float2* y = x + lines * N;
+  float2* acc = y + lines * N;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  fill_twiddles(tw, N, tid, nt);
+  const int r0 = blockIdx.x * lines;
+  for (int t = tid; t < lines * N; t += nt) acc[t] = make_float2(0.f, 0.f);
+  for (int b = 0; b < a.B; ++b) {
+    int ty, tx, p;
+    const bool valid = window_of(a, b, ty, tx, p);  // uniform across the block
+    if (!valid) continue;
+    __syncthreads();
+    for (int t = tid; t < lines * N; t += nt) {
+      const int r = r0 + t / N, c = t % N;
+      x[t] = a.psi[((int64_t)b * a.M + r) * N + c];
+    }
+    __syncthreads();
+    float2* res = fft_lines<+1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
+    for (int t = tid; t < lines * N; t += nt) {
+      const int r = r0 + t / N, c = t % N;
+      const float2 dpsi = res[t];
+      const int64_t oi = (int64_t)(ty + r) * a.W + tx + c;
+      const float2 patch = a.obj[oi];
+      const float2 pr = a.probe[r * N + c];
+      acc[t] = cadd(acc[t], cmulc(dpsi, patch));
+      const float2 dp = cmulc(dpsi, pr);
+      atomic_add_f32(&a.dobj[oi].x, dp.x);
+      atomic_add_f32(&a.dobj[oi].y, dp.y);
+    }
+  }
+  for (int t = tid; t < lines * N; t += nt) {
+    const int r = r0 + t / N, c = t % N;
+    a.dprobe[r * N + c] = acc[t];
+  }
+}
+
+static int far_col_tile(int M, int N) {
+  int C = 16;
+  while (C > 1 && (size_t)(M + 2 * C * (M + 1)) * sizeof(float2) > 65536) C >>= 1;
+  while (C > 1 && N % C) C >>= 1;
+  return C;
+}
+
+static int check_far(const PtyFar* d) {
+  PTY_REQUIRE(d, PTY_EINVAL, "farfield: descriptor is NULL");
+  PTY_REQUIRE(is_pow2(d->M) && is_pow2(d->N) && d->M >= 16 && d->N >= 16 && d->M <= 1024 && d->N <= 1024,
+              PTY_EUNSUP, "farfield: probe %dx%d must be powers of two in [16, 1024]", d->M, d->N);
+  PTY_REQUIRE(d->canvas_h >= d->M && d->canvas_w >= d->N, PTY_EINVAL, "farfield: canvas smaller than probe");
+  PTY_REQUIRE(d->n_batch >= 1 && d->n_frames >= 1, PTY_EINVAL, "farfield: empty batch or stack");
+  PTY_REQUIRE(d->loss_type >= 0 && d->loss_type <= 2, PTY_EINVAL, "farfield: bad loss_type %d", d->loss_type);
+  return PTY_OK;
+}
+
+struct FarLayout {
+  size_t psi, inten, gx, gy, partial, total;
+  int P, C;
+};
+
+static FarLayout far_layout(const PtyFar* d) {
+  FarLayout L;
+  const size_t px = (size_t)d->n_batch * d->M * d->N;
+  L.C = far_col_tile(d->M, d->N);
+  L.P = d->N / L.C;
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+  L.psi = take(px * sizeof(float2));
+  const bool fd = d->loss_type == PTY_LOSS_FD;
+  L.inten = take(px * sizeof(float));
+  L.gx = fd ? take(px * sizeof(float)) : 0;
+  L.gy = fd ? take(px * sizeof(float)) : 0;
+  L.partial = take((size_t)d->n_batch * L.P * sizeof(float));
+  L.total = off;
+  return L;
+}
+
+static FarArgs far_args(const PtyFar* d, const FarLayout& L, const float* object, const float* probe_n,
+                        const float* stack, const int32_t* bidx, const int32_t* tl, float* loss, float* dobject,
+                        float* dprobe, void* work) {
+  FarArgs a;
+  a.M = d->M; a.N = d->N; a.lgM = ilog2(d->M); a.lgN = ilog2(d->N);
+  a.H = d->canvas_h; a.W = d->canvas_w; a.B = d->n_batch; a.F = d->n_frames; a.loss = d->loss_type;
+  a.obj = (const float2*)object; a.probe = (const float2*)probe_n; a.stack = stack;
+  a.bidx = bidx; a.tl = tl;
+  char* w = (char*)work;
+  a.psi = (float2*)(w + L.psi);
+  a.inten = (float*)(w + L.inten);
+  a.gx = d->loss_type == PTY_LOSS_FD ? (float*)(w + L.gx) : nullptr;
+  a.gy = d->loss_type == PTY_LOSS_FD ? (float*)(w + L.gy) : nullptr;
+  a.partial = (float*)(w + L.partial);
+  a.P = L.P;
+  a.loss_out = loss; a.dobj = (float2*)dobject; a.dprobe = (float2*)dprobe;
+  return a;
+}
+
+static void launch_rows_fwd(const FarArgs& a, hipStream_t s) {
+  const int lines = a.N >= 1024 ? 1 : 1024 / a.N;
+  const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
+  const int blocks = (a.B * a.M + lines - 1) / lines;
+  hipLaunchKernelGGL(k_far_rows_fwd, dim3(blocks), dim3(256), shm, s, a, lines);
+}
+
+template <int MODE>
+static void launch_cols(const FarArgs& a, int C, hipStream_t s) {
+  const size_t shm = sizeof(float2) * (size_t)(a.M + 2 * C * (a.M + 1));
+  hipLaunchKernelGGL(k_far_cols<MODE>, dim3(a.N / C, a.B), dim3(256), shm, s, a, C);
+}
+
+}  // namespace pty
+
+using namespace pty;
+
+extern "C" size_t pty_farfield_workspace(const PtyFar* d) {
+  if (check_far(d)) return 0;
+  return far_layout(d).total;
+}
+
+extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* probe_n, const float* stack,
+                                    const int32_t* batch_idx, const int32_t* tl, float* loss, float* dobject,
+                                    float* dprobe, void* work, size_t work_bytes, hipStream_t stream) {
+  int rc = check_far(d);
+  if (rc) return rc;
+  PTY_REQUIRE(object && probe_n && stack && batch_idx && tl && loss && dobject && dprobe, PTY_EINVAL,
+              "farfield_fwd_bwd: null pointer");
+  const FarLayout L = far_layout(d);
+  PTY_REQUIRE(work && work_bytes >= L.total, PTY_EWORK, "farfield_fwd_bwd: workspace %zu < %zu", work_bytes, L.total);
+  FarArgs a = far_args(d, L, object, probe_n, stack, batch_idx, tl, loss, dobject, dprobe, work);
+  launch_rows_fwd(a, stream);
+  if (d->loss_type == PTY_LOSS_FD) {
+    launch_cols<1>(a, L.C, stream);
+    hipLaunchKernelGGL(k_far_fd1, dim3(a.P, a.B), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_far_fd2, dim3(a.P, a.B), dim3(256), 0, stream, a);
+    launch_cols<2>(a, L.C, stream);
+  } else {
+    launch_cols<0>(a, L.C, stream);
+  }
+  hipLaunchKernelGGL(k_far_loss_final, dim3(a.B), dim3(64), 0, stream, a);
+  const int lines = a.N >= 256 ? 1 : 256 / a.N;
+  const size_t shm = sizeof(float2) * (size_t)(a.N + 3 * lines * a.N);
+  hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines), dim3(64), shm, stream, a, lines);
+  return check_launch("farfield_fwd_bwd");
+}
+
+// Intensity output in the reference's fftshifted order.
+__global__ void k_far_shift_out(FarArgs a, float* out) {
+  const int M = a.M, N = a.N;
+  const int64_t total = (int64_t)a.B * M * N;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(t / ((int64_t)M * N));
+    const int q = (int)(t - (int64_t)b * M * N);
+    const int k0 = q / N, k1 = q - k0 * N;
+    const int y = (k0 + M / 2) & (M - 1), x = (k1 + N / 2) & (N - 1);
+    out[(int64_t)b * M * N + (int64_t)y * N + x] = a.inten[t];
+  }
+}
+
+extern "C" int pty_farfield_intensity(const PtyFar* d, const float* object, const float* probe_n,
+                                      const int32_t* batch_idx, const int32_t* tl, float* intensity, void* work,
+                                      size_t work_bytes, hipStream_t stream) {
+  int rc = check_far(d);
+  if (rc) return rc;
+  PTY_REQUIRE(object && probe_n && batch_idx && tl && intensity, PTY_EINVAL, "farfield_intensity: null pointer");
+  const FarLayout L = far_layout(d);
+  PTY_REQUIRE(work && work_bytes >= L.total, PTY_EWORK, "farfield_intensity: workspace too small");
+  FarArgs a = far_args(d, L, object, probe_n, nullptr, batch_idx, tl, nullptr, nullptr, nullptr, work);
+  launch_rows_fwd(a, stream);
+  launch_cols<1>(a, L.C, stream);
+  const int64_t total = (int64_t)a.B * a.M * a.N;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_far_shift_out, dim3((unsigned)blocks), dim3(256), 0, stream, a, intensity);
+  return check_launch("farfield_intensity");
+}

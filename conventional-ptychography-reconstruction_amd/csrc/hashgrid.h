@@ -1,0 +1,112 @@
+// Hash-grid index and interpolation math shared by the standalone encoder
+// kernels and the fused INR kernels.
+//
+// Restates the multiresolution hash encoding that the reference obtains from
+// tiny-cuda-nn's HashGrid (models/complex_inr.py:130-145): per level l,
+// pos = x*scale_l + 0.5 (two separately rounded f32 ops), g = floor(pos),
+// frac = pos - g; corner c = (c&1, c>>1); dense index (cx + cy*res) reduced
+// modulo the level size (the edge corner cx = res wraps into the next row),
+// hashed index (cx ^ cy*2654435761) mod 2^32 mod 2^log2T; bilinear weights
+// w_c = (c&1 ? fx : 1-fx) * (c&2 ? fy : 1-fy).
+#pragma once
+#include "common.h"
+
+namespace pty {
+
+int validate_grid(const PtyGrid* g);
+
+struct LevelPos {
+  uint32_t gx, gy;
+  float fx, fy;
+};
+
+__device__ __forceinline__ LevelPos level_pos(const PtyGrid& g, int l, float x0, float x1) {
+  const float s = g.scale[l];
+  float p0 = __fadd_rn(__fmul_rn(x0, s), 0.5f);
+  float p1 = __fadd_rn(__fmul_rn(x1, s), 0.5f);
+  float f0 = floorf(p0), f1 = floorf(p1);
+  LevelPos r;
+  r.gx = (uint32_t)(int)f0;
+  r.gy = (uint32_t)(int)f1;
+  r.fx = p0 - f0;
+  r.fy = p1 - f1;
+  return r;
+}
+
+// Entry index (level offset included) of corner (cx, cy) at level l.
+__device__ __forceinline__ uint32_t level_index(const PtyGrid& g, int l, uint32_t cx, uint32_t cy) {
+  const uint32_t size = g.size[l];
+  uint32_t local;
+  if ((g.dense_mask >> l) & 1u) {
+    // For inputs in [0, 1], cx, cy <= res, so cx + cy*res < 2*size and one
+    // subtraction implements the modulo; other inputs take the full modulo.
+    local = cx + cy * g.res[l];
+    if (local >= size) local = (local < 2u * size) ? local - size : local % size;
+  } else {
+    // hashed levels have size == 2^log2T
+    local = (cx ^ (cy * 2654435761u)) & (size - 1u);
+  }
+  return g.offset[l] + local;
+}
+
+__device__ __forceinline__ void corner_weights(const LevelPos& p, float w[4]) {
+  const float ax = 1.f - p.fx, ay = 1.f - p.fy;
+  w[0] = ax * ay;
+  w[1] = p.fx * ay;
+  w[2] = ax * p.fy;
+  w[3] = p.fx * p.fy;
+}
+
+__device__ __forceinline__ void corner_indices(const PtyGrid& g, int l, const LevelPos& p, uint32_t idx[4]) {
+  idx[0] = level_index(g, l, p.gx, p.gy);
+  idx[1] = level_index(g, l, p.gx + 1u, p.gy);
+  idx[2] = level_index(g, l, p.gx, p.gy + 1u);
+  idx[3] = level_index(g, l, p.gx + 1u, p.gy + 1u);
+}
+
+// Interpolated (f0, f1) of one level.
+__device__ __forceinline__ float2 level_feature(const PtyGrid& g, int l, float x0, float x1,
+                                                const float2* __restrict__ table) {
+  LevelPos p = level_pos(g, l, x0, x1);
+  uint32_t idx[4];
+  float w[4];
+  corner_indices(g, l, p, idx);
+  corner_weights(p, w);
+  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 t = table[idx[c]];
+    acc.x += w[c] * t.x;
+    acc.y += w[c] * t.y;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void level_scatter(const PtyGrid& g, int l, float x0, float x1, float2 dy,
+                                              float* __restrict__ dtable) {
+  LevelPos p = level_pos(g, l, x0, x1);
+  uint32_t idx[4];
+  float w[4];
+  corner_indices(g, l, p, idx);
+  corner_weights(p, w);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    atomic_add_f32(dtable + 2 * idx[c], w[c] * dy.x);
+    atomic_add_f32(dtable + 2 * idx[c] + 1, w[c] * dy.y);
+  }
+}
+
+// Input coordinates of point n (explicit list or the regular grid).
+__device__ __forceinline__ void point_coords(const PtyPoints& pts, int64_t n, float& x0, float& x1) {
+  if (pts.coords) {
+    x0 = pts.coords[2 * n];
+    x1 = pts.coords[2 * n + 1];
+  } else {
+    const int i = (int)(n / pts.n_axis0);
+    const int j = (int)(n - (int64_t)i * pts.n_axis0);
+    x0 = pts.axis0[j];
+    x1 = pts.axis1[i];
+  }
+}
+
+}  // namespace pty

@@ -1,0 +1,640 @@
+// Fused hash-grid encode + complex-exponential MLP on MFMA (gfx950).
+//
+// Reference: ComplexINRModel2D.forward -> HashGrid (models/complex_inr.py:279,
+// 294), ComplexMLP.net (models/complex_inr.py:57-78, 313):
+//   u0 = W0 h + b0            z0 = exp(i w0 u0)                 (real first layer)
+//   v_k = A_k z_{k-1} + c_k   z_k = exp(i w_k v_k) = e^{-w Im v}(cos w Re v, sin w Re v)
+//   o = wo . z_H + bo
+//
+// Layout.  A wavefront owns a tile of 32 points; lane l = (j = l&31, h = l>>5)
+// holds point j.  Every layer is computed transposed, Y^T = W X^T, with
+// v_mfma_f32_32x32x2f32: the points are the MFMA N dimension, so the output
+// of one layer (C layout: lane (j,h) holds rows crow(v,h) = 8(v>>2)+4h+(v&3),
+// v < 16, of column j) is already the B operand of the next layer once the
+// K index of the next weight matrix is permuted to match (kappa maps below).
+// Complex layers use the real 64x64 form [[Ar,-Ai],[Ai,Ar]] with two 32-row
+// M blocks (Re rows, Im rows), so each lane gets Re and Im of the same 16
+// units and applies exp(i w v) in registers.  No activation leaves registers
+// in the forward pass.
+//
+// The backward pass recomputes the forward for the tile, then runs the chain
+// rule layer by layer (PyTorch's complex-gradient convention g = dL/dRe +
+// i dL/dIm):
+//   g_z_H = conj(wo) g_o,  g_wo += g_o conj(z_H),  g_bo += g_o
+//   g_w = g_z conj(z),  g_v = -i w g_w,  g_w_k += Im(g_w conj(v))
+//   g_W_real += g_v x^T (MFMA with points as K; operands staged through LDS,
+//   o-major with a padded stride), g_c += sum g_v, g_x = W_real^T g_v (MFMA)
+//   layer 0: g_u0 = w0 Im(g_w0), g_w0 += u0 Im(g_w0), g_W0, g_b0, g_h = W0^T g_u0
+//   hash grid: dtable[corner] += w_c * g_h (atomics)
+#include "hashgrid.h"
+
+namespace pty {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int WS = 33;  // padded LDS row stride (floats)
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+
+// C-layout row of value v for lane half h
+__device__ __forceinline__ int crow(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
+// feature held by lane half h as value s (layer-0 B operand): level 2(s>>1)+h, feature s&1
+__device__ __forceinline__ int kappa0(int s, int h) { return 4 * (s >> 1) + 2 * h + (s & 1); }
+// inverse of crow, composed with kappa0: feature index of M row rho in g_h = W0^T g_u0
+__device__ __forceinline__ int phi0(int rho) { return kappa0(4 * (rho >> 3) + (rho & 3), (rho >> 2) & 1); }
+
+__device__ __forceinline__ v16f mfma(float a, float b, v16f c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// exp(i*w*x) for real x: (cos wx, sin wx).  Range-reduced to revolutions in
+// [-0.5, 0.5] before the hardware sin/cos.
+__device__ __forceinline__ void cis(float theta, float& c, float& s) {
+  float r = theta * 0.15915494309189535f;
+  r = r - rintf(r);
+  s = __builtin_amdgcn_sinf(r);
+  c = __builtin_amdgcn_cosf(r);
+}
+
+__device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
+template <int H>
+struct MlpLds {
+  float w0[32 * WS];   // W0[o][f], f zero-padded to 32
+  float b0[32];
+  float ar[H][32 * WS];
+  float ai[H][32 * WS];
+  float nai[H][32 * WS];
+  float cr[H][32];
+  float ci[H][32];
+  float om[H + 1];
+  float wor[32], woi[32];
+  float bo[2];
+};
+
+template <int H>
+__device__ void load_weights(MlpLds<H>& s, const PtyMlp& m) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int t = tid; t < 32 * 32; t += nt) {
+    const int o = t >> 5, f = t & 31;
+    s.w0[o * WS + f] = f < m.n_in ? m.w0[o * m.n_in + f] : 0.f;
+    for (int k = 0; k < H; ++k) {
+      const float2 a = ((const float2*)m.wh[k])[t];
+      s.ar[k][o * WS + f] = a.x;
+      s.ai[k][o * WS + f] = a.y;
+      s.nai[k][o * WS + f] = -a.y;
+    }
+  }
+  for (int t = tid; t < 32; t += nt) {
+    s.b0[t] = m.b0[t];
+    for (int k = 0; k < H; ++k) {
+      s.cr[k][t] = m.bh[k][2 * t];
+      s.ci[k][t] = m.bh[k][2 * t + 1];
+    }
+    s.wor[t] = m.wo[2 * t];
+    s.woi[t] = m.wo[2 * t + 1];
+  }
+  if (tid <= H) s.om[tid] = *m.omega[tid];
+  if (tid == 0) { s.bo[0] = m.bo[0]; s.bo[1] = m.bo[1]; }
+}
+
+// Features of the lane's 8 levels: feat[s] = feature kappa0(s, h).
+__device__ __forceinline__ void encode_lane(const PtyGrid& g, float x0, float x1, int h,
+                                            const float2* __restrict__ table, float feat[16]) {
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int l = 2 * m + h;
+    float2 f = make_float2(0.f, 0.f);
+    if (l < g.n_levels) f = level_feature(g, l, x0, x1, table);
+    feat[2 * m] = f.x;
+    feat[2 * m + 1] = f.y;
+  }
+}
+
+// Layer 0: u0[v] (unit crow(v,h)) = b0 + W0 feat.
+template <int H>
+__device__ __forceinline__ v16f layer0(const MlpLds<H>& s, const float feat[16], int j, int h) {
+  v16f acc;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc[v] = s.b0[crow(v, h)];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) acc = mfma(s.w0[j * WS + kappa0(st, h)], feat[st], acc);
+  return acc;
+}
+
+// Hidden layer k: (vr, vi)[v] for unit crow(v,h), input x[s] in kappa1 order
+// (s < 16: Re z of unit crow(s,h); s >= 16: Im z of unit crow(s-16,h)).
+template <int H>
+__device__ __forceinline__ void hidden_fwd(const MlpLds<H>& s, int k, const float zr[16], const float zi[16], int j,
+                                           int h, v16f& vr, v16f& vi) {
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    vr[v] = s.cr[k][crow(v, h)];
+    vi[v] = s.ci[k][crow(v, h)];
+  }
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int u = crow(st, h);
+    vr = mfma(s.ar[k][j * WS + u], zr[st], vr);
+    vi = mfma(s.ai[k][j * WS + u], zr[st], vi);
+  }
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int u = crow(st, h);
+    vr = mfma(s.nai[k][j * WS + u], zi[st], vr);
+    vi = mfma(s.ar[k][j * WS + u], zi[st], vi);
+  }
+}
+
+__device__ __forceinline__ void gabor_act(float w, const v16f& vr, const v16f& vi, float zr[16], float zi[16]) {
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    float c, sn;
+    cis(w * vr[v], c, sn);
+    const float mag = exp_f(-w * vi[v]);
+    zr[v] = mag * c;
+    zi[v] = mag * sn;
+  }
+}
+
+// ============================================================== forward
+template <int H>
+__global__ __launch_bounds__(kThreads) void k_inr_fwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
+                                                      PtyPoints pts, float2* __restrict__ out) {
+  __shared__ MlpLds<H> s;
+  load_weights<H>(s, m);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  const int64_t ntiles = (pts.n + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+    const int64_t p = tile * 32 + j;
+    const bool valid = p < pts.n;
+    float x0 = 0.5f, x1 = 0.5f;
+    if (valid) point_coords(pts, p, x0, x1);
+    float feat[16];
+    encode_lane(g, x0, x1, h, table, feat);
+    v16f u0 = layer0<H>(s, feat, j, h);
+    float zr[16], zi[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      v16f vr, vi;
+      hidden_fwd<H>(s, k, zr, zi, j, h, vr, vi);
+      gabor_act(s.om[k + 1], vr, vi, zr, zi);
+    }
+    float orr = 0.f, oi = 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int u = crow(v, h);
+      orr += s.wor[u] * zr[v] - s.woi[u] * zi[v];
+      oi += s.wor[u] * zi[v] + s.woi[u] * zr[v];
+    }
+    orr += __shfl_xor(orr, 32);
+    oi += __shfl_xor(oi, 32);
+    if (valid && h == 0) out[p] = make_float2(orr + s.bo[0], oi + s.bo[1]);
+  }
+}
+
+// ComplexMLP.net over precomputed features [n, n_in] (models/complex_inr.py:313).
+template <int H>
+__global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __restrict__ feats, int64_t n,
+                                                      float2* __restrict__ out) {
+  __shared__ MlpLds<H> s;
+  load_weights<H>(s, m);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  const int64_t ntiles = (n + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+    const int64_t p = tile * 32 + j;
+    const bool valid = p < n;
+    float feat[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int f = kappa0(v, h);
+      feat[v] = (valid && f < m.n_in) ? feats[p * m.n_in + f] : 0.f;
+    }
+    v16f u0 = layer0<H>(s, feat, j, h);
+    float zr[16], zi[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      v16f vr, vi;
+      hidden_fwd<H>(s, k, zr, zi, j, h, vr, vi);
+      gabor_act(s.om[k + 1], vr, vi, zr, zi);
+    }
+    float orr = 0.f, oi = 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int u = crow(v, h);
+      orr += s.wor[u] * zr[v] - s.woi[u] * zi[v];
+      oi += s.wor[u] * zi[v] + s.woi[u] * zr[v];
+    }
+    orr += __shfl_xor(orr, 32);
+    oi += __shfl_xor(oi, 32);
+    if (valid && h == 0) out[p] = make_float2(orr + s.bo[0], oi + s.bo[1]);
+  }
+}
+
+// ============================================================== backward
+// Flat gradient layout of one workgroup partial (floats):
+//   w0 [32*32] | b0 [32] | per hidden k: A [32*32*2] (re,im) | c [32*2] | omega [H+1] | wo [32*2] | bo [2]
+template <int H>
+struct GradLayout {
+  static constexpr int w0 = 0;
+  static constexpr int b0 = 1024;
+  static constexpr int A(int k) { return 1056 + k * 2112; }
+  static constexpr int c(int k) { return 1056 + k * 2112 + 2048; }
+  static constexpr int om = 1056 + H * 2112;
+  static constexpr int wo = om + H + 1;
+  static constexpr int bo = wo + 64;
+  static constexpr int total = bo + 2;
+};
+
+template <int H>
+struct BwdStage {
+  float X[H][64 * WS];   // z_k (k < H) o-major: rows 0..31 Re, 32..63 Im, column = point
+  float F[32 * WS];      // features, feature-major
+  float G[64 * WS];      // gradient staging
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int H>
+__global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
+                                                      float* __restrict__ dtable, PtyPoints pts,
+                                                      const float2* __restrict__ dout, float* __restrict__ partial) {
+  __shared__ MlpLds<H> s;
+  __shared__ BwdStage<H> st[kWaves];
+  load_weights<H>(s, m);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  BwdStage<H>& S = st[wave];
+
+  v16f dW[H][4];
+  v16f dW0;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    dW0[v] = 0.f;
+#pragma unroll
+    for (int k = 0; k < H; ++k)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) dW[k][b][v] = 0.f;
+  }
+  float acc_c[H][2];
+  float acc_om[H + 1];
+#pragma unroll
+  for (int k = 0; k < H; ++k) acc_c[k][0] = acc_c[k][1] = 0.f;
+#pragma unroll
+  for (int k = 0; k <= H; ++k) acc_om[k] = 0.f;
+  float acc_b0 = 0.f, acc_wo = 0.f, acc_bor = 0.f, acc_boi = 0.f;
+
+  const int64_t ntiles = (pts.n + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+    const int64_t p = tile * 32 + j;
+    const bool valid = p < pts.n;
+    float x0 = 0.5f, x1 = 0.5f;
+    if (valid) point_coords(pts, p, x0, x1);
+    // ---------------- recompute forward
+    float feat[16];
+    encode_lane(g, x0, x1, h, table, feat);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * WS + j] = feat[v];
+    const v16f u0 = layer0<H>(s, feat, j, h);
+    float zr[16], zi[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
+    v16f vr[H], vi[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        S.X[k][crow(v, h) * WS + j] = zr[v];
+        S.X[k][(32 + crow(v, h)) * WS + j] = zi[v];
+      }
+      hidden_fwd<H>(s, k, zr, zi, j, h, vr[k], vi[k]);
+      gabor_act(s.om[k + 1], vr[k], vi[k], zr, zi);
+    }
+    // ---------------- output layer
+    float2 go = make_float2(0.f, 0.f);
+    if (valid) go = dout[p];
+    if (h == 0) { acc_bor += go.x; acc_boi += go.y; }
+    float gzr[16], gzi[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int u = crow(v, h);
+      // g_wo[u] += g_o * conj(z_u); g_z = conj(wo_u) * g_o
+      S.G[u * WS + j] = go.x * zr[v] + go.y * zi[v];
+      S.G[(32 + u) * WS + j] = go.y * zr[v] - go.x * zi[v];
+      gzr[v] = s.wor[u] * go.x + s.woi[u] * go.y;
+      gzi[v] = s.wor[u] * go.y - s.woi[u] * go.x;
+    }
+    wave_sync();
+    {
+      float t = 0.f;
+#pragma unroll 8
+      for (int q = 0; q < 32; ++q) t += S.G[lane * WS + q];
+      acc_wo += t;
+    }
+    wave_sync();
+    // ---------------- hidden layers, last to first
+#pragma unroll
+    for (int k = H - 1; k >= 0; --k) {
+      const float w = s.om[k + 1];
+      float gvr[16], gvi[16];
+      float om_acc = 0.f;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        // g_w = g_z * conj(z)
+        const float gwr = gzr[v] * zr[v] + gzi[v] * zi[v];
+        const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];
+        om_acc += gwi * vr[k][v] - gwr * vi[k][v];
+        gvr[v] = w * gwi;
+        gvi[v] = -w * gwr;
+        const int u = crow(v, h);
+        S.G[u * WS + j] = gvr[v];
+        S.G[(32 + u) * WS + j] = gvi[v];
+      }
+      acc_om[k + 1] += om_acc;
+      wave_sync();
+      // dW_real += G X^T over the 32 points (K = points); X[k] holds the
+      // input of hidden layer k.
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int col = 2 * q + h;
+        const float a0 = S.G[j * WS + col];
+        const float a1 = S.G[(32 + j) * WS + col];
+        const float* Xp = S.X[k];
+        const float b0v = Xp[j * WS + col];
+        const float b1v = Xp[(32 + j) * WS + col];
+        dW[k][0] = mfma(a0, b0v, dW[k][0]);
+        dW[k][1] = mfma(a0, b1v, dW[k][1]);
+        dW[k][2] = mfma(a1, b0v, dW[k][2]);
+        dW[k][3] = mfma(a1, b1v, dW[k][3]);
+        acc_c[k][0] += a0;
+        acc_c[k][1] += a1;
+      }
+      // g_x = W_real^T g_v; also fetch the layer input z (C layout) from LDS
+      v16f gxr, gxi;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) { gxr[v] = 0.f; gxi[v] = 0.f; }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int u = crow(q, h);
+        gxr = mfma(s.ar[k][u * WS + j], gvr[q], gxr);
+        gxi = mfma(s.nai[k][u * WS + j], gvr[q], gxi);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int u = crow(q, h);
+        gxr = mfma(s.ai[k][u * WS + j], gvi[q], gxr);
+        gxi = mfma(s.ar[k][u * WS + j], gvi[q], gxi);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        gzr[v] = gxr[v];
+        gzi[v] = gxi[v];
+        zr[v] = S.X[k][crow(v, h) * WS + j];
+        zi[v] = S.X[k][(32 + crow(v, h)) * WS + j];
+      }
+      wave_sync();
+    }
+    // ---------------- layer 0 (real): z0 = exp(i w0 u0)
+    {
+      const float w = s.om[0];
+      float gu[16];
+      float om_acc = 0.f;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];  // Im(g_z conj(z0))
+        om_acc += u0[v] * gwi;
+        gu[v] = w * gwi;
+        S.G[crow(v, h) * WS + j] = gu[v];
+      }
+      acc_om[0] += om_acc;
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int col = 2 * q + h;
+        const float a = S.G[j * WS + col];
+        dW0 = mfma(a, S.F[j * WS + col], dW0);
+        acc_b0 += a;
+      }
+      v16f gh;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) gh[v] = 0.f;
+      const int fj = phi0(j);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) gh = mfma(s.w0[crow(q, h) * WS + fj], gu[q], gh);
+      wave_sync();
+      if (valid) {
+#pragma unroll
+        for (int mm = 0; mm < 8; ++mm) {
+          const int l = 2 * mm + h;
+          if (l < g.n_levels) level_scatter(g, l, x0, x1, make_float2(gh[2 * mm], gh[2 * mm + 1]), dtable);
+        }
+      }
+    }
+  }
+
+  // ---------------- workgroup reduction into partial[blockIdx.x][GradLayout::total]
+  using GL = GradLayout<H>;
+  static_assert(sizeof(st) / (sizeof(float)) >= (size_t)GL::total, "reduction buffer does not fit");
+  __syncthreads();
+  float* red = &st[0].X[0][0];  // reuse the staging memory as one flat buffer
+  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) red[t] = 0.f;
+  __syncthreads();
+  for (int w = 0; w < kWaves; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int o = crow(v, h);
+        red[GL::w0 + o * 32 + j] += dW0[v];
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          // block (mb, nb): rows 32mb + o, cols 32nb + j
+          const float dar = dW[k][0][v] + dW[k][3][v];
+          const float dai = dW[k][2][v] - dW[k][1][v];
+          red[GL::A(k) + 2 * (o * 32 + j)] += dar;
+          red[GL::A(k) + 2 * (o * 32 + j) + 1] += dai;
+        }
+      }
+      // bias-like sums: lane (j, h) holds partial sums over its half of the points
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const float cr = acc_c[k][0] + __shfl_xor(acc_c[k][0], 32);
+        const float ci = acc_c[k][1] + __shfl_xor(acc_c[k][1], 32);
+        if (h == 0) {
+          red[GL::c(k) + 2 * j] += cr;
+          red[GL::c(k) + 2 * j + 1] += ci;
+        }
+      }
+      const float b0s = acc_b0 + __shfl_xor(acc_b0, 32);
+      if (h == 0) red[GL::b0 + j] += b0s;
+      // acc_wo: lane l holds row l (l < 32: Re of unit l, else Im of unit l-32)
+      red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
+#pragma unroll
+      for (int k = 0; k <= H; ++k) {
+        const float t = wave_sum(acc_om[k]);
+        if (lane == 0) red[GL::om + k] += t;
+      }
+      const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
+      if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
+}
+
+// grad += sum over workgroups of the partials (fixed order).
+template <int H>
+__global__ void k_inr_grad_reduce(const float* __restrict__ partial, int nblocks, PtyMlp gm) {
+  using GL = GradLayout<H>;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= GL::total) return;
+  float acc = 0.f;
+  for (int b = 0; b < nblocks; ++b) acc += partial[(int64_t)b * GL::total + t];
+  if (t < GL::b0) {
+    const int o = t >> 5, f = t & 31;
+    if (f < gm.n_in) gm.w0[o * gm.n_in + f] += acc;
+  } else if (t < GL::A(0)) {
+    gm.b0[t - GL::b0] += acc;
+  } else if (t < GL::om) {
+    const int k = (t - GL::A(0)) / 2112, r = (t - GL::A(0)) % 2112;
+    if (r < 2048) gm.wh[k][r] += acc;
+    else gm.bh[k][r - 2048] += acc;
+  } else if (t < GL::wo) {
+    *gm.omega[t - GL::om] += acc;
+  } else if (t < GL::bo) {
+    gm.wo[t - GL::wo] += acc;
+  } else {
+    gm.bo[t - GL::bo] += acc;
+  }
+}
+
+static int validate_mlp(const PtyMlp* m, bool need_all) {
+  PTY_REQUIRE(m != nullptr, PTY_EINVAL, "mlp descriptor is NULL");
+  PTY_REQUIRE(m->n_in >= 1 && m->n_in <= 32, PTY_EUNSUP, "mlp: n_in %d outside 1..32", m->n_in);
+  PTY_REQUIRE(m->n_hidden >= 1 && m->n_hidden <= 2, PTY_EUNSUP, "mlp: hidden_layers %d outside 1..2", m->n_hidden);
+  PTY_REQUIRE(m->precision == PTY_PREC_FP32, PTY_EUNSUP, "mlp: precision %d not built", m->precision);
+  if (need_all) {
+    PTY_REQUIRE(m->w0 && m->b0 && m->wo && m->bo, PTY_EINVAL, "mlp: null weight pointer");
+    for (int k = 0; k < m->n_hidden; ++k)
+      PTY_REQUIRE(m->wh[k] && m->bh[k], PTY_EINVAL, "mlp: null hidden weight pointer");
+    for (int k = 0; k <= m->n_hidden; ++k) PTY_REQUIRE(m->omega[k], PTY_EINVAL, "mlp: null omega pointer");
+  }
+  return PTY_OK;
+}
+
+static int validate_points(const PtyPoints* p) {
+  PTY_REQUIRE(p != nullptr, PTY_EINVAL, "points descriptor is NULL");
+  PTY_REQUIRE(p->n >= 0, PTY_EINVAL, "points: negative count");
+  if (!p->coords) {
+    PTY_REQUIRE(p->axis0 && p->axis1 && p->n_axis0 > 0 && p->n_axis1 > 0, PTY_EINVAL, "points: no coordinates");
+    PTY_REQUIRE((int64_t)p->n_axis0 * p->n_axis1 == p->n, PTY_EINVAL, "points: n != n_axis0*n_axis1");
+  }
+  return PTY_OK;
+}
+
+static int bwd_blocks(int64_t n) {
+  const int64_t tiles = (n + 31) / 32;
+  int64_t b = (tiles + kWaves - 1) / kWaves;
+  if (b > 512) b = 512;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+static int fwd_blocks(int64_t n) {
+  const int64_t tiles = (n + 31) / 32;
+  int64_t b = (tiles + kWaves - 1) / kWaves;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+}  // namespace pty
+
+using namespace pty;
+
+extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* table, const PtyPoints* pts, float* out,
+                           hipStream_t stream) {
+  int rc = validate_grid(g);
+  if (!rc) rc = validate_mlp(mlp, true);
+  if (!rc) rc = validate_points(pts);
+  if (rc) return rc;
+  PTY_REQUIRE(mlp->n_in == g->n_levels * g->n_features, PTY_EINVAL, "mlp n_in %d != levels*features %d", mlp->n_in,
+              g->n_levels * g->n_features);
+  PTY_REQUIRE(table && out, PTY_EINVAL, "inr_fwd: null pointer");
+  if (pts->n == 0) return PTY_OK;
+  const int blocks = fwd_blocks(pts->n);
+  if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_fwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, *pts,
+                       (float2*)out);
+  else
+    hipLaunchKernelGGL(k_inr_fwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, *pts,
+                       (float2*)out);
+  return check_launch("inr_fwd");
+}
+
+extern "C" size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts) {
+  if (!mlp || !pts) return 0;
+  const int H = mlp->n_hidden;
+  const size_t total = (H == 1) ? GradLayout<1>::total : GradLayout<2>::total;
+  return sizeof(float) * total * (size_t)bwd_blocks(pts->n);
+}
+
+extern "C" int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table, float* dtable,
+                           const PtyPoints* pts, const float* dout, void* work, size_t work_bytes,
+                           hipStream_t stream) {
+  int rc = validate_grid(g);
+  if (!rc) rc = validate_mlp(mlp, true);
+  if (!rc) rc = validate_mlp(grad, true);
+  if (!rc) rc = validate_points(pts);
+  if (rc) return rc;
+  PTY_REQUIRE(grad->n_in == mlp->n_in && grad->n_hidden == mlp->n_hidden, PTY_EINVAL, "grad descriptor mismatch");
+  PTY_REQUIRE(mlp->n_in == g->n_levels * g->n_features, PTY_EINVAL, "mlp n_in != levels*features");
+  PTY_REQUIRE(table && dtable && dout, PTY_EINVAL, "inr_bwd: null pointer");
+  if (pts->n == 0) return PTY_OK;
+  PTY_REQUIRE(work && work_bytes >= pty_inr_bwd_workspace(g, mlp, pts), PTY_EWORK, "inr_bwd: workspace too small");
+  const int blocks = bwd_blocks(pts->n);
+  float* partial = (float*)work;
+  if (mlp->n_hidden == 1) {
+    hipLaunchKernelGGL(k_inr_bwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, dtable,
+                       *pts, (const float2*)dout, partial);
+    hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((GradLayout<1>::total + 255) / 256), dim3(256), 0, stream,
+                       (const float*)partial, blocks, *grad);
+  } else {
+    hipLaunchKernelGGL(k_inr_bwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, dtable,
+                       *pts, (const float2*)dout, partial);
+    hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((GradLayout<2>::total + 255) / 256), dim3(256), 0, stream,
+                       (const float*)partial, blocks, *grad);
+  }
+  return check_launch("inr_bwd");
+}
+
+extern "C" int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream) {
+  int rc = validate_mlp(mlp, true);
+  if (rc) return rc;
+  PTY_REQUIRE(n >= 0 && (n == 0 || (feats && out)), PTY_EINVAL, "mlp_fwd: null pointer");
+  if (n == 0) return PTY_OK;
+  const int blocks = fwd_blocks(n);
+  if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_mlp_fwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *mlp, feats, n, (float2*)out);
+  else
+    hipLaunchKernelGGL(k_mlp_fwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *mlp, feats, n, (float2*)out);
+  return check_launch("mlp_fwd");
+}

@@ -1,0 +1,239 @@
+"""Fused reconstruction step: one optimizer step of Recovery.py:247-321 as a
+fixed sequence of HIP launches on one stream, with no autograd graph and no
+host synchronisation (every pointer is fixed at construction, so the step can
+be replayed or captured).
+
+Per step (batch of positions ``batch_idx`` on the device):
+  object  = Upsample(INR_obj(grid))            [+ k_o * . + initial_o]
+  probe   = INR_prb(grid)                      [+ k_p * . + initial_p]
+  probe_n = COM-roll + energy-normalise(probe)
+  loss_b, dobject, dprobe_n = far field + loss + adjoint over the batch
+  dprobe  = normalise^T(dprobe_n);  residual^T;  dsmall = Upsample^T(dobject)
+  grads   = INR^T (recomputed activations, hash-table atomics)
+  Adam over all trainable parameters (one flat launch)
+Parameters of the model are rebound as views into one flat fp32 buffer so
+that Adam is a single elementwise kernel with torch.optim.Adam semantics.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .complex_inr import ComplexINRModel2D, mlp_descriptor
+from .farfield import check_windows, far_descriptor, positions_tensor, prepare_stack
+
+
+def _real_numel(p: torch.Tensor) -> int:
+    return p.numel() * (2 if p.is_complex() else 1)
+
+
+class FlatParams:
+    """All kernel-visible parameters of the model packed into one fp32 buffer.
+
+    Trainable parameters come first (Adam runs over that prefix); frozen
+    ones (e.g. omega_0 with trainable_omega0=False) follow.  Each
+    nn.Parameter is rebound to a view of the buffer; a grad view of the same
+    shape is kept for the kernels to accumulate into.
+    """
+
+    def __init__(self, params: list[torch.nn.Parameter], device):
+        train = [p for p in params if p.requires_grad]
+        frozen = [p for p in params if not p.requires_grad]
+        order = train + frozen
+        total = sum(_real_numel(p) for p in order)
+        self.n_train = sum(_real_numel(p) for p in train)
+        self.data = torch.empty(total, dtype=torch.float32, device=device)
+        self.grad = torch.zeros(total, dtype=torch.float32, device=device)
+        self.grad_views = {}
+        off = 0
+        for p in order:
+            n = _real_numel(p)
+            dv = self.data[off:off + n]
+            gv = self.grad[off:off + n]
+            if p.is_complex():
+                dv.copy_(torch.view_as_real(p.detach().to(device)).reshape(-1))
+                pview = torch.view_as_complex(dv.view(*p.shape, 2))
+                gview = torch.view_as_complex(gv.view(*p.shape, 2))
+            else:
+                dv.copy_(p.detach().to(device).reshape(-1))
+                pview = dv.view(p.shape)
+                gview = gv.view(p.shape)
+            p.data = pview
+            self.grad_views[id(p)] = gview
+            off += n
+
+    def g(self, p):
+        return self.grad_views[id(p)]
+
+
+class FusedReconstruction:
+    """Fused training step for a ComplexINRModel2D on a measured stack."""
+
+    def __init__(self, model: ComplexINRModel2D, stack_mnf: torch.Tensor, tlY, tlX, loss_type="smooth_L1_loss",
+                 lr=1e-3, batch_size=10, lr_decay_step=10000, lr_decay_gamma=0.1, betas=(0.9, 0.999),
+                 eps=1e-8):
+        dev = model.xy_hash_encoder.params.device
+        if dev.type != "cuda":
+            raise RuntimeError("FusedReconstruction needs the model on the GPU")
+        if not model.update_probe:
+            raise RuntimeError("update_probe must be True for conventional ptychography")
+        self.model, self.device, self.loss_type = model, dev, loss_type
+        self.batch_size = int(batch_size)
+        self.lr0, self.gamma, self.decay_step = float(lr), float(lr_decay_gamma), int(lr_decay_step)
+        self.beta1, self.beta2, self.eps = float(betas[0]), float(betas[1]), float(eps)
+        self.M, self.N = model.probe_width, model.probe_height
+        self.H, self.W = model.output_width, model.output_height
+        self.n_small = model.downsampled_width
+        self.ds = int(model.downsample_factor)
+        if model.downsampled_width != model.downsampled_height or self.n_small * self.ds != self.H:
+            raise ValueError("the fused step needs a square canvas equal to downsample_factor * INR grid")
+        if self.M != self.N:
+            raise ValueError("the fused step needs a square probe")
+        check_windows(tlY, tlX, self.M, self.N, self.H, self.W)
+        self.frames = stack_mnf.shape[2]
+        self.stack = prepare_stack(stack_mnf.to(dev))
+        self.tl = positions_tensor(tlY, tlX, dev)
+
+        for mlp in (model.object_complex_mlp, model.probe_complex_mlp):
+            mlp.check_supported()
+        self.obj_ts = model.object_complex_mlp.tensors()
+        self.prb_ts = model.probe_complex_mlp.tensors()
+        params = [model.xy_hash_encoder.params, model.probe_hash_encoder.params, *self.obj_ts, *self.prb_ts]
+        if model.has_object_residual:
+            params.append(model.k_object)
+        if model.has_probe_residual:
+            params.append(model.k_probe)
+        self.flat = FlatParams(params, dev)
+        self.m = torch.zeros(self.flat.n_train, dtype=torch.float32, device=dev)
+        self.v = torch.zeros(self.flat.n_train, dtype=torch.float32, device=dev)
+        self.hyper = torch.tensor([self.lr0, 0.0], dtype=torch.float32, device=dev)
+        self._lr_epoch = 0
+
+        prec = model.mlp_precision
+        nin = model.num_features
+        self.obj_spec = model.xy_hash_encoder.spec
+        self.prb_spec = model.probe_hash_encoder.spec
+        self.obj_grid = self.obj_spec.descriptor()
+        self.prb_grid = self.prb_spec.descriptor()
+        self.obj_pts = model.grid_points("object")
+        self.prb_pts = model.grid_points("probe")
+        self.obj_pdesc = self.obj_pts.descriptor()
+        self.prb_pdesc = self.prb_pts.descriptor()
+        self.obj_mlp = mlp_descriptor(self.obj_ts, nin, prec)
+        self.prb_mlp = mlp_descriptor(self.prb_ts, nin, prec)
+        self.obj_gmlp = mlp_descriptor([self.flat.g(t) for t in self.obj_ts], nin, prec)
+        self.prb_gmlp = mlp_descriptor([self.flat.g(t) for t in self.prb_ts], nin, prec)
+        self.obj_dtable = self.flat.g(model.xy_hash_encoder.params)
+        self.prb_dtable = self.flat.g(model.probe_hash_encoder.params)
+
+        c64 = torch.complex64
+        n, M, N, H, W = self.n_small, self.M, self.N, self.H, self.W
+        self.obj_small = torch.empty(n * n, dtype=c64, device=dev)
+        self.obj_up = torch.empty(H, W, dtype=c64, device=dev)
+        self.obj = torch.empty(H, W, dtype=c64, device=dev) if model.has_object_residual else self.obj_up
+        self.prb_mlp_out = torch.empty(M * N, dtype=c64, device=dev)
+        self.probe = torch.empty(M, N, dtype=c64, device=dev) if model.has_probe_residual else \
+            self.prb_mlp_out.view(M, N)
+        self.probe_n = torch.empty(M, N, dtype=c64, device=dev)
+        self.stats = torch.zeros(8, dtype=torch.float32, device=dev)
+        self.dobj = torch.empty(H, W, dtype=c64, device=dev)
+        self.dobj_up = torch.empty(H, W, dtype=c64, device=dev) if model.has_object_residual else self.dobj
+        self.dprobe_n = torch.empty(M, N, dtype=c64, device=dev)
+        self.dprobe = torch.empty(M, N, dtype=c64, device=dev)
+        self.dprb_mlp = torch.empty(M, N, dtype=c64, device=dev) if model.has_probe_residual else self.dprobe
+        self.dsmall = torch.empty(n, n, dtype=c64, device=dev)
+        self.loss = torch.zeros(self.batch_size, dtype=torch.float32, device=dev)
+
+        q = _lib.query
+        self.ws_up = _lib.workspace(q("pty_upsample_workspace", n, self.ds), dev)
+        self.ws_probe = _lib.workspace(q("pty_probe_workspace", M, N), dev)
+        self.ws_res = _lib.workspace(max(q("pty_residual_workspace", H * W), q("pty_residual_workspace", M * N)),
+                                     dev)
+        self.ws_obj = _lib.workspace(q("pty_inr_bwd_workspace", self.obj_grid, self.obj_mlp, self.obj_pdesc), dev)
+        self.ws_prb = _lib.workspace(q("pty_inr_bwd_workspace", self.prb_grid, self.prb_mlp, self.prb_pdesc), dev)
+        self.far = {}
+        for b in {self.batch_size, self.frames % self.batch_size}:
+            if b > 0:
+                d = far_descriptor(M, N, H, W, b, self.frames, loss_type)
+                self.far[b] = (d, _lib.workspace(q("pty_farfield_workspace", d), dev))
+        self.epoch = 0
+
+    # ------------------------------------------------------------------ step
+    def set_lr(self, lr: float):
+        _lib.call("pty_fill", self.hyper.data_ptr(), 1, float(lr), _lib.stream_ptr(self.device))
+
+    def forward_fields(self):
+        s = _lib.stream_ptr(self.device)
+        C = _lib.call
+        m = self.model
+        C("pty_inr_fwd", self.obj_grid, self.obj_mlp, m.xy_hash_encoder.params.data_ptr(), self.obj_pdesc,
+          self.obj_small.data_ptr(), s)
+        C("pty_upsample_fwd", self.obj_small.data_ptr(), self.n_small, self.ds, self.obj_up.data_ptr(),
+          self.ws_up.data_ptr(), self.ws_up.numel(), s)
+        if m.has_object_residual:
+            C("pty_residual_fwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), m.object_initial.data_ptr(),
+              self.obj_up.numel(), self.obj.data_ptr(), s)
+        C("pty_inr_fwd", self.prb_grid, self.prb_mlp, m.probe_hash_encoder.params.data_ptr(), self.prb_pdesc,
+          self.prb_mlp_out.data_ptr(), s)
+        if m.has_probe_residual:
+            C("pty_residual_fwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), m.probe_initial.data_ptr(),
+              self.prb_mlp_out.numel(), self.probe.data_ptr(), s)
+        C("pty_probe_normalize_fwd", self.probe.data_ptr(), self.M, self.N, self.probe_n.data_ptr(),
+          self.stats.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
+
+    def step(self, batch_idx: torch.Tensor, update: bool = True):
+        """One optimizer step on the positions ``batch_idx`` (int32, device)."""
+        s = _lib.stream_ptr(self.device)
+        C = _lib.call
+        m = self.model
+        b = batch_idx.numel()
+        d, ws_far = self.far[b]
+        C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
+        self.forward_fields()
+        C("pty_fill", self.dobj.data_ptr(), 2 * self.dobj.numel(), 0.0, s)
+        C("pty_farfield_fwd_bwd", d, self.obj.data_ptr(), self.probe_n.data_ptr(), self.stack.data_ptr(),
+          batch_idx.data_ptr(), self.tl.data_ptr(), self.loss.data_ptr(), self.dobj.data_ptr(),
+          self.dprobe_n.data_ptr(), ws_far.data_ptr(), ws_far.numel(), s)
+        C("pty_probe_normalize_bwd", self.probe.data_ptr(), self.dprobe_n.data_ptr(), self.M, self.N,
+          self.stats.data_ptr(), self.dprobe.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
+        if m.has_probe_residual:
+            C("pty_residual_bwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), self.dprobe.data_ptr(),
+              self.dprobe.numel(), self.dprb_mlp.data_ptr(), self.flat.g(m.k_probe).data_ptr(),
+              self.ws_res.data_ptr(), self.ws_res.numel(), s)
+        if m.has_object_residual:
+            C("pty_residual_bwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), self.dobj.data_ptr(),
+              self.dobj.numel(), self.dobj_up.data_ptr(), self.flat.g(m.k_object).data_ptr(),
+              self.ws_res.data_ptr(), self.ws_res.numel(), s)
+        C("pty_upsample_adj", self.dobj_up.data_ptr(), self.n_small, self.ds, self.dsmall.data_ptr(),
+          self.ws_up.data_ptr(), self.ws_up.numel(), s)
+        C("pty_inr_bwd", self.obj_grid, self.obj_mlp, self.obj_gmlp, m.xy_hash_encoder.params.data_ptr(),
+          self.obj_dtable.data_ptr(), self.obj_pdesc, self.dsmall.data_ptr(), self.ws_obj.data_ptr(),
+          self.ws_obj.numel(), s)
+        C("pty_inr_bwd", self.prb_grid, self.prb_mlp, self.prb_gmlp, m.probe_hash_encoder.params.data_ptr(),
+          self.prb_dtable.data_ptr(), self.prb_pdesc, self.dprb_mlp.data_ptr(), self.ws_prb.data_ptr(),
+          self.ws_prb.numel(), s)
+        if update:
+            C("pty_adam_tick", self.hyper.data_ptr(), s)
+            C("pty_adam", self.flat.data.data_ptr(), self.flat.grad.data_ptr(), self.m.data_ptr(),
+              self.v.data_ptr(), self.flat.n_train, self.hyper.data_ptr(), self.beta1, self.beta2, self.eps, s)
+        return self.loss[:b]
+
+    def run_epoch(self, generator: torch.Generator | None = None):
+        """One epoch: randperm over frames, ceil(frames/batch) steps (Recovery.py:243-350)."""
+        lr = self.lr0 * self.gamma ** (self.epoch // self.decay_step)
+        if self.epoch == 0 or self.epoch % self.decay_step == 0:
+            self.set_lr(lr)
+        perm = torch.randperm(self.frames, device=self.device, generator=generator).to(torch.int32)
+        total = torch.zeros((), dtype=torch.float32, device=self.device)
+        for start in range(0, self.frames, self.batch_size):
+            total += self.step(perm[start:start + self.batch_size]).sum()
+        self.epoch += 1
+        return total
+
+    def current_fields(self):
+        """(object [H,W], normalised probe [M,N]) of the current parameters."""
+        self.forward_fields()
+        return self.obj, self.probe_n

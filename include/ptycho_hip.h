@@ -1,0 +1,200 @@
+/*
+ * ptycho_hip.h -- C ABI of the MI355X (gfx950) INR-ptychography hot path.
+ *
+ * Every entry point takes raw device pointers, plain sizes and a hipStream_t,
+ * launches asynchronously on that stream, never allocates device memory and
+ * never synchronises (caller-provided workspace, sized by the *_workspace
+ * queries).  Return value: 0 on success, a negative PTY_E* code on error,
+ * with a message available from pty_last_error() (thread-local).
+ *
+ * Complex arrays are interleaved (re, im) float pairs, i.e. the memory of a
+ * torch complex64 tensor.  2-D arrays are row-major.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   tcnn.Encoding "HashGrid" fwd/bwd   models/complex_inr.py:130-145, 279, 294
+ *   ComplexMLP.net (Gabor layers)      models/complex_inr.py:6-82, 282, 295, 313
+ *   fourier_upsample + its autograd    models/complex_inr.py:252-271, 287
+ *   apply_residual                     models/complex_inr.py:316-334
+ *   probe COM roll + energy norm       Recovery.py:262-285
+ *   per-position far field + losses    Recovery.py:287-307 (utils/optics.py:102-112)
+ *   torch.optim.Adam step              Recovery.py:238-241, 321
+ */
+#ifndef PTYCHO_HIP_H
+#define PTYCHO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* hipStream_t;
+
+#define PTY_ABI_VERSION 1
+#define PTY_MAX_LEVELS 16
+#define PTY_MAX_HIDDEN 4
+
+enum {
+  PTY_OK = 0,
+  PTY_EINVAL = -1,    /* bad argument / descriptor */
+  PTY_EUNSUP = -2,    /* shape or option outside what the kernels implement */
+  PTY_EWORK = -3,     /* workspace too small */
+  PTY_ELAUNCH = -4,   /* HIP launch error */
+};
+
+enum { PTY_LOSS_SMOOTH_L1 = 0, PTY_LOSS_FD = 1, PTY_LOSS_POISSON = 2 };
+enum { PTY_PREC_FP32 = 0, PTY_PREC_BF16 = 1 };
+
+/* Multiresolution hash-grid layout (2-D input, F = 2 features per level).
+ * Built on the host from (n_levels, log2_hashmap_size, base_resolution,
+ * per_level_scale): scale[l] = f32(exp2(l*log2(pls)))*base - 1 (f32 steps),
+ * res[l] = ceil(scale[l]) + 1, size[l] = min(next_mult8(res^2), 2^log2T),
+ * offset[l] = sum of earlier sizes, dense when res^2 <= size.
+ * Table memory: fp32 [n_entries * 2], entry-major, features interleaved
+ * (the layout of tcnn.Encoding.params). */
+typedef struct PtyGrid {
+  int32_t abi_version;
+  int32_t n_levels;
+  int32_t n_features;
+  int32_t log2_hashmap_size;
+  float scale[PTY_MAX_LEVELS];
+  uint32_t res[PTY_MAX_LEVELS];
+  uint32_t size[PTY_MAX_LEVELS];
+  uint32_t offset[PTY_MAX_LEVELS];
+  uint32_t dense_mask;
+  uint32_t n_entries;
+} PtyGrid;
+
+/* Complex-exponential MLP (ComplexMLP with hidden_features = 64, i.e. 32
+ * complex units).  Device pointers; complex tensors interleaved.
+ *   w0 [32, n_in] f32, b0 [32] f32            first (real) Gabor layer
+ *   wh[k] [32, 32] c64, bh[k] [32] c64        hidden Gabor layers k < n_hidden
+ *   omega[k] -> [1] f32                       omega_0 of layer k (0 = first)
+ *   wo [1, 32] c64, bo [1] c64                final complex Linear(32, 1)
+ * For gradients (PtyMlp used as the gradient descriptor) every pointer is an
+ * accumulation target of the same shape (the kernels add into it). */
+typedef struct PtyMlp {
+  int32_t n_in;        /* = n_levels * n_features, <= 32 */
+  int32_t n_hidden;    /* 0 .. PTY_MAX_HIDDEN */
+  int32_t precision;   /* PTY_PREC_* for the GEMMs (accumulation is fp32) */
+  int32_t pad_;
+  float* w0;
+  float* b0;
+  float* wh[PTY_MAX_HIDDEN];
+  float* bh[PTY_MAX_HIDDEN];
+  float* omega[PTY_MAX_HIDDEN + 1];
+  float* wo;
+  float* bo;
+} PtyMlp;
+
+/* Coordinate source of the INR evaluation: either explicit coords [n, 2]
+ * (column 0 = first grid input) or, when coords == NULL, the regular grid of
+ * ComplexINRModel2D.init_scale_grids (models/complex_inr.py:179-207): point
+ * n = i*n_axis0 + j has input (axis0[j], axis1[i]), n = n_axis1 * n_axis0. */
+typedef struct PtyPoints {
+  const float* coords;
+  const float* axis0;
+  const float* axis1;
+  int32_t n_axis0;
+  int32_t n_axis1;
+  int64_t n;
+} PtyPoints;
+
+/* Far-field batch descriptor (Recovery.py:287-307). */
+typedef struct PtyFar {
+  int32_t M, N;              /* pattern / probe size; powers of two, 16..1024 */
+  int32_t canvas_h, canvas_w;
+  int32_t n_batch;           /* positions in this batch */
+  int32_t n_frames;          /* patterns in the measured stack */
+  int32_t loss_type;         /* PTY_LOSS_* */
+  int32_t pad_;
+} PtyFar;
+
+int pty_abi_version(void);
+const char* pty_last_error(void);
+
+/* ---- hash grid (replaces tcnn.Encoding fwd/bwd, complex_inr.py:138-145) ---- */
+/* out [n, n_levels*2] f32 features. */
+int pty_hashgrid_fwd(const PtyGrid* g, const float* coords, int64_t n, const float* table,
+                     float* out, hipStream_t stream);
+/* dtable[entry*2+f] += sum over points/corners of w_c * dout (atomics). */
+int pty_hashgrid_bwd(const PtyGrid* g, const float* coords, int64_t n, const float* dout,
+                     float* dtable, hipStream_t stream);
+/* Corner entry indices [n, n_levels, 4] (u32, level offset included): the
+ * integer part of the encoding, exposed for bit-exact checks. */
+int pty_hashgrid_indices(const PtyGrid* g, const float* coords, int64_t n, uint32_t* idx,
+                         hipStream_t stream);
+
+/* ---- fused hash encode + complex MLP (complex_inr.py:279-282, 294-295) ---- */
+/* out [n] c64 = ComplexMLP.net(HashGrid(points)). */
+int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* table, const PtyPoints* pts,
+                float* out, hipStream_t stream);
+size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts);
+/* Given dout [n] c64, recomputes the activations and accumulates the
+ * gradients of every MLP parameter into *grad and of the table into dtable. */
+int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table,
+                float* dtable, const PtyPoints* pts, const float* dout, void* work,
+                size_t work_bytes, hipStream_t stream);
+/* ComplexMLP.net over given features [n, n_in] (complex_inr.py:313). */
+int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream);
+
+/* ---- Fourier upsample (complex_inr.py:252-271) and its adjoint ---- */
+size_t pty_upsample_workspace(int32_t n, int32_t ds);
+/* big [ds*n, ds*n] c64 = ds^2 ifft2(ifftshift(pad(fftshift(fft2(small))))). */
+int pty_upsample_fwd(const float* small, int32_t n, int32_t ds, float* big, void* work,
+                     size_t work_bytes, hipStream_t stream);
+/* gsmall [n, n] c64 = ifft2_n(crop(fft2_{ds n}(gbig))) (exact adjoint). */
+int pty_upsample_adj(const float* gbig, int32_t n, int32_t ds, float* gsmall, void* work,
+                     size_t work_bytes, hipStream_t stream);
+
+/* ---- residual out = k * x + initial (complex_inr.py:316-334) ---- */
+int pty_residual_fwd(const float* k, const float* x, const float* initial, int64_t n, float* out,
+                     hipStream_t stream);
+size_t pty_residual_workspace(int64_t n);
+/* gx = k * gout; gk += Re sum(gout * conj(x)). */
+int pty_residual_bwd(const float* k, const float* x, const float* gout, int64_t n, float* gx,
+                     float* gk, void* work, size_t work_bytes, hipStream_t stream);
+
+/* ---- probe COM roll + energy normalisation (Recovery.py:262-285) ---- */
+size_t pty_probe_workspace(int32_t M, int32_t N);
+/* stats (device, 8 floats): [cp0, cp1, scale, energy, applied_roll, applied_scale, 0, 0]. */
+int pty_probe_normalize_fwd(const float* probe, int32_t M, int32_t N, float* probe_n, float* stats,
+                            void* work, size_t work_bytes, hipStream_t stream);
+int pty_probe_normalize_bwd(const float* probe, const float* gprobe_n, int32_t M, int32_t N,
+                            const float* stats, float* gprobe, void* work, size_t work_bytes,
+                            hipStream_t stream);
+
+/* ---- far field, losses and their adjoint (Recovery.py:287-307) ---- */
+/* Reorders the measured stack from the reference layout [M, N, frames] into
+ * pattern-major [frames, M, N] with ifftshift applied (natural FFT order). */
+int pty_stack_prepare(const float* stack_mnf, int32_t M, int32_t N, int32_t frames, float* out,
+                      hipStream_t stream);
+size_t pty_farfield_workspace(const PtyFar* d);
+/* loss[b] = per-pattern loss (mean over pixels) of position batch_idx[b];
+ * dobject (c64 canvas) += d(sum_b loss[b])/d object; dprobe (c64 [M,N]) =
+ * d(sum_b loss[b])/d probe_n (overwritten).  tl is [frames, 2] = (tlY, tlX). */
+int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* probe_n,
+                         const float* stack, const int32_t* batch_idx, const int32_t* tl,
+                         float* loss, float* dobject, float* dprobe, void* work,
+                         size_t work_bytes, hipStream_t stream);
+/* Intensities [n_batch, M, N] = |fftshift(fft2(probe_n * patch))|^2 (forward only). */
+int pty_farfield_intensity(const PtyFar* d, const float* object, const float* probe_n,
+                           const int32_t* batch_idx, const int32_t* tl, float* intensity,
+                           void* work, size_t work_bytes, hipStream_t stream);
+
+/* ---- Adam (torch.optim.Adam, amsgrad=False, weight_decay=0; Recovery.py:238, 321) ---- */
+/* hyper (device, 2 floats): [lr, step]; bias corrections 1 - beta^step are
+ * formed on the device.  pty_adam_tick adds 1 to step (call once per step
+ * before the pty_adam calls of that step). */
+int pty_adam(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+             const float* hyper, float beta1, float beta2, float eps, hipStream_t stream);
+int pty_adam_tick(float* hyper, hipStream_t stream);
+
+/* fill kernel (graph-safe replacement for hipMemsetAsync) */
+int pty_fill(float* p, int64_t n, float value, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTYCHO_HIP_H */

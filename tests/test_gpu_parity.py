@@ -1,0 +1,261 @@
+"""GPU parity: every HIP entry point against the FP64 CPU oracle, through the C ABI.
+
+Tolerances (FP32 kernels vs FP64 oracle on the same inputs):
+  hash indices: bit-exact; features 1e-6 relative (norm);
+  far-field intensities: 1e-5 relative per pattern (max-norm);
+  fields / gradients: stated per test (FP32 accumulation of long sums).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ptycho_oracle as orc
+from tests.helpers import oracle_params, oracle_table, rel_err, step_reference
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(pkg, dev, canvas, M, log2T=12, residual=True, probe_init=None, seed=0, hidden_layers=2):
+    torch.manual_seed(seed)
+    init = probe_init if probe_init is not None else torch.ones(M, M, dtype=torch.complex64)
+    return pkg.ComplexINRModel2D(canvas, canvas, 2, probe_width=M, probe_height=M, log2_hashmap_size=log2T,
+                                 hidden_omega_0=1.0, trainable_omega0=True, use_residual=residual,
+                                 probe_initial=init if residual else None, hidden_layers=hidden_layers).to(dev)
+
+
+# ---------------------------------------------------------------- hash grid
+@pytest.mark.parametrize("log2T", [14, 18])
+def test_hash_indices_bit_exact(pkg, dev, log2T):
+    spec = pkg.level_table(16, 2, log2T, 16, 1.5)
+    table = orc.level_table(16, log2T, 16, 1.5)
+    rng = np.random.default_rng(log2T)
+    pts = np.concatenate([rng.random((4000, 2)), orc.grid_coords(128, 128).numpy()[::7],
+                          np.array([[0.0, 0.0], [0.999999, 0.999999], [0.5, 0.25]])]).astype(np.float32)
+    idx = pkg.hashgrid_indices(torch.from_numpy(pts).to(dev), spec).cpu().numpy()
+    ref, _ = orc.hash_corners(pts, table)
+    assert np.array_equal(idx, ref)
+
+
+def test_hash_encode_fwd_bwd(pkg, dev):
+    torch.manual_seed(0)
+    enc = pkg.HashGridEncoding(2, {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2,
+                                   "log2_hashmap_size": 14, "base_resolution": 16, "per_level_scale": 1.5}).to(dev)
+    with torch.no_grad():
+        enc.params.normal_()
+    x = torch.rand(3000, 2, device=dev)
+    out = enc(x)
+    table = orc.level_table(16, 14, 16, 1.5)
+    tp = enc.params.detach().cpu().double().requires_grad_(True)
+    ref = orc.hashgrid_encode(x.cpu().numpy(), tp, table)
+    assert rel_err(out, ref) < 1e-6
+    y = torch.randn_like(out)
+    (out * y).sum().backward()
+    (ref * y.cpu().double()).sum().backward()
+    assert rel_err(enc.params.grad, tp.grad) < 1e-5
+
+
+# ---------------------------------------------------------------- fused INR field
+@pytest.mark.parametrize("hidden_layers", [1, 2])
+def test_inr_field_fwd_bwd(pkg, dev, hidden_layers):
+    model = _model(pkg, dev, 64, 16, log2T=12, residual=False, hidden_layers=hidden_layers)
+    with torch.no_grad():   # features of order one so that every layer matters
+        model.xy_hash_encoder.params.uniform_(-1, 1)
+    pts = model.grid_points("object")
+    f = model._field(model.xy_hash_encoder, model.object_complex_mlp, "object")
+    params = oracle_params(model)
+    table = oracle_table(model)
+    ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], table, params["obj_mlp"], 32, 32)
+    assert rel_err(f.reshape(32, 32), ref) < 2e-6
+    g = torch.randn(f.shape, dtype=torch.complex64, device=dev)
+    (torch.real(torch.conj(g) * f)).sum().backward()
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref.reshape(-1))).sum().backward()
+    assert rel_err(model.xy_hash_encoder.params.grad, params["obj_table"].grad) < 1e-4
+    mlp = params["obj_mlp"]
+    ours = model.object_complex_mlp.tensors()
+    theirs = []
+    for (w, b, om) in mlp["layers"]:
+        theirs += [w, b, om]
+    theirs += [mlp["out_weight"], mlp["out_bias"]]
+    for a, b in zip(ours, theirs):
+        assert rel_err(a.grad, b.grad) < 1e-4, (a.shape, rel_err(a.grad, b.grad))
+
+
+def test_mlp_forward_from_features(pkg, dev):
+    model = _model(pkg, dev, 64, 16, residual=False)
+    feats = torch.randn(777, 32, device=dev)
+    with torch.no_grad():
+        out = model.object_complex_mlp.net[0].linear.weight  # noqa: F841 (touch)
+        got = pkg.mlp_forward(model.object_complex_mlp, feats)
+    params = oracle_params(model)["obj_mlp"]
+    ref = orc.complex_mlp(feats.cpu().double(), params["layers"], params["out_weight"], params["out_bias"])
+    assert rel_err(got, ref) < 2e-6
+
+
+# ---------------------------------------------------------------- upsample
+@pytest.mark.parametrize("n", [16, 64, 128, 512])
+def test_upsample_fwd_adj(pkg, dev, n):
+    torch.manual_seed(n)
+    x = torch.randn(n, n, dtype=torch.complex64, device=dev).requires_grad_(True)
+    y = pkg.fourier_upsample(x, 2)
+    ref = orc.fourier_upsample(x.detach().cpu().to(torch.complex128)[None, None], 2)[0, 0]
+    assert rel_err(y, ref) < 2e-6
+    assert rel_err(y[::2, ::2], x) < 2e-6          # exact even-sample identity
+    g = torch.randn(2 * n, 2 * n, dtype=torch.complex64, device=dev)
+    (torch.real(torch.conj(g) * y)).sum().backward()
+    xr = x.detach().cpu().to(torch.complex128).requires_grad_(True)
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) *
+                orc.fourier_upsample(xr[None, None], 2)[0, 0])).sum().backward()
+    assert rel_err(x.grad, xr.grad) < 2e-6
+
+
+# ---------------------------------------------------------------- probe normalisation
+def test_probe_normalize_fwd_bwd(pkg, dev):
+    rng = np.random.default_rng(0)
+    M = 64
+    yy, xx = np.mgrid[0:M, 0:M]
+    amp = np.exp(-((yy - 40.3) ** 2 + (xx - 22.7) ** 2) / 80.0)
+    p0 = (amp * np.exp(1j * rng.uniform(-1, 1, (M, M)))).astype(np.complex64)
+    p = torch.from_numpy(p0).to(dev).requires_grad_(True)
+    pn, stats = pkg.normalize_probe(p, return_stats=True)
+    pr = torch.from_numpy(p0).to(torch.complex128).requires_grad_(True)
+    ref = orc.probe_center_normalize(pr)
+    cp = orc.probe_com_shift(pr)
+    st = stats.cpu().numpy()
+    assert int(st[0]) == int(cp[0]) % M and int(st[1]) == int(cp[1]) % M and st[4] == 1.0
+    assert rel_err(pn, ref) < 1e-6
+    g = torch.randn(M, M, dtype=torch.complex64, device=dev)
+    (torch.real(torch.conj(g) * pn)).sum().backward()
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref)).sum().backward()
+    assert rel_err(p.grad, pr.grad) < 1e-5
+
+
+# ---------------------------------------------------------------- far field
+@pytest.mark.parametrize("M", [16, 64, 256])
+def test_farfield_intensity(pkg, dev, M):
+    rng = np.random.default_rng(M)
+    H = 2 * M + 24
+    obj = (rng.standard_normal((H, H)) + 1j * rng.standard_normal((H, H))).astype(np.complex64)
+    probe = (rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M))).astype(np.complex64)
+    tlY = rng.integers(0, H - M + 1, 7)
+    tlX = rng.integers(0, H - M + 1, 7)
+    tl = pkg.positions_tensor(tlY, tlX, dev)
+    bidx = torch.tensor([3, 0, 6, 6, 1], dtype=torch.int32, device=dev)
+    I = pkg.farfield_intensity(torch.from_numpy(obj).to(dev), torch.from_numpy(probe).to(dev), bidx, tl, 7)
+    for b, i in enumerate([3, 0, 6, 6, 1]):
+        ref = orc.farfield_intensity(torch.from_numpy(obj).to(torch.complex128),
+                                     torch.from_numpy(probe).to(torch.complex128), int(tlY[i]), int(tlX[i]))
+        err = (I[b].cpu().double() - ref).abs().max() / ref.abs().max()
+        assert err < 1e-5, (b, float(err))
+
+
+@pytest.mark.parametrize("loss_type", ["smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss"])
+@pytest.mark.parametrize("M", [16, 64])
+def test_farfield_loss_and_grads(pkg, dev, loss_type, M):
+    rng = np.random.default_rng(M + len(loss_type))
+    H = 3 * M
+    F = 9
+    obj = (0.5 + 0.5 * rng.random((H, H))) * np.exp(1j * rng.uniform(-1, 1, (H, H)))
+    probe = (rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M))) * 0.3
+    tlY = rng.integers(0, H - M + 1, F)
+    tlX = rng.integers(0, H - M + 1, F)
+    stack = rng.poisson(5.0, (M, M, F)).astype(np.float32)
+    batch = [4, 1, 7, 2]
+    o = torch.from_numpy(obj.astype(np.complex64)).to(dev).requires_grad_(True)
+    p = torch.from_numpy(probe.astype(np.complex64)).to(dev).requires_grad_(True)
+    st = pkg.prepare_stack(torch.from_numpy(stack).to(dev))
+    tl = pkg.positions_tensor(tlY, tlX, dev)
+    L, per = pkg.farfield_loss(o, p, st, torch.tensor(batch, dtype=torch.int32, device=dev), tl, loss_type,
+                               return_per_pattern=True)
+    L.backward()
+    o64 = torch.from_numpy(obj.astype(np.complex64)).to(torch.complex128).requires_grad_(True)
+    p64 = torch.from_numpy(probe.astype(np.complex64)).to(torch.complex128).requires_grad_(True)
+    Lr = orc.batch_loss(o64, p64, torch.from_numpy(stack).double(), batch, tlY, tlX, loss_type)
+    Lr.backward()
+    assert abs(L.item() - Lr.item()) < 2e-5 * abs(Lr.item())
+    for b, i in enumerate(batch):
+        li = orc.batch_loss(o64.detach(), p64.detach(), torch.from_numpy(stack).double(), [i], tlY, tlX, loss_type)
+        assert abs(per[b].item() - li.item()) < 2e-5 * abs(li.item())
+    assert rel_err(o.grad, o64.grad) < 1e-4
+    assert rel_err(p.grad, p64.grad) < 1e-4
+
+
+# ---------------------------------------------------------------- fused step
+@pytest.mark.parametrize("loss_type", ["smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss"])
+def test_fused_step_matches_oracle(pkg, dev, loss_type):
+    inst = pkg.synthetic.make_named("small", seed=1)
+    M = inst.probe.shape[0]
+    model = _model(pkg, dev, inst.canvas, M, log2T=12, residual=True,
+                   probe_init=torch.from_numpy(inst.probe.astype(np.complex64)))
+    params = oracle_params(model)
+    eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                  loss_type=loss_type, batch_size=5)
+    batch = [0, 7, 13, 22, 35]
+    loss = eng.step(torch.tensor(batch, dtype=torch.int32, device=dev), update=False).sum().item()
+    ref_loss, g = step_reference(model, params, inst, batch, loss_type)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss), (loss, ref_loss)
+    assert rel_err(eng.flat.g(model.xy_hash_encoder.params), g["obj_table"]) < 2e-3
+    assert rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"]) < 2e-3
+    for a, b in zip(model.object_complex_mlp.tensors(), g["obj_mlp"]):
+        assert rel_err(eng.flat.g(a), b) < 2e-3, ("obj", a.shape, rel_err(eng.flat.g(a), b))
+    for a, b in zip(model.probe_complex_mlp.tensors(), g["prb_mlp"]):
+        assert rel_err(eng.flat.g(a), b) < 2e-3, ("prb", a.shape, rel_err(eng.flat.g(a), b))
+    assert rel_err(eng.flat.g(model.k_probe), g["k_probe"]) < 2e-3
+
+
+def test_fused_step_equals_autograd_dropin(pkg, dev):
+    """The fused step and the drop-in autograd path (model.forward + loss) agree."""
+    inst = pkg.synthetic.make_named("small", seed=2)
+    M = inst.probe.shape[0]
+    pinit = torch.from_numpy(inst.probe.astype(np.complex64))
+    m1 = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=5)
+    m2 = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=5)
+    batch = torch.tensor([1, 4, 9], dtype=torch.int32, device=dev)
+    eng = pkg.FusedReconstruction(m1, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX, batch_size=3)
+    l1 = eng.step(batch, update=False).sum()
+    obj, probe = m2()
+    pn = pkg.normalize_probe(probe)
+    st = pkg.prepare_stack(torch.from_numpy(inst.stack).to(dev))
+    tl = pkg.positions_tensor(inst.tlY, inst.tlX, dev)
+    l2 = pkg.farfield_loss(obj, pn, st, batch, tl)
+    l2.backward()
+    assert abs(l1.item() - l2.item()) < 1e-5 * abs(l2.item())
+    assert rel_err(eng.flat.g(m1.xy_hash_encoder.params), m2.xy_hash_encoder.params.grad) < 1e-5
+    assert rel_err(eng.flat.g(m1.k_probe), m2.k_probe.grad) < 1e-5
+
+
+def test_adam_matches_torch(pkg, dev):
+    torch.manual_seed(0)
+    p = torch.randn(1000, device=dev)
+    ref = p.clone().requires_grad_(True)
+    opt = torch.optim.Adam([ref], lr=1e-3)
+    m = torch.zeros_like(p)
+    v = torch.zeros_like(p)
+    hyper = torch.tensor([1e-3, 0.0], device=dev)
+    s = pkg._lib.stream_ptr(dev)
+    for _ in range(5):
+        g = torch.randn(1000, device=dev)
+        ref.grad = g.clone()
+        opt.step()
+        pkg._lib.call("pty_adam_tick", hyper.data_ptr(), s)
+        pkg._lib.call("pty_adam", p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), 1000, hyper.data_ptr(),
+                      0.9, 0.999, 1e-8, s)
+    assert rel_err(p, ref.detach()) < 1e-6
+
+
+# ---------------------------------------------------------------- full-size properties
+def test_full_size_properties(pkg, dev):
+    """cfg1 shapes (INR 512^2 -> canvas 1024, probe 256^2, T=2^18): size-independent checks."""
+    torch.manual_seed(0)
+    model = _model(pkg, dev, 1024, 256, log2T=18, residual=True)
+    obj, probe = model()
+    assert obj.shape == (1024, 1024) and probe.shape == (256, 256)
+    small = model._field(model.xy_hash_encoder, model.object_complex_mlp, "object").reshape(512, 512)
+    assert rel_err(obj.detach()[::2, ::2], small.detach()) < 2e-6      # upsample identity at full size
+    assert torch.isfinite(torch.view_as_real(obj)).all() and torch.isfinite(torch.view_as_real(probe)).all()
+    # Parseval of the far field at 256^2
+    pn = pkg.normalize_probe(probe.detach())
+    tl = pkg.positions_tensor(np.array([0, 300, 768]), np.array([10, 500, 768]), dev)
+    I = pkg.farfield_intensity(obj.detach(), pn, torch.tensor([0, 1, 2], dtype=torch.int32, device=dev), tl, 3)
+    for b, (y, x) in enumerate([(0, 10), (300, 500), (768, 768)]):
+        ew = (pn * obj.detach()[y:y + 256, x:x + 256]).abs().pow(2).double().sum()
+        assert abs(I[b].double().sum() / (256 * 256) - ew) < 1e-5 * ew
