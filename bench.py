@@ -1,0 +1,374 @@
+"""Throughput of the fused INR-ptychography training step on one MI355X.
+
+Workload (the reference-style "plant-like" reconstruction): canvas 1024x1024
+(INR evaluated on 512x512 and Fourier-upsampled), probe 256x256, 20x20 scan
+with 40 px step and +/-4 px jitter (400 positions), 1e5 photons per pattern
+with Poisson noise, HashGrid 16 levels x 2 features with log2_hashmap_size
+18, complex-exp MLP 64 wide with 2 hidden layers (omega 10 / 1), initProbe +
+use_residual, smooth-L1 loss, Adam lr 1e-3, batch 10, FP32.  Synthetic
+seeded data (no dataset is shipped with the reference).
+
+A "step" is one optimizer step of Recovery.py:247-321 (whole-canvas INR
+forward, one batch of 10 positions, backward, Adam); the reference's "iters"
+are epochs = ceil(400/10) = 40 steps.  value = iters/s = epochs per second.
+
+Also reported: the roofline of the dominant kernel (object INR backward,
+measured with HIP events on its launch stream inside the timed region) and
+the CPU baseline (the FP64/FP32 oracle restatement on the host cores, a
+bounded sample of steps).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+WORKLOAD = dict(canvas=1024, M=256, n_side=20, step=40, jitter=4, photons=1e5)
+LOG2T = 18
+BATCH = 10
+MLP_FLOP_BWD_PER_POINT = 37376      # dX + dW of the complex MLP per point (recompute not counted)
+MLP_FLOP_FWD_PER_POINT = 18688
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def build_problem(pkg, device, seed=0):
+    import numpy as np
+    import torch
+    inst = pkg.synthetic.make_instance(seed=seed, **WORKLOAD)
+    M = inst.probe.shape[0]
+    torch.manual_seed(seed)
+    model = pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M,
+                                  log2_hashmap_size=LOG2T, hidden_omega_0=1.0, trainable_omega0=True,
+                                  use_residual=True,
+                                  probe_initial=torch.from_numpy(inst.probe.astype(np.complex64)))
+    return inst, model.to(device)
+
+
+def cpu_baseline(inst, seconds=15.0):
+    """Oracle step (autograd + Adam) in FP32 on the host cores, bounded sample."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+    from oracle import ptycho_oracle as orc
+    from tests.helpers import oracle_geom, oracle_params, oracle_table
+
+    pkg = __graft_entry__.load_package()
+    torch.set_num_threads(os.cpu_count() or 1)
+    _, model = build_problem(pkg, "cpu")
+    params = oracle_params(model, dtype=torch.float32)
+    if "probe_initial" in params:
+        params["probe_initial"] = params["probe_initial"].to(torch.complex64)
+    leaves = [params["obj_table"], params["prb_table"], params["k_probe"]]
+    for key in ("obj_mlp", "prb_mlp"):
+        for (w, b, om) in params[key]["layers"]:
+            leaves += [w, b, om]
+        leaves += [params[key]["out_weight"], params[key]["out_bias"]]
+    opt = torch.optim.Adam(leaves, lr=1e-3)
+    table = oracle_table(model)
+    geom = oracle_geom(model, inst.tlY, inst.tlX)
+    stack = torch.from_numpy(inst.stack)
+    perm = torch.randperm(stack.shape[2])
+    times = []
+    t_end = time.perf_counter() + seconds
+    k = 0
+    while time.perf_counter() < t_end or k < 2:
+        batch = perm[(k * BATCH) % 400:(k * BATCH) % 400 + BATCH].tolist()
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        loss, _, _ = orc.step_loss(params, table, geom, stack, batch, "smooth_L1_loss", dtype=torch.float32)
+        loss.backward()
+        opt.step()
+        times.append(time.perf_counter() - t0)
+        k += 1
+    steps_per_s = len(times[1:]) / sum(times[1:]) if len(times) > 1 else 1.0 / times[0]
+    return dict(value=steps_per_s / 40.0, unit="iters/s", cores=torch.get_num_threads(), kind="port",
+                sample=f"{len(times) - 1} timed steps (+1 warm-up) of the FP32 oracle (pure-torch hash grid, "
+                       f"autograd, torch Adam) at the same shapes; steps/s / 40",
+                steps_per_s=steps_per_s)
+
+
+# ----------------------------------------------------------------------------
+# "plain" leg: the reference loop (Recovery.py:243-350) as plain PyTorch-ROCm
+# FP32 on the GPU, with a pure-torch hash grid in place of tiny-cuda-nn (which
+# has no ROCm build) and the reference's own per-position loop and host syncs.
+# Used only by --mode nrmse, to time "wall-s to the plain run's final NRMSE".
+# ----------------------------------------------------------------------------
+def _plain_hashgrid(coords, params, spec):
+    import torch
+    x = coords.float()
+    outs = []
+    tab = params.view(-1, 2)
+    for l in range(spec.n_levels):
+        s = torch.tensor(spec.scale[l], dtype=torch.float32, device=x.device)
+        pos = x * s
+        pos = pos + 0.5
+        g = torch.floor(pos)
+        f = pos - g
+        gi = g.to(torch.int64)
+        acc = 0.0
+        for c in range(4):
+            cx = gi[:, 0] + (c & 1)
+            cy = gi[:, 1] + ((c >> 1) & 1)
+            if spec.dense[l]:
+                idx = (cx + cy * spec.res[l]) % spec.size[l]
+            else:
+                idx = ((cx & 0xFFFFFFFF) ^ ((cy * 2654435761) & 0xFFFFFFFF)) % spec.size[l]
+            wx = f[:, 0] if c & 1 else 1 - f[:, 0]
+            wy = f[:, 1] if c & 2 else 1 - f[:, 1]
+            acc = acc + (wx * wy)[:, None] * tab[spec.offset[l] + idx]
+        outs.append(acc)
+    return torch.cat(outs, 1)
+
+
+def _plain_mlp(h, mlp):
+    import torch
+    import torch.nn.functional as F
+    layers = list(mlp.net)
+    z = h
+    for layer in layers[:-1]:
+        z = torch.exp(1j * layer.omega_0 * F.linear(z, layer.linear.weight, layer.linear.bias))
+    return F.linear(z, layers[-1].weight, layers[-1].bias)
+
+
+def _plain_upsample(x, sf):
+    import torch
+    import torch.nn.functional as F
+    H, W = x.shape[-2:]
+    X = torch.fft.fftshift(torch.fft.fft2(x, dim=(-2, -1)), dim=(-2, -1))
+    nH, nW = int(H * sf), int(W * sf)
+    ph, pw = (nH - H) // 2, (nW - W) // 2
+    X = F.pad(X, (pw, nW - W - pw, ph, nH - H - ph), mode="constant", value=0)
+    return torch.fft.ifft2(torch.fft.ifftshift(X, dim=(-2, -1)), dim=(-2, -1)) * (sf ** 2)
+
+
+def plain_reconstruct(model, inst, epochs, loss_type="smooth_L1_loss", lr=1e-3, batch=BATCH, seed=0,
+                      check_every=None, on_check=None):
+    """The reference training loop on the GPU in plain PyTorch (FP32, use_amp false)."""
+    import torch
+    import torch.nn.functional as F
+    dev = model.xy_hash_encoder.params.device
+    inp = torch.from_numpy(inst.stack).float().to(dev)
+    frames = inp.shape[2]
+    tlY, tlX = inst.tlY, inst.tlX
+    M = inst.probe.shape[0]
+    brY, brX = tlY + M, tlX + M
+    opt = torch.optim.Adam(lr=lr, params=filter(lambda p: p.requires_grad, model.parameters()))
+    sched = torch.optim.lr_scheduler.StepLR(opt, step_size=10000, gamma=0.1)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    ospec, pspec = model.xy_hash_encoder.spec, model.probe_hash_encoder.spec
+
+    def fields():
+        o = _plain_mlp(_plain_hashgrid(model.xy_coords, model.xy_hash_encoder.params, ospec),
+                       model.object_complex_mlp).reshape(model.downsampled_width, model.downsampled_height)
+        o = _plain_upsample(o[None, None], model.downsample_factor)[0, 0]
+        p = _plain_mlp(_plain_hashgrid(model.probe_coords, model.probe_hash_encoder.params, pspec),
+                       model.probe_complex_mlp).reshape(model.probe_width, model.probe_height)
+        if model.has_probe_residual:
+            p = model.k_probe * p + model.probe_initial
+        return o, p
+
+    t0 = time.perf_counter()
+    for epoch in range(epochs):
+        perm = torch.randperm(frames, device=dev, generator=gen)
+        for bs in range(0, frames, batch):
+            idx = perm[bs:bs + batch]
+            opt.zero_grad()
+            obj, probe = fields()
+            with torch.no_grad():
+                absP2 = probe.abs() ** 2
+                Mp, Np = absP2.shape
+                tot = absP2.sum()
+                if tot == 0:
+                    cp = None
+                else:
+                    mn = torch.tensor([Mp, Np], device=dev)
+                    cp = torch.trunc(mn / 2 - mn * torch.stack((absP2.sum(1).cumsum(0).mean(),
+                                                                absP2.sum(0).cumsum(0).mean())) / tot + 1).long()
+            if cp is not None and cp.any():
+                probe = torch.roll(probe, (-cp[0].item(), -cp[1].item()), (0, 1))
+            energy = (probe.abs() ** 2).sum()
+            if energy > 0:
+                probe = probe * (float(Mp * Np) / energy).sqrt()
+            loss_b = 0.0
+            for i in idx:
+                ew = probe * obj[tlY[i]:brY[i], tlX[i]:brX[i]]
+                I = torch.abs(torch.fft.fftshift(torch.fft.fft2(ew), dim=(-2, -1))) ** 2
+                loss_b = loss_b + F.smooth_l1_loss(I, inp[:, :, i])
+            loss_b.item()
+            loss_b.backward()
+            opt.step()
+        sched.step()
+        if check_every and on_check and ((epoch + 1) % check_every == 0 or epoch == epochs - 1):
+            torch.cuda.synchronize()
+            with torch.no_grad():
+                o, _ = fields()
+            on_check(epoch + 1, time.perf_counter() - t0, o)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def nrmse_mode(args):
+    """Wall-clock to the plain run's final object NRMSE: plain PyTorch vs fused HIP."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    shapes = dict(pkg.synthetic.SHAPES[args.workload])
+    inst = pkg.synthetic.make_instance(seed=0, **shapes)
+    M = inst.probe.shape[0]
+    log2T = 14 if args.workload == "cfg0" else LOG2T
+    mask = pkg.metrics.coverage_mask(inst.tlY, inst.tlX, M, M, inst.obj.shape, min_count=2)
+
+    def make():
+        torch.manual_seed(0)
+        return pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M,
+                                     log2_hashmap_size=log2T, hidden_omega_0=1.0, trainable_omega0=True,
+                                     use_residual=True,
+                                     probe_initial=torch.from_numpy(inst.probe.astype(np.complex64))).to(dev)
+
+    def err(o):
+        return pkg.metrics.aligned_nrmse(o.detach().cpu().numpy(), inst.obj, mask, max_shift=2)
+
+    curve_plain = []
+    wall_plain = plain_reconstruct(make(), inst, args.epochs, check_every=args.check_every,
+                                   on_check=lambda e, t, o: curve_plain.append((e, t, err(o))))
+    final_plain = curve_plain[-1][2]
+    target = final_plain * 1.01
+    model = make()
+    eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX, batch_size=BATCH)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    curve_fused = []
+    t0 = time.perf_counter()
+    hit = None
+    for ep in range(args.epochs):
+        eng.run_epoch(gen)
+        if (ep + 1) % args.check_every == 0 or ep == args.epochs - 1:
+            torch.cuda.synchronize()
+            o, _ = eng.current_fields()
+            e = err(o)
+            t = time.perf_counter() - t0
+            curve_fused.append((ep + 1, t, e))
+            if hit is None and e <= target:
+                hit = (ep + 1, t)
+    wall_fused_total = time.perf_counter() - t0
+    out = {
+        "metric": "wall-s to plain-final object NRMSE + iters/s: HIP fused vs plain, 1 MI355X",
+        "workload": args.workload, "epochs": args.epochs,
+        "plain": {"wall_s": wall_plain, "final_nrmse": final_plain, "iters_per_s": args.epochs / wall_plain,
+                  "curve": curve_plain},
+        "fused": {"wall_s_total": wall_fused_total, "final_nrmse": curve_fused[-1][2],
+                  "iters_per_s": args.epochs / wall_fused_total, "curve": curve_fused,
+                  "epochs_to_target": hit[0] if hit else None, "wall_s_to_target": hit[1] if hit else None},
+        "target_nrmse": target,
+        "speedup_to_target": (wall_plain / hit[1]) if hit else None,
+        "note": "NRMSE checks (host transfer + alignment) are inside both wall clocks",
+    }
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--mode", choices=["throughput", "nrmse"], default="throughput")
+    ap.add_argument("--workload", default="cfg0", help="nrmse mode: cfg0 | cfg1 | small")
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--check-every", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.mode == "nrmse":
+        nrmse_mode(args)
+        return
+
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    inst, model = build_problem(pkg, dev)
+    eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                  loss_type="smooth_L1_loss", lr=1e-3, batch_size=BATCH)
+    frames = eng.frames
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+
+    state = {"perm": None, "pos": frames}
+
+    def next_batch():
+        if state["pos"] >= frames:
+            state["perm"] = torch.randperm(frames, device=dev, generator=gen).to(torch.int32)
+            state["pos"] = 0
+        b = state["perm"][state["pos"]:state["pos"] + BATCH]
+        state["pos"] += BATCH
+        return b
+
+    eng.set_lr(1e-3)
+    for _ in range(args.warmup):
+        eng.step(next_batch())
+    torch.cuda.synchronize()
+    eng.enable_timing(["obj_inr_bwd", "obj_inr_fwd", "farfield"])
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        eng.step(next_batch())
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(stop)
+    ms_per_step = total_ms / args.steps
+    steps_per_s = 1000.0 / ms_per_step
+    tim = eng.timings_ms()
+    bwd_ms = sum(tim["obj_inr_bwd"]) / len(tim["obj_inr_bwd"])
+    n_obj = model.downsampled_width * model.downsampled_height
+    achieved = MLP_FLOP_BWD_PER_POINT * n_obj / (bwd_ms * 1e-3) / 1e12
+    loss_now = float(eng.loss.sum().item())
+
+    line = {
+        "metric": "wall-s to plain-final object NRMSE + iters/s: HIP fused vs plain, 1 MI355X",
+        "value": steps_per_s / 40.0,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson 1e5 photons)",
+        "config": {"workload": "canvas1024_inr512_probe256_scan20x20_batch10_log2T18_smoothL1",
+                   "steps_per_iter": 40, "global_batch": BATCH, "positions": frames,
+                   "parallelism": "single-gpu"},
+        "steps_per_s": steps_per_s,
+        "wall_s_timed": wall,
+        "kernel_ms": {k: (sum(v) / len(v) if v else None) for k, v in tim.items()},
+        "loss_last_step": loss_now,
+        "roofline": {"bound": "mfma", "kernel": "k_inr_bwd<2> (object, 512x512 points, + grad reduce)",
+                     "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic": f"{MLP_FLOP_BWD_PER_POINT} FLOP/point x {n_obj} points per launch"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

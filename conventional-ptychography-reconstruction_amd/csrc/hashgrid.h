@@ -14,6 +14,10 @@
 namespace pty {
 
 int validate_grid(const PtyGrid* g);
+// table gradient from level-major feature gradients gfeat[l][p] (hashbwd.hip)
+size_t table_bwd_workspace(const PtyGrid& g, const PtyPoints& pts);
+int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat, float* dtable, char* work,
+                     hipStream_t stream);
 
 struct LevelPos {
   uint32_t gx, gy;
@@ -21,9 +25,12 @@ struct LevelPos {
 };
 
 __device__ __forceinline__ LevelPos level_pos(const PtyGrid& g, int l, float x0, float x1) {
+  // x*scale and +0.5 are two separately rounded operations: no FMA contraction
+  // (HIP's __fmul_rn is a plain multiply and would otherwise be fused).
+#pragma clang fp contract(off)
   const float s = g.scale[l];
-  float p0 = __fadd_rn(__fmul_rn(x0, s), 0.5f);
-  float p1 = __fadd_rn(__fmul_rn(x1, s), 0.5f);
+  float p0 = x0 * s + 0.5f;
+  float p1 = x1 * s + 0.5f;
   float f0 = floorf(p0), f1 = floorf(p1);
   LevelPos r;
   r.gx = (uint32_t)(int)f0;

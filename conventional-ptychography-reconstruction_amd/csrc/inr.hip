@@ -168,6 +168,9 @@ __global__ __launch_bounds__(kThreads) void k_inr_fwd(PtyGrid g, PtyMlp m, const
   const int j = lane & 31, h = lane >> 5;
   const int64_t ntiles = (pts.n + 31) / 32;
   for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+    // weights are re-read from LDS at each use instead of being hoisted into
+    // ~270 registers for the whole loop (which forces one wave per SIMD)
+    asm volatile("" ::: "memory");
     const int64_t p = tile * 32 + j;
     const bool valid = p < pts.n;
     float x0 = 0.5f, x1 = 0.5f;
@@ -208,6 +211,9 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __r
   const int j = lane & 31, h = lane >> 5;
   const int64_t ntiles = (n + 31) / 32;
   for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+    // weights are re-read from LDS at each use instead of being hoisted into
+    // ~270 registers for the whole loop (which forces one wave per SIMD)
+    asm volatile("" ::: "memory");
     const int64_t p = tile * 32 + j;
     const bool valid = p < n;
     float feat[16];
@@ -275,7 +281,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 template <int H>
 __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
-                                                      float* __restrict__ dtable, PtyPoints pts,
+                                                      float2* __restrict__ gfeat, PtyPoints pts,
                                                       const float2* __restrict__ dout, float* __restrict__ partial) {
   __shared__ MlpLds<H> s;
   __shared__ BwdStage<H> st[kWaves];
@@ -285,15 +291,19 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
   const int j = lane & 31, h = lane >> 5;
   BwdStage<H>& S = st[wave];
 
-  v16f dW[H][4];
+  // dW of hidden layer k in complex form: dAr = G_re X_re^T + G_im X_im^T,
+  // dAi = G_im X_re^T - G_re X_im^T (the four real blocks share two
+  // accumulators).
+  v16f dWr[H], dWi[H];
   v16f dW0;
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
     dW0[v] = 0.f;
 #pragma unroll
-    for (int k = 0; k < H; ++k)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) dW[k][b][v] = 0.f;
+    for (int k = 0; k < H; ++k) {
+      dWr[k][v] = 0.f;
+      dWi[k][v] = 0.f;
+    }
   }
   float acc_c[H][2];
   float acc_om[H + 1];
@@ -305,6 +315,9 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
 
   const int64_t ntiles = (pts.n + 31) / 32;
   for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+    // weights are re-read from LDS at each use instead of being hoisted into
+    // ~270 registers for the whole loop (which forces one wave per SIMD)
+    asm volatile("" ::: "memory");
     const int64_t p = tile * 32 + j;
     const bool valid = p < pts.n;
     float x0 = 0.5f, x1 = 0.5f;
@@ -381,10 +394,10 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
         const float* Xp = S.X[k];
         const float b0v = Xp[j * WS + col];
         const float b1v = Xp[(32 + j) * WS + col];
-        dW[k][0] = mfma(a0, b0v, dW[k][0]);
-        dW[k][1] = mfma(a0, b1v, dW[k][1]);
-        dW[k][2] = mfma(a1, b0v, dW[k][2]);
-        dW[k][3] = mfma(a1, b1v, dW[k][3]);
+        dWr[k] = mfma(a0, b0v, dWr[k]);
+        dWr[k] = mfma(a1, b1v, dWr[k]);
+        dWi[k] = mfma(a1, b0v, dWi[k]);
+        dWi[k] = mfma(-a0, b1v, dWi[k]);
         acc_c[k][0] += a0;
         acc_c[k][1] += a1;
       }
@@ -441,11 +454,12 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
 #pragma unroll
       for (int q = 0; q < 16; ++q) gh = mfma(s.w0[crow(q, h) * WS + fj], gu[q], gh);
       wave_sync();
+      // feature gradients, level-major [L][n] (consumed by the table kernels)
       if (valid) {
 #pragma unroll
         for (int mm = 0; mm < 8; ++mm) {
           const int l = 2 * mm + h;
-          if (l < g.n_levels) level_scatter(g, l, x0, x1, make_float2(gh[2 * mm], gh[2 * mm + 1]), dtable);
+          if (l < g.n_levels) gfeat[(int64_t)l * pts.n + p] = make_float2(gh[2 * mm], gh[2 * mm + 1]);
         }
       }
     }
@@ -467,8 +481,8 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
 #pragma unroll
         for (int k = 0; k < H; ++k) {
           // block (mb, nb): rows 32mb + o, cols 32nb + j
-          const float dar = dW[k][0][v] + dW[k][3][v];
-          const float dai = dW[k][2][v] - dW[k][1][v];
+          const float dar = dWr[k][v];
+          const float dai = dWi[k][v];
           red[GL::A(k) + 2 * (o * 32 + j)] += dar;
           red[GL::A(k) + 2 * (o * 32 + j) + 1] += dai;
         }
@@ -500,14 +514,30 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
   for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
 }
 
-// grad += sum over workgroups of the partials (fixed order).
+// Two-stage fixed-order reduction of the workgroup partials:
+// stage 1: partial2[c][t] = sum of rows [c*R, (c+1)*R) of partial[.][t];
+// stage 2: grad[t] += sum over c of partial2[c][t].
+constexpr int kReduceChunks = 32;
+
+__global__ void k_inr_grad_reduce1(const float* __restrict__ partial, int nblocks, int total,
+                                   float* __restrict__ partial2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (t >= total) return;
+  const int R = (nblocks + kReduceChunks - 1) / kReduceChunks;
+  const int b0 = c * R, b1 = min(nblocks, b0 + R);
+  float acc = 0.f;
+  for (int b = b0; b < b1; ++b) acc += partial[(int64_t)b * total + t];
+  partial2[(int64_t)c * total + t] = acc;
+}
+
 template <int H>
-__global__ void k_inr_grad_reduce(const float* __restrict__ partial, int nblocks, PtyMlp gm) {
+__global__ void k_inr_grad_reduce(const float* __restrict__ partial2, PtyMlp gm) {
   using GL = GradLayout<H>;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= GL::total) return;
   float acc = 0.f;
-  for (int b = 0; b < nblocks; ++b) acc += partial[(int64_t)b * GL::total + t];
+  for (int c = 0; c < kReduceChunks; ++c) acc += partial2[(int64_t)c * GL::total + t];
   if (t < GL::b0) {
     const int o = t >> 5, f = t & 31;
     if (f < gm.n_in) gm.w0[o * gm.n_in + f] += acc;
@@ -590,11 +620,20 @@ extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* tab
   return check_launch("inr_fwd");
 }
 
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static size_t grad_total(const PtyMlp* mlp) {
+  return (mlp->n_hidden == 1) ? GradLayout<1>::total : GradLayout<2>::total;
+}
+
+static size_t partial_bytes(const PtyMlp* mlp, const PtyPoints* pts) {
+  return align256(sizeof(float) * grad_total(mlp) * (size_t)(bwd_blocks(pts->n) + kReduceChunks));
+}
+
 extern "C" size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts) {
-  if (!mlp || !pts) return 0;
-  const int H = mlp->n_hidden;
-  const size_t total = (H == 1) ? GradLayout<1>::total : GradLayout<2>::total;
-  return sizeof(float) * total * (size_t)bwd_blocks(pts->n);
+  if (!g || !mlp || !pts) return 0;
+  return partial_bytes(mlp, pts) + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n) +
+         table_bwd_workspace(*g, *pts);
 }
 
 extern "C" int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table, float* dtable,
@@ -611,19 +650,29 @@ extern "C" int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* gr
   if (pts->n == 0) return PTY_OK;
   PTY_REQUIRE(work && work_bytes >= pty_inr_bwd_workspace(g, mlp, pts), PTY_EWORK, "inr_bwd: workspace too small");
   const int blocks = bwd_blocks(pts->n);
-  float* partial = (float*)work;
-  if (mlp->n_hidden == 1) {
-    hipLaunchKernelGGL(k_inr_bwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, dtable,
+  char* w = (char*)work;
+  float* partial = (float*)w;
+  float2* gfeat = (float2*)(w + partial_bytes(mlp, pts));
+  char* tab_work = (char*)gfeat + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n);
+  const int total = (int)grad_total(mlp);
+  float* partial2 = partial + (size_t)blocks * total;
+  if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_bwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, gfeat,
                        *pts, (const float2*)dout, partial);
-    hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((GradLayout<1>::total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)partial, blocks, *grad);
-  } else {
-    hipLaunchKernelGGL(k_inr_bwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, dtable,
+  else
+    hipLaunchKernelGGL(k_inr_bwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, gfeat,
                        *pts, (const float2*)dout, partial);
-    hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((GradLayout<2>::total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)partial, blocks, *grad);
-  }
-  return check_launch("inr_bwd");
+  hipLaunchKernelGGL(k_inr_grad_reduce1, dim3((total + 255) / 256, kReduceChunks), dim3(256), 0, stream,
+                     (const float*)partial, blocks, total, partial2);
+  if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((total + 255) / 256), dim3(256), 0, stream,
+                       (const float*)partial2, *grad);
+  else
+    hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((total + 255) / 256), dim3(256), 0, stream,
+                       (const float*)partial2, *grad);
+  rc = check_launch("inr_bwd");
+  if (rc) return rc;
+  return launch_table_bwd(*g, *pts, gfeat, dtable, tab_work, stream);
 }
 
 extern "C" int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream) {

@@ -30,6 +30,10 @@ def _real_numel(p: torch.Tensor) -> int:
     return p.numel() * (2 if p.is_complex() else 1)
 
 
+def _padded(n: int) -> int:
+    return (n + 3) // 4 * 4
+
+
 class FlatParams:
     """All kernel-visible parameters of the model packed into one fp32 buffer.
 
@@ -43,9 +47,11 @@ class FlatParams:
         train = [p for p in params if p.requires_grad]
         frozen = [p for p in params if not p.requires_grad]
         order = train + frozen
-        total = sum(_real_numel(p) for p in order)
-        self.n_train = sum(_real_numel(p) for p in train)
-        self.data = torch.empty(total, dtype=torch.float32, device=device)
+        # every parameter starts on a 16-byte boundary (complex views need an
+        # even offset; padding entries keep zero gradients)
+        total = sum(_padded(_real_numel(p)) for p in order)
+        self.n_train = sum(_padded(_real_numel(p)) for p in train)
+        self.data = torch.zeros(total, dtype=torch.float32, device=device)
         self.grad = torch.zeros(total, dtype=torch.float32, device=device)
         self.grad_views = {}
         off = 0
@@ -63,7 +69,7 @@ class FlatParams:
                 gview = gv.view(p.shape)
             p.data = pview
             self.grad_views[id(p)] = gview
-            off += n
+            off += _padded(n)
 
     def g(self, p):
         return self.grad_views[id(p)]
@@ -160,6 +166,28 @@ class FusedReconstruction:
                 d = far_descriptor(M, N, H, W, b, self.frames, loss_type)
                 self.far[b] = (d, _lib.workspace(q("pty_farfield_workspace", d), dev))
         self.epoch = 0
+        self._timers = None
+
+    # ------------------------------------------------------------------ timing
+    def enable_timing(self, names):
+        """Record HIP events (on the launch stream) around the named launches
+        ('obj_inr_bwd', 'obj_inr_fwd', 'prb_inr_bwd', 'farfield', 'upsample_adj', ...)."""
+        self._timers = {n: [] for n in names}
+
+    def _mark(self, name, begin):
+        if self._timers is None or name not in self._timers:
+            return None
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        if begin:
+            self._timers[name].append([ev, None])
+        else:
+            self._timers[name][-1][1] = ev
+        return ev
+
+    def timings_ms(self):
+        """{name: [ms per recorded launch]} (call after synchronising)."""
+        return {n: [a.elapsed_time(b) for a, b in v] for n, v in (self._timers or {}).items()}
 
     # ------------------------------------------------------------------ step
     def set_lr(self, lr: float):
@@ -169,8 +197,10 @@ class FusedReconstruction:
         s = _lib.stream_ptr(self.device)
         C = _lib.call
         m = self.model
+        self._mark("obj_inr_fwd", True)
         C("pty_inr_fwd", self.obj_grid, self.obj_mlp, m.xy_hash_encoder.params.data_ptr(), self.obj_pdesc,
           self.obj_small.data_ptr(), s)
+        self._mark("obj_inr_fwd", False)
         C("pty_upsample_fwd", self.obj_small.data_ptr(), self.n_small, self.ds, self.obj_up.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
         if m.has_object_residual:
@@ -194,9 +224,11 @@ class FusedReconstruction:
         C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
         self.forward_fields()
         C("pty_fill", self.dobj.data_ptr(), 2 * self.dobj.numel(), 0.0, s)
+        self._mark("farfield", True)
         C("pty_farfield_fwd_bwd", d, self.obj.data_ptr(), self.probe_n.data_ptr(), self.stack.data_ptr(),
           batch_idx.data_ptr(), self.tl.data_ptr(), self.loss.data_ptr(), self.dobj.data_ptr(),
           self.dprobe_n.data_ptr(), ws_far.data_ptr(), ws_far.numel(), s)
+        self._mark("farfield", False)
         C("pty_probe_normalize_bwd", self.probe.data_ptr(), self.dprobe_n.data_ptr(), self.M, self.N,
           self.stats.data_ptr(), self.dprobe.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
         if m.has_probe_residual:
@@ -209,9 +241,11 @@ class FusedReconstruction:
               self.ws_res.data_ptr(), self.ws_res.numel(), s)
         C("pty_upsample_adj", self.dobj_up.data_ptr(), self.n_small, self.ds, self.dsmall.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
+        self._mark("obj_inr_bwd", True)
         C("pty_inr_bwd", self.obj_grid, self.obj_mlp, self.obj_gmlp, m.xy_hash_encoder.params.data_ptr(),
           self.obj_dtable.data_ptr(), self.obj_pdesc, self.dsmall.data_ptr(), self.ws_obj.data_ptr(),
           self.ws_obj.numel(), s)
+        self._mark("obj_inr_bwd", False)
         C("pty_inr_bwd", self.prb_grid, self.prb_mlp, self.prb_gmlp, m.probe_hash_encoder.params.data_ptr(),
           self.prb_dtable.data_ptr(), self.prb_pdesc, self.dprb_mlp.data_ptr(), self.ws_prb.data_ptr(),
           self.ws_prb.numel(), s)
