@@ -130,8 +130,9 @@ class GridPoints:
         self.ys = torch.linspace(half_dy, 1 - half_dy, height).to(device)
         self.width, self.height = width, height
         self.n = width * height
+        self._plans = {}
 
-    def descriptor(self) -> _lib.PtyPoints:
+    def _raw(self) -> _lib.PtyPoints:
         d = _lib.PtyPoints()
         d.coords = None
         d.axis0 = self.ys.data_ptr()
@@ -139,6 +140,24 @@ class GridPoints:
         d.n_axis0 = self.height
         d.n_axis1 = self.width
         d.n = self.n
+        d.plan = None
+        return d
+
+    def plan(self, spec: GridSpec) -> torch.Tensor:
+        """Device plan of this grid for the table gradient (built once per level table)."""
+        if spec not in self._plans:
+            raw = self._raw()
+            g = spec.descriptor()
+            nbytes = _lib.query("pty_points_plan_bytes", g, raw)
+            buf = _lib.workspace(nbytes, self.xs.device)
+            _lib.call("pty_points_plan_build", g, raw, buf.data_ptr(), buf.numel(), _lib.stream_ptr(self.xs.device))
+            self._plans[spec] = buf
+        return self._plans[spec]
+
+    def descriptor(self, spec: GridSpec | None = None) -> _lib.PtyPoints:
+        d = self._raw()
+        if spec is not None:
+            d.plan = self.plan(spec).data_ptr()
         return d
 
 
@@ -149,10 +168,11 @@ class CoordPoints:
         self.coords = coords.detach().contiguous().float()
         self.n = self.coords.shape[0]
 
-    def descriptor(self) -> _lib.PtyPoints:
+    def descriptor(self, spec: GridSpec | None = None) -> _lib.PtyPoints:
         d = _lib.PtyPoints()
         d.coords = self.coords.data_ptr()
         d.n = self.n
+        d.plan = None
         return d
 
 
@@ -168,7 +188,7 @@ class InrFieldFn(torch.autograd.Function):
         ts = tuple(t.contiguous() for t in ts)
         out = torch.empty(points.n, dtype=torch.complex64, device=table.device)
         _lib.call("pty_inr_fwd", spec.descriptor(), mlp_descriptor(ts, spec.n_output_dims, precision),
-                  table.data_ptr(), points.descriptor(), out.data_ptr(), _lib.stream_ptr(table.device))
+                  table.data_ptr(), points.descriptor(spec), out.data_ptr(), _lib.stream_ptr(table.device))
         ctx.spec, ctx.points, ctx.precision = spec, points, precision
         ctx.save_for_backward(table, *ts)
         return out
@@ -182,7 +202,7 @@ class InrFieldFn(torch.autograd.Function):
         dtable = torch.zeros_like(table)
         desc = mlp_descriptor(ts, spec.n_output_dims, precision)
         gdesc = mlp_descriptor(grads, spec.n_output_dims, precision)
-        pdesc = points.descriptor()
+        pdesc = points.descriptor(spec)
         nbytes = _lib.query("pty_inr_bwd_workspace", spec.descriptor(), desc, pdesc)
         ws = _lib.workspace(nbytes, table.device)
         _lib.call("pty_inr_bwd", spec.descriptor(), desc, gdesc, table.data_ptr(), dtable.data_ptr(), pdesc,

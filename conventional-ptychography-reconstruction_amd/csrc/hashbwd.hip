@@ -1,33 +1,121 @@
 // Hash-table gradient from per-point feature gradients gfeat[l][p] (float2,
-// level-major), the adjoint of the encoding in the fused INR backward.
+// level-major): the adjoint of the encoding in the fused INR backward.
 //
-// Hashed levels (and every level for explicit coordinates) scatter
-// w_c * gfeat into the 4 corner entries with atomics; those entries are
-// spread over the table so the atomics hardly ever collide.
+// Explicit coordinates: every level scatters w_c * gfeat into its 4 corner
+// entries with atomics.
 //
-// Dense levels on the regular grid of init_scale_grids use the separable
-// structure of the bilinear weights instead of atomics: with
-//   gx(j), fx(j) = cell / fraction of axis0[j],  gy(i), fy(i) of axis1[i],
-// the gradient of virtual corner (cx, cy) (cx, cy in [0, res]) is
-//   sum_{dy} sum_{i: gy(i)=cy-dy} wy(i,dy) * A[i][cx],
-//   A[i][cx] = sum_{dx} sum_{j: gx(j)=cx-dx} wx(j,dx) * gfeat[l][i*n0 + j],
-// w(.,0) = 1 - f, w(.,1) = f.  gx is non-decreasing in j (the axes are
-// increasing and f32 x*s+0.5 is monotone), so each cell's points form a
-// contiguous range found by binary search.  Each virtual corner is summed by
-// one thread in a fixed order and added once to its entry (the only
-// collisions are the edge corners cx = res that wrap onto the next row).
+// Regular grid (init_scale_grids): owner-computes, no atomics, deterministic.
+// With gx(j), fx(j) the cell / fraction of axis0[j] and gy(i), fy(i) those of
+// axis1[i] at level l, the gradient of virtual corner (cx, cy), cx, cy in
+// [0, res], is
+//   G(cx,cy) = sum_{dx,dy} sum_{j: gx(j)=cx-dx} sum_{i: gy(i)=cy-dy} w(fx,dx) w(fy,dy) g[l][i*n0+j]
+// with w(f,0) = 1-f, w(f,1) = f (the bilinear weights are separable).  The
+// cells are monotone in the axis index, so each cell's points are a
+// contiguous index range (plan: starts[c] = first index with cell >= c).
+// One thread owns each table entry e and sums the virtual corners mapping to
+// it:
+//   dense level : local = cx + cy*res (minus size once past the end), so the
+//                 preimages are cx + cy*res in {e, e+size}: at most 4 corners;
+//   hashed level: local = (cx ^ h(cy)) & (T-1), h(cy) = cy*2654435761 mod 2^32,
+//                 so for each cy the corner is cx = (e ^ h(cy)) & (T-1) (+ k T).
+//                 Only rows cy touched by a grid point can contribute; the
+//                 plan buckets them by the bits of h(cy) above log2(res+1),
+//                 which must equal those of e for cx <= res.
+// Dense levels first contract along axis0 (A[i][cx], pass 1) so that the
+// coarse levels (tens of points per cell) stay cheap.
 #include "hashgrid.h"
 
 namespace pty {
 
-struct TableBwdLayout {
-  int n_dense;                  // dense levels handled by the gather path
-  int lev[PTY_MAX_LEVELS];      // level of each dense slot
-  int off_s[PTY_MAX_LEVELS];    // offset of the level's start table (res + 2 ints)
-  int64_t off_a[PTY_MAX_LEVELS + 1];  // offset of the level's A block (n1 * (res + 1) float2)
-  int64_t off_c[PTY_MAX_LEVELS + 1];  // prefix of (res + 1)^2 over dense slots
-  int n_starts;                 // total ints per axis start table
+constexpr uint32_t kPrime = 2654435761u;
+
+// Plan layout, a pure function of (grid, axes), shared by build and use.
+struct PlanLayout {
+  int n0, n1;
+  int off_s[PTY_MAX_LEVELS];    // starts tables: res + 2 ints per level (axis0 and axis1 alike)
+  int n_starts;
+  int bbits[PTY_MAX_LEVELS];    // hashed: bits of cx (2^bbits > res), dense: 0
+  int off_b[PTY_MAX_LEVELS];    // hashed: bucket starts (nb + 1 ints)
+  int off_cy[PTY_MAX_LEVELS];   // hashed: bucketed rows (res + 1 ints capacity)
+  int n_bucket_ints;
+  int n_cy_ints;
+  // byte offsets
+  size_t s0, s1, fx, fy, bstart, bcy, total;
+  // dense pass-1 buffer (workspace, not plan): A[l] of n1 * (res + 1) float2
+  int64_t off_a[PTY_MAX_LEVELS + 1];
 };
+
+static size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int log2_ceil(uint32_t v) {
+  int b = 0;
+  while ((1u << b) < v && b < 31) ++b;
+  return b;
+}
+
+static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
+  PlanLayout L{};
+  L.n0 = pts.n_axis0;
+  L.n1 = pts.n_axis1;
+  int s = 0, bo = 0, co = 0;
+  int64_t a = 0;
+  for (int l = 0; l < g.n_levels; ++l) {
+    L.off_s[l] = s;
+    s += (int)g.res[l] + 2;
+    const bool dense = (g.dense_mask >> l) & 1u;
+    L.off_a[l] = a;
+    if (dense) {
+      L.bbits[l] = 0;
+      L.off_b[l] = bo;
+      L.off_cy[l] = co;
+      a += (int64_t)L.n1 * (g.res[l] + 1);
+    } else {
+      const int log2T = g.log2_hashmap_size;
+      int b = log2_ceil(g.res[l] + 1);
+      if (b > log2T) b = log2T;
+      L.bbits[l] = b;
+      const int nb = 1 << (log2T - b);
+      L.off_b[l] = bo;
+      bo += nb + 1;
+      L.off_cy[l] = co;
+      co += (int)g.res[l] + 1;
+    }
+  }
+  L.off_a[g.n_levels] = a;
+  L.n_starts = s;
+  L.n_bucket_ints = bo;
+  L.n_cy_ints = co;
+  size_t off = 0;
+  L.s0 = off; off += a256(sizeof(int) * (size_t)s);
+  L.s1 = off; off += a256(sizeof(int) * (size_t)s);
+  L.fx = off; off += a256(sizeof(float) * (size_t)g.n_levels * L.n0);
+  L.fy = off; off += a256(sizeof(float) * (size_t)g.n_levels * L.n1);
+  L.bstart = off; off += a256(sizeof(int) * (size_t)(bo > 0 ? bo : 1));
+  L.bcy = off; off += a256(sizeof(int) * (size_t)(co > 0 ? co : 1));
+  L.total = off;
+  return L;
+}
+
+struct PlanPtrs {
+  const int* s0;
+  const int* s1;
+  const float* fx;
+  const float* fy;
+  const int* bstart;
+  const int* bcy;
+};
+
+static PlanPtrs plan_ptrs(const PlanLayout& L, const void* plan) {
+  const char* p = (const char*)plan;
+  PlanPtrs P;
+  P.s0 = (const int*)(p + L.s0);
+  P.s1 = (const int*)(p + L.s1);
+  P.fx = (const float*)(p + L.fx);
+  P.fy = (const float*)(p + L.fy);
+  P.bstart = (const int*)(p + L.bstart);
+  P.bcy = (const int*)(p + L.bcy);
+  return P;
+}
 
 __device__ __forceinline__ uint32_t axis_cell(const PtyGrid& g, int l, float x, float& frac) {
 #pragma clang fp contract(off)
@@ -37,48 +125,104 @@ __device__ __forceinline__ uint32_t axis_cell(const PtyGrid& g, int l, float x, 
   return (uint32_t)(int)f;
 }
 
-// starts[off_s[d] + c] = first index j with cell(axis[j]) >= c, c in [0, res + 2)
-__global__ void k_axis_starts(PtyGrid g, TableBwdLayout L, const float* __restrict__ axis, int n,
-                              int* __restrict__ starts) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.n_starts) return;
-  int d = 0;
-  while (d + 1 < L.n_dense && t >= L.off_s[d + 1]) ++d;
-  const int l = L.lev[d];
-  const int c = t - L.off_s[d];
-  int lo = 0, hi = n;  // first j in [0, n) with cell >= c, else n
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    float fr;
-    if ((int)axis_cell(g, l, axis[mid], fr) >= c) hi = mid; else lo = mid + 1;
-  }
-  starts[t] = lo;
+__device__ __forceinline__ int level_of_starts(const PlanLayout& L, int n_levels, int t) {
+  int l = 0;
+  while (l + 1 < n_levels && t >= L.off_s[l + 1]) ++l;
+  return l;
 }
 
-// A[i][cx] for every dense slot
-__global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, TableBwdLayout L, PtyPoints pts,
-                                                     const float2* __restrict__ gfeat, const int* __restrict__ s0,
-                                                     float2* __restrict__ A) {
+// ------------------------------------------------------------------ plan build
+// starts (both axes) and fractions (both axes)
+__global__ void k_plan_axes(PtyGrid g, PlanLayout L, const float* __restrict__ a0, const float* __restrict__ a1,
+                            int* __restrict__ s0, int* __restrict__ s1, float* __restrict__ fx,
+                            float* __restrict__ fy) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < L.n_starts) {
+    const int l = level_of_starts(L, g.n_levels, t);
+    const int c = t - L.off_s[l];
+    for (int ax = 0; ax < 2; ++ax) {
+      const float* a = ax ? a1 : a0;
+      const int n = ax ? L.n1 : L.n0;
+      int lo = 0, hi = n;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        float fr;
+        if ((int)axis_cell(g, l, a[mid], fr) >= c) hi = mid; else lo = mid + 1;
+      }
+      (ax ? s1 : s0)[t] = lo;
+    }
+  }
+  const int u = t;
+  if (u < g.n_levels * L.n0) {
+    const int l = u / L.n0, j = u - l * L.n0;
+    float fr;
+    axis_cell(g, l, a0[j], fr);
+    fx[u] = fr;
+  }
+  if (u < g.n_levels * L.n1) {
+    const int l = u / L.n1, i = u - l * L.n1;
+    float fr;
+    axis_cell(g, l, a1[i], fr);
+    fy[u] = fr;
+  }
+}
+
+// bucketed rows of each hashed level: rows cy in [0, res] with a point in
+// cell cy-1 or cy, grouped by ((h(cy) & (T-1)) >> bbits), ascending cy.
+// One workgroup per level; counting in LDS-free sequential passes by lane 0
+// (a one-time build).
+__global__ void k_plan_buckets(PtyGrid g, PlanLayout L, const int* __restrict__ s1, int* __restrict__ bstart,
+                               int* __restrict__ bcy) {
+  const int l = blockIdx.x;
+  if (((g.dense_mask >> l) & 1u) || threadIdx.x != 0) return;
+  const int res = (int)g.res[l];
+  const uint32_t T = 1u << g.log2_hashmap_size;
+  const int b = L.bbits[l];
+  const int nb = 1 << (g.log2_hashmap_size - b);
+  const int* st = s1 + L.off_s[l];
+  int* bs = bstart + L.off_b[l];
+  int* out = bcy + L.off_cy[l];
+  for (int k = 0; k <= nb; ++k) bs[k] = 0;
+  for (int cy = 0; cy <= res; ++cy) {
+    const bool touched = (st[cy + 1] > st[cy]) || (cy > 0 && st[cy] > st[cy - 1]);
+    if (!touched) continue;
+    const uint32_t key = ((cy * kPrime) & (T - 1u)) >> b;
+    bs[key + 1] += 1;
+  }
+  for (int k = 0; k < nb; ++k) bs[k + 1] += bs[k];
+  // fill (bs[k] used as a cursor, then restored)
+  for (int cy = 0; cy <= res; ++cy) {
+    const bool touched = (st[cy + 1] > st[cy]) || (cy > 0 && st[cy] > st[cy - 1]);
+    if (!touched) continue;
+    const uint32_t key = ((cy * kPrime) & (T - 1u)) >> b;
+    out[bs[key]++] = cy;
+  }
+  for (int k = nb; k > 0; --k) bs[k] = bs[k - 1];
+  bs[0] = 0;
+}
+
+// ------------------------------------------------------------------ gradient
+// Dense pass 1: A[l][i][cx] = sum_dx sum_{j in cell cx-dx} w(fx_j, dx) g[l][i n0 + j]
+__global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
+                                                     const float2* __restrict__ gfeat, float2* __restrict__ A) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.off_a[L.n_dense]) return;
-  int d = 0;
-  while (d + 1 < L.n_dense && t >= L.off_a[d + 1]) ++d;
-  const int l = L.lev[d];
+  if (t >= L.off_a[g.n_levels]) return;
+  int l = 0;
+  while (l + 1 < g.n_levels && t >= L.off_a[l + 1]) ++l;
   const int res1 = (int)g.res[l] + 1;
-  const int64_t r = t - L.off_a[d];
+  const int64_t r = t - L.off_a[l];
   const int i = (int)(r / res1), cx = (int)(r - (int64_t)i * res1);
-  const int* st = s0 + L.off_s[d];
-  const float2* gl = gfeat + (int64_t)l * pts.n + (int64_t)i * pts.n_axis0;
+  const int* st = P.s0 + L.off_s[l];
+  const float* fxl = P.fx + (int64_t)l * L.n0;
+  const float2* gl = gfeat + (int64_t)l * pts.n + (int64_t)i * L.n0;
   float ax = 0.f, ay = 0.f;
 #pragma unroll
   for (int dx = 0; dx < 2; ++dx) {
     const int cell = cx - dx;
     if (cell < 0) continue;
-    const int j0 = st[cell], j1 = st[cell + 1];
-    for (int j = j0; j < j1; ++j) {
-      float fr;
-      axis_cell(g, l, pts.axis0[j], fr);
-      const float w = dx ? fr : 1.f - fr;
+    const int j1 = st[cell + 1];
+    for (int j = st[cell]; j < j1; ++j) {
+      const float w = dx ? fxl[j] : 1.f - fxl[j];
       const float2 v = gl[j];
       ax += w * v.x;
       ay += w * v.y;
@@ -87,49 +231,120 @@ __global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, TableBwdLayout L
   A[t] = make_float2(ax, ay);
 }
 
-// corner sums and one add per virtual corner
-__global__ __launch_bounds__(256) void k_dense_pass2(PtyGrid g, TableBwdLayout L, PtyPoints pts,
-                                                     const int* __restrict__ s1, const float2* __restrict__ A,
-                                                     float* __restrict__ dtable) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.off_c[L.n_dense]) return;
-  int d = 0;
-  while (d + 1 < L.n_dense && t >= L.off_c[d + 1]) ++d;
-  const int l = L.lev[d];
-  const int res1 = (int)g.res[l] + 1;
-  const int64_t r = t - L.off_c[d];
-  const int cy = (int)(r / res1), cx = (int)(r - (int64_t)cy * res1);
-  const int* st = s1 + L.off_s[d];
-  const float2* Al = A + L.off_a[d];
+// G(cx, cy) from A (dense levels)
+__device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanPtrs& P, int l, int res,
+                                                const float2* __restrict__ Al, int cx, int cy) {
+  const int* st = P.s1 + L.off_s[l];
+  const float* fyl = P.fy + (int64_t)l * L.n1;
   float ax = 0.f, ay = 0.f;
 #pragma unroll
   for (int dy = 0; dy < 2; ++dy) {
     const int cell = cy - dy;
-    if (cell < 0) continue;
-    const int i0 = st[cell], i1 = st[cell + 1];
-    for (int i = i0; i < i1; ++i) {
-      float fr;
-      axis_cell(g, l, pts.axis1[i], fr);
-      const float w = dy ? fr : 1.f - fr;
-      const float2 v = Al[(int64_t)i * res1 + cx];
+    if (cell < 0 || cell > res) continue;
+    const int i1 = st[cell + 1];
+    for (int i = st[cell]; i < i1; ++i) {
+      const float w = dy ? fyl[i] : 1.f - fyl[i];
+      const float2 v = Al[(int64_t)i * (res + 1) + cx];
       ax += w * v.x;
       ay += w * v.y;
     }
   }
-  if (ax != 0.f || ay != 0.f) {
-    const uint32_t e = level_index(g, l, (uint32_t)cx, (uint32_t)cy);
-    atomic_add_f32(dtable + 2 * e, ax);
-    atomic_add_f32(dtable + 2 * e + 1, ay);
+  return make_float2(ax, ay);
+}
+
+// G(cx, cy) directly from the point gradients (hashed levels: few points per cell)
+__device__ __forceinline__ float2 corner_direct(const PlanLayout& L, const PlanPtrs& P, int l, int res,
+                                                const float2* __restrict__ gl, int cx, int cy) {
+  const int* s0 = P.s0 + L.off_s[l];
+  const int* s1 = P.s1 + L.off_s[l];
+  const float* fxl = P.fx + (int64_t)l * L.n0;
+  const float* fyl = P.fy + (int64_t)l * L.n1;
+  float ax = 0.f, ay = 0.f;
+#pragma unroll
+  for (int dx = 0; dx < 2; ++dx) {
+    const int cxx = cx - dx;
+    if (cxx < 0 || cxx > res) continue;
+    const int j0 = s0[cxx], j1 = s0[cxx + 1];
+    if (j0 == j1) continue;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+      const int cyy = cy - dy;
+      if (cyy < 0 || cyy > res) continue;
+      const int i1 = s1[cyy + 1];
+      for (int i = s1[cyy]; i < i1; ++i) {
+        const float wy = dy ? fyl[i] : 1.f - fyl[i];
+        for (int j = j0; j < j1; ++j) {
+          const float w = (dx ? fxl[j] : 1.f - fxl[j]) * wy;
+          const float2 v = gl[(int64_t)i * L.n0 + j];
+          ax += w * v.x;
+          ay += w * v.y;
+        }
+      }
+    }
+  }
+  return make_float2(ax, ay);
+}
+
+// one thread per table entry
+__global__ __launch_bounds__(256) void k_table_owner(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
+                                                     const float2* __restrict__ gfeat,
+                                                     const float2* __restrict__ A, float2* __restrict__ dtable) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n_entries) return;
+  int l = 0;
+  while (l + 1 < g.n_levels && t >= g.offset[l + 1]) ++l;
+  const uint32_t e = t - g.offset[l];
+  const int res = (int)g.res[l];
+  const uint32_t size = g.size[l];
+  float2 acc = make_float2(0.f, 0.f);
+  if ((g.dense_mask >> l) & 1u) {
+    const float2* Al = A + L.off_a[l];
+    const int64_t top = (int64_t)res * res + res;  // largest cx + cy*res
+    for (int k = 0; k < 2; ++k) {
+      const int64_t target = (int64_t)e + (int64_t)k * size;
+      if (target > top) break;
+      // cy with cx = target - cy*res in [0, res]
+      int64_t cy_lo = target > res ? (target - res + res - 1) / res : 0;
+      const int64_t cy_hi = target / res;
+      for (int64_t cy = cy_lo; cy <= cy_hi && cy <= res; ++cy) {
+        const int cx = (int)(target - cy * res);
+        const float2 v = corner_from_A(L, P, l, res, Al, cx, (int)cy);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+    }
+  } else {
+    const uint32_t T = 1u << g.log2_hashmap_size;
+    const int b = L.bbits[l];
+    const uint32_t key = e >> b;
+    const int* bs = P.bstart + L.off_b[l];
+    const int* rows = P.bcy + L.off_cy[l];
+    const float2* gl = gfeat + (int64_t)l * pts.n;
+    const int r1 = bs[key + 1];
+    for (int r = bs[key]; r < r1; ++r) {
+      const int cy = rows[r];
+      uint32_t cx = (e ^ ((uint32_t)cy * kPrime)) & (T - 1u);
+      for (; cx <= (uint32_t)res; cx += T) {
+        const float2 v = corner_direct(L, P, l, res, gl, (int)cx, cy);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+    }
+  }
+  if (acc.x != 0.f || acc.y != 0.f) {
+    float2 d = dtable[t];
+    d.x += acc.x;
+    d.y += acc.y;
+    dtable[t] = d;
   }
 }
 
-// scatter for the levels in `mask`
-__global__ __launch_bounds__(256) void k_level_scatter(PtyGrid g, uint32_t mask, PtyPoints pts,
-                                                       const float2* __restrict__ gfeat,
+// explicit coordinates: scatter with atomics
+__global__ __launch_bounds__(256) void k_level_scatter(PtyGrid g, PtyPoints pts, const float2* __restrict__ gfeat,
                                                        float* __restrict__ dtable) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int l = blockIdx.y;
-  if (p >= pts.n || !((mask >> l) & 1u)) return;
+  if (p >= pts.n) return;
   float x0, x1;
   point_coords(pts, p, x0, x1);
   const float2 dy = gfeat[(int64_t)l * pts.n + p];
@@ -137,60 +352,72 @@ __global__ __launch_bounds__(256) void k_level_scatter(PtyGrid g, uint32_t mask,
   level_scatter(g, l, x0, x1, dy, dtable);
 }
 
-static TableBwdLayout table_layout(const PtyGrid& g, const PtyPoints& pts) {
-  TableBwdLayout L{};
-  L.n_dense = 0;
-  if (!pts.coords) {
-    int s = 0;
-    int64_t a = 0, c = 0;
-    for (int l = 0; l < g.n_levels; ++l) {
-      if (!((g.dense_mask >> l) & 1u)) continue;
-      const int d = L.n_dense++;
-      L.lev[d] = l;
-      L.off_s[d] = s;
-      L.off_a[d] = a;
-      L.off_c[d] = c;
-      s += (int)g.res[l] + 2;
-      a += (int64_t)pts.n_axis1 * (g.res[l] + 1);
-      c += (int64_t)(g.res[l] + 1) * (g.res[l] + 1);
-    }
-    L.n_starts = s;
-    L.off_a[L.n_dense] = a;
-    L.off_c[L.n_dense] = c;
-  }
-  return L;
+size_t points_plan_bytes(const PtyGrid& g, const PtyPoints& pts) {
+  if (pts.coords) return 0;
+  return plan_layout(g, pts).total;
+}
+
+int build_points_plan(const PtyGrid& g, const PtyPoints& pts, void* plan, hipStream_t stream) {
+  const PlanLayout L = plan_layout(g, pts);
+  char* p = (char*)plan;
+  const int nax = L.n0 > L.n1 ? L.n0 : L.n1;
+  const int nt = L.n_starts > g.n_levels * nax ? L.n_starts : g.n_levels * nax;
+  hipLaunchKernelGGL(k_plan_axes, dim3((nt + 255) / 256), dim3(256), 0, stream, g, L, pts.axis0, pts.axis1,
+                     (int*)(p + L.s0), (int*)(p + L.s1), (float*)(p + L.fx), (float*)(p + L.fy));
+  hipLaunchKernelGGL(k_plan_buckets, dim3(g.n_levels), dim3(64), 0, stream, g, L, (const int*)(p + L.s1),
+                     (int*)(p + L.bstart), (int*)(p + L.bcy));
+  return check_launch("points_plan_build");
 }
 
 size_t table_bwd_workspace(const PtyGrid& g, const PtyPoints& pts) {
-  const TableBwdLayout L = table_layout(g, pts);
-  if (L.n_dense == 0) return 0;
-  return sizeof(int) * 2 * (size_t)L.n_starts + 256 + sizeof(float2) * (size_t)L.off_a[L.n_dense];
+  if (pts.coords) return 0;
+  const PlanLayout L = plan_layout(g, pts);
+  size_t bytes = a256(sizeof(float2) * (size_t)L.off_a[g.n_levels]);
+  if (!pts.plan) bytes += L.total;
+  return bytes;
 }
 
 int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat, float* dtable, char* work,
                      hipStream_t stream) {
-  const TableBwdLayout L = table_layout(g, pts);
-  uint32_t scatter_mask = (g.n_levels >= 32) ? 0xFFFFFFFFu : ((1u << g.n_levels) - 1u);
-  if (L.n_dense > 0) {
-    int* s0 = (int*)work;
-    int* s1 = s0 + L.n_starts;
-    float2* A = (float2*)(work + (((sizeof(int) * 2 * (size_t)L.n_starts) + 255) & ~size_t(255)));
-    hipLaunchKernelGGL(k_axis_starts, dim3((L.n_starts + 255) / 256), dim3(256), 0, stream, g, L, pts.axis0,
-                       pts.n_axis0, s0);
-    hipLaunchKernelGGL(k_axis_starts, dim3((L.n_starts + 255) / 256), dim3(256), 0, stream, g, L, pts.axis1,
-                       pts.n_axis1, s1);
-    const int64_t na = L.off_a[L.n_dense], nc = L.off_c[L.n_dense];
-    hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, gfeat,
-                       (const int*)s0, A);
-    hipLaunchKernelGGL(k_dense_pass2, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, stream, g, L, pts,
-                       (const int*)s1, (const float2*)A, dtable);
-    for (int d = 0; d < L.n_dense; ++d) scatter_mask &= ~(1u << L.lev[d]);
-  }
-  if (scatter_mask) {
+  if (pts.coords) {
     hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)((pts.n + 255) / 256), g.n_levels), dim3(256), 0, stream, g,
-                       scatter_mask, pts, gfeat, dtable);
+                       pts, gfeat, dtable);
+    return check_launch("table_bwd_scatter");
   }
+  const PlanLayout L = plan_layout(g, pts);
+  float2* A = (float2*)work;
+  const void* plan = pts.plan;
+  if (!plan) {
+    char* pw = work + a256(sizeof(float2) * (size_t)L.off_a[g.n_levels]);
+    int rc = build_points_plan(g, pts, pw, stream);
+    if (rc) return rc;
+    plan = pw;
+  }
+  const PlanPtrs P = plan_ptrs(L, plan);
+  const int64_t na = L.off_a[g.n_levels];
+  if (na > 0)
+    hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, P, gfeat,
+                       A);
+  hipLaunchKernelGGL(k_table_owner, dim3((g.n_entries + 255) / 256), dim3(256), 0, stream, g, L, pts, P, gfeat,
+                     (const float2*)A, (float2*)dtable);
   return check_launch("table_bwd");
 }
 
 }  // namespace pty
+
+using namespace pty;
+
+extern "C" size_t pty_points_plan_bytes(const PtyGrid* g, const PtyPoints* pts) {
+  if (!g || !pts) return 0;
+  return points_plan_bytes(*g, *pts);
+}
+
+extern "C" int pty_points_plan_build(const PtyGrid* g, const PtyPoints* pts, void* plan, size_t plan_bytes,
+                                     hipStream_t stream) {
+  int rc = validate_grid(g);
+  if (rc) return rc;
+  PTY_REQUIRE(pts && !pts->coords && pts->axis0 && pts->axis1 && pts->n_axis0 > 0 && pts->n_axis1 > 0, PTY_EINVAL,
+              "points_plan_build: needs a regular grid (axis0/axis1)");
+  PTY_REQUIRE(plan && plan_bytes >= points_plan_bytes(*g, *pts), PTY_EWORK, "points_plan_build: buffer too small");
+  return build_points_plan(*g, *pts, plan, stream);
+}

@@ -15,6 +15,7 @@ namespace pty {
 
 int validate_grid(const PtyGrid* g);
 // table gradient from level-major feature gradients gfeat[l][p] (hashbwd.hip)
+size_t points_plan_bytes(const PtyGrid& g, const PtyPoints& pts);
 size_t table_bwd_workspace(const PtyGrid& g, const PtyPoints& pts);
 int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat, float* dtable, char* work,
                      hipStream_t stream);

@@ -126,8 +126,8 @@ class FusedReconstruction:
         self.prb_grid = self.prb_spec.descriptor()
         self.obj_pts = model.grid_points("object")
         self.prb_pts = model.grid_points("probe")
-        self.obj_pdesc = self.obj_pts.descriptor()
-        self.prb_pdesc = self.prb_pts.descriptor()
+        self.obj_pdesc = self.obj_pts.descriptor(self.obj_spec)
+        self.prb_pdesc = self.prb_pts.descriptor(self.prb_spec)
         self.obj_mlp = mlp_descriptor(self.obj_ts, nin, prec)
         self.prb_mlp = mlp_descriptor(self.prb_ts, nin, prec)
         self.obj_gmlp = mlp_descriptor([self.flat.g(t) for t in self.obj_ts], nin, prec)

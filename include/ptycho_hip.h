@@ -31,7 +31,7 @@ extern "C" {
 
 typedef struct ihipStream_t* hipStream_t;
 
-#define PTY_ABI_VERSION 1
+#define PTY_ABI_VERSION 2
 #define PTY_MAX_LEVELS 16
 #define PTY_MAX_HIDDEN 4
 
@@ -99,6 +99,10 @@ typedef struct PtyPoints {
   int32_t n_axis0;
   int32_t n_axis1;
   int64_t n;
+  /* Regular grid only: device plan built once by pty_points_plan_build for
+   * this (grid, axes) pair (cell ranges, fractions, hash buckets); NULL makes
+   * pty_inr_bwd build it in its workspace on every call. */
+  const void* plan;
 } PtyPoints;
 
 /* Far-field batch descriptor (Recovery.py:287-307). */
@@ -136,6 +140,11 @@ size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoint
 int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table,
                 float* dtable, const PtyPoints* pts, const float* dout, void* work,
                 size_t work_bytes, hipStream_t stream);
+/* Static plan of a regular grid for the table gradient (no counterpart in
+ * the reference: it only depends on the hash-grid layout and the axes). */
+size_t pty_points_plan_bytes(const PtyGrid* g, const PtyPoints* pts);
+int pty_points_plan_build(const PtyGrid* g, const PtyPoints* pts, void* plan, size_t plan_bytes,
+                          hipStream_t stream);
 /* ComplexMLP.net over given features [n, n_in] (complex_inr.py:313). */
 int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream);
 
