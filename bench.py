@@ -281,6 +281,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", choices=["throughput", "nrmse"], default="throughput")
     ap.add_argument("--workload", default="cfg0", help="nrmse mode: cfg0 | cfg1 | small")
@@ -338,6 +339,21 @@ def main():
     n_obj = model.downsampled_width * model.downsampled_height
     achieved = MLP_FLOP_BWD_PER_POINT * n_obj / (bwd_ms * 1e-3) / 1e12
     loss_now = float(eng.loss.sum().item())
+    # the same steps replayed from one captured HIP graph
+    graph_ms = None
+    if not args.no_graph:
+        eng.capture()
+        for _ in range(5):
+            eng.graph_step(next_batch())
+        torch.cuda.synchronize()
+        g0 = torch.cuda.Event(enable_timing=True)
+        g1 = torch.cuda.Event(enable_timing=True)
+        g0.record()
+        for _ in range(args.steps):
+            eng.graph_step(next_batch())
+        g1.record()
+        torch.cuda.synchronize()
+        graph_ms = g0.elapsed_time(g1) / args.steps
 
     line = {
         "metric": "wall-s to plain-final object NRMSE + iters/s: HIP fused vs plain, 1 MI355X",
@@ -356,6 +372,7 @@ def main():
                    "steps_per_iter": 40, "global_batch": BATCH, "positions": frames,
                    "parallelism": "single-gpu"},
         "steps_per_s": steps_per_s,
+        "graph_ms_per_step": graph_ms,
         "wall_s_timed": wall,
         "kernel_ms": {k: (sum(v) / len(v) if v else None) for k, v in tim.items()},
         "loss_last_step": loss_now,

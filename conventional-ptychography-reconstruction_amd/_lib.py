@@ -87,7 +87,7 @@ _SIGNATURES = {
     "pty_farfield_intensity": (c_int32, [P(PtyFar), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_size_t, c_void_p]),
     "pty_adam": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_float, c_float,
-                           c_float, c_void_p]),
+                           c_float, c_int32, c_void_p]),
     "pty_adam_tick": (c_int32, [c_void_p, c_void_p]),
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
 }

@@ -76,24 +76,53 @@ __global__ void k_sum_into(const float* __restrict__ partial, int m, float* __re
 // torch.optim.Adam (amsgrad=False, maximize=False, weight_decay=0) on real
 // views: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
 // p -= lr/bc1 * m / (sqrt(v)/sqrt(bc2) + eps), bc_i = 1 - b_i^step.
-// hyper = [lr, step] on the device; pty_adam_tick advances step.
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, int64_t n, const float* __restrict__ hyper, float b1, float b2,
-                       float eps) {
+// hyper = [lr, step] on the device; pty_adam_tick advances step.  float4
+// lanes over n4 = n/4 groups (the flat buffer is padded to 4 floats); the
+// consumed gradient is cleared for the next step when clear != 0.
+__global__ void k_adam(float4* __restrict__ p, float4* __restrict__ g, float4* __restrict__ m,
+                       float4* __restrict__ v, int64_t n4, const float* __restrict__ hyper, float b1, float b2,
+                       float eps, int clear) {
   const float lr = hyper[0], t = hyper[1];
   const float bc1 = 1.f - powf(b1, t);
   const float sbc2 = sqrtf(1.f - powf(b2, t));
   const float step = lr / bc1;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float gi = g[i];
-    float mi = m[i], vi = v[i];
-    mi = b1 * mi + (1.f - b1) * gi;
-    vi = b2 * vi + (1.f - b2) * gi * gi;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 gi = g[i];
+    float4 mi = m[i], vi = v[i], pi = p[i];
+    float* gp = (float*)&gi;
+    float* mp = (float*)&mi;
+    float* vp = (float*)&vi;
+    float* pp = (float*)&pi;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      mp[k] = b1 * mp[k] + (1.f - b1) * gp[k];
+      vp[k] = b2 * vp[k] + (1.f - b2) * gp[k] * gp[k];
+      const float denom = sqrtf(vp[k]) / sbc2 + eps;
+      pp[k] = pp[k] - step * (mp[k] / denom);
+    }
     m[i] = mi;
     v[i] = vi;
-    const float denom = sqrtf(vi) / sbc2 + eps;
-    p[i] = p[i] - step * (mi / denom);
+    p[i] = pi;
+    if (clear) g[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
+}
+
+// scalar tail (n not a multiple of 4)
+__global__ void k_adam_tail(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int64_t i0, int64_t n, const float* __restrict__ hyper, float b1,
+                            float b2, float eps, int clear) {
+  const float lr = hyper[0], t = hyper[1];
+  const float bc1 = 1.f - powf(b1, t);
+  const float sbc2 = sqrtf(1.f - powf(b2, t));
+  const int64_t i = i0 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] = p[i] - (lr / bc1) * (mi / (sqrtf(vi) / sbc2 + eps));
+  if (clear) g[i] = 0.f;
 }
 
 // [M, N, F] -> [F, M, N] with ifftshift over (M, N): out[f, k0, k1] =
@@ -162,12 +191,20 @@ extern "C" int pty_residual_bwd(const float* k, const float* x, const float* gou
   return check_launch("residual_bwd");
 }
 
-extern "C" int pty_adam(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
-                        const float* hyper, float beta1, float beta2, float eps, hipStream_t stream) {
+extern "C" int pty_adam(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                        const float* hyper, float beta1, float beta2, float eps, int32_t clear_grad,
+                        hipStream_t stream) {
   PTY_REQUIRE(param && grad && exp_avg && exp_avg_sq && hyper && n >= 0, PTY_EINVAL, "adam: bad arguments");
   if (n == 0) return PTY_OK;
-  hipLaunchKernelGGL(k_adam, dim3(grid_for(n, 256)), dim3(256), 0, stream, param, grad, exp_avg, exp_avg_sq, n,
-                     hyper, beta1, beta2, eps);
+  const bool aligned = ((((uintptr_t)param) | ((uintptr_t)grad) | ((uintptr_t)exp_avg) | ((uintptr_t)exp_avg_sq)) &
+                        15) == 0;
+  const int64_t n4 = aligned ? n / 4 : 0;
+  if (n4 > 0)
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n4, 256)), dim3(256), 0, stream, (float4*)param, (float4*)grad,
+                       (float4*)exp_avg, (float4*)exp_avg_sq, n4, hyper, beta1, beta2, eps, (int)clear_grad);
+  for (int64_t i0 = n4 * 4; i0 < n; i0 += 1024)
+    hipLaunchKernelGGL(k_adam_tail, dim3(1), dim3(1024), 0, stream, param, grad, exp_avg, exp_avg_sq, i0, n, hyper,
+                       beta1, beta2, eps, (int)clear_grad);
   return check_launch("adam");
 }
 

@@ -14,8 +14,10 @@
 //   F1 rows  : Psi1[b, r, :] = FFT_N(probe_n[r,:] * patch[r,:])
 //   F2 cols  : FFT_M; smooth-L1 / Poisson: loss, dI, dPsi, IFFT_M in place
 //              FD: store Psi and I, then FD1 (loss, gx, gy), FD2 (dPsi), F2i (IFFT_M)
-//   F3 rows  : per row r, for every b: IFFT_N, dprobe row accumulation and
-//              atomic scatter of dpatch into dobject
+//   F3 rows  : per (row block, b): dpsi = IFFT_N(dPsi rows), in place
+//   F4       : dprobe = sum_b dpsi_b * conj(patch_b)             (fixed order)
+//   F5       : dobject[px] = sum over windows covering px of dpsi_b * conj(probe_n)
+//              (a per-pixel gather in batch order: no atomics, deterministic)
 #include "common.h"
 
 namespace pty {
@@ -240,45 +242,76 @@ __global__ void k_far_loss_final(FarArgs a) {
   a.loss_out[b] = acc.value() / ((float)a.M * (float)a.N);
 }
 
-// F3: rows, inverse.  One wave per workgroup, `lines` rows r; loops over the
-// batch so that each dprobe row has a single owner.
-__global__ __launch_bounds__(64) void k_far_rows_bwd(FarArgs a, int lines) {
+// F3: rows, inverse.  Workgroup (row block, pattern b): dpsi = IFFT_N of the
+// dPsi rows (unnormalised), written back in place.
+__global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   extern __shared__ float2 smem[];
   const int N = a.N;
   float2* tw = smem;
   float2* x = tw + N;
   float2* y = x + lines * N;
-  float2* acc = y + lines * N;
   const int tid = threadIdx.x, nt = blockDim.x;
+  const int b = blockIdx.y;
+  int ty, tx, p;
+  if (!window_of(a, b, ty, tx, p)) return;  // uniform across the block
   fill_twiddles(tw, N, tid, nt);
   const int r0 = blockIdx.x * lines;
-  for (int t = tid; t < lines * N; t += nt) acc[t] = make_float2(0.f, 0.f);
-  for (int b = 0; b < a.B; ++b) {
-    int ty, tx, p;
-    const bool valid = window_of(a, b, ty, tx, p);  // uniform across the block
-    if (!valid) continue;
-    __syncthreads();
-    for (int t = tid; t < lines * N; t += nt) {
-      const int r = r0 + t / N, c = t % N;
-      x[t] = a.psi[((int64_t)b * a.M + r) * N + c];
+  float2* base = a.psi + (int64_t)b * a.M * N;
+  for (int t = tid; t < lines * N; t += nt) x[t] = base[(int64_t)(r0 + t / N) * N + (t % N)];
+  __syncthreads();
+  float2* res = fft_lines<+1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
+  for (int t = tid; t < lines * N; t += nt) base[(int64_t)(r0 + t / N) * N + (t % N)] = res[t];
+}
+
+// F4: dprobe = sum_b dpsi_b * conj(patch_b) over valid batch entries (fixed order)
+__global__ __launch_bounds__(256) void k_far_dprobe(FarArgs a) {
+  const int MN = a.M * a.N;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < MN; q += gridDim.x * blockDim.x) {
+    const int r = q / a.N, c = q - r * a.N;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int b = 0; b < a.B; ++b) {
+      int ty, tx, p;
+      if (!window_of(a, b, ty, tx, p)) continue;
+      const float2 v = cmulc(a.psi[(int64_t)b * MN + q], a.obj[(int64_t)(ty + r) * a.W + tx + c]);
+      acc.x += v.x;
+      acc.y += v.y;
     }
-    __syncthreads();
-    float2* res = fft_lines<+1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
-    for (int t = tid; t < lines * N; t += nt) {
-      const int r = r0 + t / N, c = t % N;
-      const float2 dpsi = res[t];
-      const int64_t oi = (int64_t)(ty + r) * a.W + tx + c;
-      const float2 patch = a.obj[oi];
-      const float2 pr = a.probe[r * N + c];
-      acc[t] = cadd(acc[t], cmulc(dpsi, patch));
-      const float2 dp = cmulc(dpsi, pr);
-      atomic_add_f32(&a.dobj[oi].x, dp.x);
-      atomic_add_f32(&a.dobj[oi].y, dp.y);
-    }
+    a.dprobe[q] = acc;
   }
-  for (int t = tid; t < lines * N; t += nt) {
-    const int r = r0 + t / N, c = t % N;
-    a.dprobe[r * N + c] = acc[t];
+}
+
+// F5: dobject (overwritten, whole canvas) = sum over the windows covering each
+// pixel of dpsi_b * conj(probe_n), in batch order: a gather, no atomics.
+constexpr int kMaxWinLds = 1024;
+
+__global__ __launch_bounds__(256) void k_far_dobject(FarArgs a) {
+  __shared__ int2 win[kMaxWinLds];
+  for (int b = threadIdx.x; b < a.B && b < kMaxWinLds; b += blockDim.x) {
+    int ty, tx, p;
+    win[b] = window_of(a, b, ty, tx, p) ? make_int2(ty, tx) : make_int2(-(1 << 30), -(1 << 30));
+  }
+  __syncthreads();
+  const int64_t HW = (int64_t)a.H * a.W;
+  const int MN = a.M * a.N;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < HW; q += (int64_t)gridDim.x * blockDim.x) {
+    const int yy = (int)(q / a.W), xx = (int)(q - (int64_t)yy * a.W);
+    float2 acc = make_float2(0.f, 0.f);
+    for (int b = 0; b < a.B; ++b) {
+      int2 w;
+      if (b < kMaxWinLds) {
+        w = win[b];
+      } else {
+        int ty, tx, p;
+        w = window_of(a, b, ty, tx, p) ? make_int2(ty, tx) : make_int2(-(1 << 30), -(1 << 30));
+      }
+      const int r = yy - w.x, c = xx - w.y;
+      if ((unsigned)r < (unsigned)a.M && (unsigned)c < (unsigned)a.N) {
+        const float2 v = cmulc(a.psi[(int64_t)b * MN + r * a.N + c], a.probe[r * a.N + c]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+    }
+    a.dobj[q] = acc;
   }
 }
 
@@ -382,9 +415,14 @@ extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const 
     launch_cols<0>(a, L.C, stream);
   }
   hipLaunchKernelGGL(k_far_loss_final, dim3(a.B), dim3(64), 0, stream, a);
-  const int lines = a.N >= 256 ? 1 : 256 / a.N;
-  const size_t shm = sizeof(float2) * (size_t)(a.N + 3 * lines * a.N);
-  hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines), dim3(64), shm, stream, a, lines);
+  const int lines = a.N >= 1024 ? 1 : 1024 / a.N < a.M ? 1024 / a.N : a.M;
+  const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
+  hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), shm, stream, a, lines);
+  const int mn = a.M * a.N;
+  hipLaunchKernelGGL(k_far_dprobe, dim3((mn + 255) / 256 < 1024 ? (mn + 255) / 256 : 1024), dim3(256), 0, stream, a);
+  const int64_t hw = (int64_t)a.H * a.W;
+  hipLaunchKernelGGL(k_far_dobject, dim3((unsigned)((hw + 255) / 256 < 4096 ? (hw + 255) / 256 : 4096)), dim3(256), 0,
+                     stream, a);
   return check_launch("farfield_fwd_bwd");
 }
 

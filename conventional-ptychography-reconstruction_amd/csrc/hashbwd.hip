@@ -37,10 +37,12 @@ struct PlanLayout {
   int bbits[PTY_MAX_LEVELS];    // hashed: bits of cx (2^bbits > res), dense: 0
   int off_b[PTY_MAX_LEVELS];    // hashed: bucket starts (nb + 1 ints)
   int off_cy[PTY_MAX_LEVELS];   // hashed: bucketed rows (res + 1 ints capacity)
+  int off_m[PTY_MAX_LEVELS];    // hashed: touched-column bitmask, (res + 32) / 32 words
   int n_bucket_ints;
   int n_cy_ints;
+  int n_mask_words;
   // byte offsets
-  size_t s0, s1, fx, fy, bstart, bcy, total;
+  size_t s0, s1, fx, fy, bstart, bcy, mask, total;
   // dense pass-1 buffer (workspace, not plan): A[l] of n1 * (res + 1) float2
   int64_t off_a[PTY_MAX_LEVELS + 1];
 };
@@ -57,7 +59,7 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
   PlanLayout L{};
   L.n0 = pts.n_axis0;
   L.n1 = pts.n_axis1;
-  int s = 0, bo = 0, co = 0;
+  int s = 0, bo = 0, co = 0, mo = 0;
   int64_t a = 0;
   for (int l = 0; l < g.n_levels; ++l) {
     L.off_s[l] = s;
@@ -68,6 +70,7 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
       L.bbits[l] = 0;
       L.off_b[l] = bo;
       L.off_cy[l] = co;
+      L.off_m[l] = mo;
       a += (int64_t)L.n1 * (g.res[l] + 1);
     } else {
       const int log2T = g.log2_hashmap_size;
@@ -79,12 +82,15 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
       bo += nb + 1;
       L.off_cy[l] = co;
       co += (int)g.res[l] + 1;
+      L.off_m[l] = mo;
+      mo += ((int)g.res[l] + 32) / 32;
     }
   }
   L.off_a[g.n_levels] = a;
   L.n_starts = s;
   L.n_bucket_ints = bo;
   L.n_cy_ints = co;
+  L.n_mask_words = mo;
   size_t off = 0;
   L.s0 = off; off += a256(sizeof(int) * (size_t)s);
   L.s1 = off; off += a256(sizeof(int) * (size_t)s);
@@ -92,6 +98,7 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
   L.fy = off; off += a256(sizeof(float) * (size_t)g.n_levels * L.n1);
   L.bstart = off; off += a256(sizeof(int) * (size_t)(bo > 0 ? bo : 1));
   L.bcy = off; off += a256(sizeof(int) * (size_t)(co > 0 ? co : 1));
+  L.mask = off; off += a256(sizeof(uint32_t) * (size_t)(mo > 0 ? mo : 1));
   L.total = off;
   return L;
 }
@@ -103,6 +110,7 @@ struct PlanPtrs {
   const float* fy;
   const int* bstart;
   const int* bcy;
+  const uint32_t* mask;
 };
 
 static PlanPtrs plan_ptrs(const PlanLayout& L, const void* plan) {
@@ -114,6 +122,7 @@ static PlanPtrs plan_ptrs(const PlanLayout& L, const void* plan) {
   P.fy = (const float*)(p + L.fy);
   P.bstart = (const int*)(p + L.bstart);
   P.bcy = (const int*)(p + L.bcy);
+  P.mask = (const uint32_t*)(p + L.mask);
   return P;
 }
 
@@ -171,8 +180,8 @@ __global__ void k_plan_axes(PtyGrid g, PlanLayout L, const float* __restrict__ a
 // cell cy-1 or cy, grouped by ((h(cy) & (T-1)) >> bbits), ascending cy.
 // One workgroup per level; counting in LDS-free sequential passes by lane 0
 // (a one-time build).
-__global__ void k_plan_buckets(PtyGrid g, PlanLayout L, const int* __restrict__ s1, int* __restrict__ bstart,
-                               int* __restrict__ bcy) {
+__global__ void k_plan_buckets(PtyGrid g, PlanLayout L, const int* __restrict__ s0, const int* __restrict__ s1,
+                               int* __restrict__ bstart, int* __restrict__ bcy, uint32_t* __restrict__ mask) {
   const int l = blockIdx.x;
   if (((g.dense_mask >> l) & 1u) || threadIdx.x != 0) return;
   const int res = (int)g.res[l];
@@ -199,6 +208,20 @@ __global__ void k_plan_buckets(PtyGrid g, PlanLayout L, const int* __restrict__ 
   }
   for (int k = nb; k > 0; --k) bs[k] = bs[k - 1];
   bs[0] = 0;
+  // column cx is touched when axis0 cell cx-1 or cx holds a point
+  const int* s0l = s0 + L.off_s[l];
+  uint32_t* m = mask + L.off_m[l];
+  const int words = (res + 32) / 32;
+  for (int w = 0; w < words; ++w) {
+    uint32_t bits = 0;
+    for (int k = 0; k < 32; ++k) {
+      const int cx = w * 32 + k;
+      if (cx > res) break;
+      const bool touched = (s0l[cx + 1] > s0l[cx]) || (cx > 0 && s0l[cx] > s0l[cx - 1]);
+      if (touched) bits |= 1u << k;
+    }
+    m[w] = bits;
+  }
 }
 
 // ------------------------------------------------------------------ gradient
@@ -252,89 +275,149 @@ __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanP
   return make_float2(ax, ay);
 }
 
-// G(cx, cy) directly from the point gradients (hashed levels: few points per cell)
-__device__ __forceinline__ float2 corner_direct(const PlanLayout& L, const PlanPtrs& P, int l, int res,
-                                                const float2* __restrict__ gl, int cx, int cy) {
-  const int* s0 = P.s0 + L.off_s[l];
-  const int* s1 = P.s1 + L.off_s[l];
-  const float* fxl = P.fx + (int64_t)l * L.n0;
-  const float* fyl = P.fy + (int64_t)l * L.n1;
-  float ax = 0.f, ay = 0.f;
-#pragma unroll
-  for (int dx = 0; dx < 2; ++dx) {
-    const int cxx = cx - dx;
-    if (cxx < 0 || cxx > res) continue;
-    const int j0 = s0[cxx], j1 = s0[cxx + 1];
-    if (j0 == j1) continue;
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy) {
-      const int cyy = cy - dy;
-      if (cyy < 0 || cyy > res) continue;
-      const int i1 = s1[cyy + 1];
-      for (int i = s1[cyy]; i < i1; ++i) {
-        const float wy = dy ? fyl[i] : 1.f - fyl[i];
-        for (int j = j0; j < j1; ++j) {
-          const float w = (dx ? fxl[j] : 1.f - fxl[j]) * wy;
-          const float2 v = gl[(int64_t)i * L.n0 + j];
-          ax += w * v.x;
-          ay += w * v.y;
-        }
-      }
-    }
-  }
-  return make_float2(ax, ay);
-}
-
-// one thread per table entry
-__global__ __launch_bounds__(256) void k_table_owner(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
-                                                     const float2* __restrict__ gfeat,
-                                                     const float2* __restrict__ A, float2* __restrict__ dtable) {
+// Dense levels: one thread per table entry (entries of hashed levels are
+// skipped here and handled by k_table_hashed).
+__global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, PlanPtrs P,
+                                                     const float2* __restrict__ A, uint32_t n_dense_entries,
+                                                     float2* __restrict__ dtable) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.n_entries) return;
+  if (t >= n_dense_entries) return;
   int l = 0;
   while (l + 1 < g.n_levels && t >= g.offset[l + 1]) ++l;
   const uint32_t e = t - g.offset[l];
   const int res = (int)g.res[l];
   const uint32_t size = g.size[l];
+  const float2* Al = A + L.off_a[l];
+  const int64_t top = (int64_t)res * res + res;  // largest cx + cy*res
   float2 acc = make_float2(0.f, 0.f);
-  if ((g.dense_mask >> l) & 1u) {
-    const float2* Al = A + L.off_a[l];
-    const int64_t top = (int64_t)res * res + res;  // largest cx + cy*res
-    for (int k = 0; k < 2; ++k) {
-      const int64_t target = (int64_t)e + (int64_t)k * size;
-      if (target > top) break;
-      // cy with cx = target - cy*res in [0, res]
-      int64_t cy_lo = target > res ? (target - res + res - 1) / res : 0;
-      const int64_t cy_hi = target / res;
-      for (int64_t cy = cy_lo; cy <= cy_hi && cy <= res; ++cy) {
-        const int cx = (int)(target - cy * res);
-        const float2 v = corner_from_A(L, P, l, res, Al, cx, (int)cy);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-    }
-  } else {
-    const uint32_t T = 1u << g.log2_hashmap_size;
-    const int b = L.bbits[l];
-    const uint32_t key = e >> b;
-    const int* bs = P.bstart + L.off_b[l];
-    const int* rows = P.bcy + L.off_cy[l];
-    const float2* gl = gfeat + (int64_t)l * pts.n;
-    const int r1 = bs[key + 1];
-    for (int r = bs[key]; r < r1; ++r) {
-      const int cy = rows[r];
-      uint32_t cx = (e ^ ((uint32_t)cy * kPrime)) & (T - 1u);
-      for (; cx <= (uint32_t)res; cx += T) {
-        const float2 v = corner_direct(L, P, l, res, gl, (int)cx, cy);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
+  for (int k = 0; k < 2; ++k) {
+    const int64_t target = (int64_t)e + (int64_t)k * size;
+    if (target > top) break;
+    // cy with cx = target - cy*res in [0, res]
+    const int64_t cy_lo = target > res ? (target - res + res - 1) / res : 0;
+    const int64_t cy_hi = target / res;
+    for (int64_t cy = cy_lo; cy <= cy_hi && cy <= res; ++cy) {
+      const float2 v = corner_from_A(L, P, l, res, Al, (int)(target - cy * res), (int)cy);
+      acc.x += v.x;
+      acc.y += v.y;
     }
   }
   if (acc.x != 0.f || acc.y != 0.f) {
     float2 d = dtable[t];
     d.x += acc.x;
     d.y += acc.y;
+    dtable[t] = d;
+  }
+}
+
+// Hashed levels: a workgroup owns 256 consecutive entries of one level.  When
+// 2^bbits >= 256 they share one bucket of candidate rows, which is staged in
+// LDS together with the rows' point ranges and the level's touched-column
+// bitmask; a candidate (cx, cy) is summed only if column cx is touched.
+constexpr int kRowsLds = 512;
+constexpr int kMaskLds = 4096;
+
+__global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
+                                                      const float2* __restrict__ gfeat,
+                                                      float2* __restrict__ dtable) {
+  __shared__ uint32_t s_h[kRowsLds];
+  __shared__ int4 s_ir[kRowsLds];      // (i0, i1) of cell cy-1, (i0, i1) of cell cy
+  __shared__ uint32_t s_mask[kMaskLds];
+  // blockIdx.x -> (level, chunk) over the hashed levels in order
+  int l = 0;
+  uint32_t chunk = blockIdx.x;
+  for (; l < g.n_levels; ++l) {
+    if ((g.dense_mask >> l) & 1u) continue;
+    const uint32_t nch = (g.size[l] + 255) / 256;
+    if (chunk < nch) break;
+    chunk -= nch;
+  }
+  if (l >= g.n_levels) return;
+  const int res = (int)g.res[l];
+  const uint32_t T = 1u << g.log2_hashmap_size;
+  const int b = L.bbits[l];
+  const uint32_t e = chunk * 256 + threadIdx.x;
+  const bool valid = e < g.size[l];
+  const int* s0 = P.s0 + L.off_s[l];
+  const int* s1 = P.s1 + L.off_s[l];
+  const float* fxl = P.fx + (int64_t)l * L.n0;
+  const float* fyl = P.fy + (int64_t)l * L.n1;
+  const float2* gl = gfeat + (int64_t)l * pts.n;
+  const int* bs = P.bstart + L.off_b[l];
+  const int* rows = P.bcy + L.off_cy[l];
+  const uint32_t* mk = P.mask + L.off_m[l];
+  const int words = (res + 32) / 32;
+  const bool mask_lds = words <= kMaskLds;
+  if (mask_lds)
+    for (int w = threadIdx.x; w < words; w += blockDim.x) s_mask[w] = mk[w];
+  const bool shared_key = b >= 8;  // all 256 entries of the chunk share e >> b
+  float ax = 0.f, ay = 0.f;
+  auto corner = [&](int cx, int i0a, int i1a, int i0b, int i1b) {
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+      const int cxx = cx - dx;
+      if (cxx < 0 || cxx > res) continue;
+      const int j0 = s0[cxx], j1 = s0[cxx + 1];
+      for (int j = j0; j < j1; ++j) {
+        const float wx = dx ? fxl[j] : 1.f - fxl[j];
+        for (int i = i0a; i < i1a; ++i) {          // cell cy-1: dy = 1
+          const float w = wx * fyl[i];
+          const float2 v = gl[(int64_t)i * L.n0 + j];
+          ax += w * v.x;
+          ay += w * v.y;
+        }
+        for (int i = i0b; i < i1b; ++i) {          // cell cy: dy = 0
+          const float w = wx * (1.f - fyl[i]);
+          const float2 v = gl[(int64_t)i * L.n0 + j];
+          ax += w * v.x;
+          ay += w * v.y;
+        }
+      }
+    }
+  };
+  auto touched = [&](uint32_t cx) -> bool {
+    const uint32_t w = mask_lds ? s_mask[cx >> 5] : mk[cx >> 5];
+    return (w >> (cx & 31)) & 1u;
+  };
+  if (shared_key) {
+    const uint32_t key = (chunk * 256) >> b;
+    const int r0 = bs[key], r1 = bs[key + 1];
+    for (int rb = r0; rb < r1; rb += kRowsLds) {
+      const int nr = min(kRowsLds, r1 - rb);
+      __syncthreads();
+      for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+        const int cy = rows[rb + r];
+        s_h[r] = (uint32_t)cy * kPrime;
+        const int a0 = cy > 0 ? s1[cy - 1] : 0, a1 = cy > 0 ? s1[cy] : 0;
+        s_ir[r] = make_int4(a0, a1, s1[cy], s1[cy + 1]);
+      }
+      __syncthreads();
+      if (valid) {
+        for (int r = 0; r < nr; ++r) {
+          for (uint32_t cx = (e ^ s_h[r]) & (T - 1u); cx <= (uint32_t)res; cx += T) {
+            if (!touched(cx)) continue;
+            const int4 ir = s_ir[r];
+            corner((int)cx, ir.x, ir.y, ir.z, ir.w);
+          }
+        }
+      }
+    }
+  } else if (valid) {
+    const uint32_t key = e >> b;
+    const int r1 = bs[key + 1];
+    for (int r = bs[key]; r < r1; ++r) {
+      const int cy = rows[r];
+      for (uint32_t cx = (e ^ ((uint32_t)cy * kPrime)) & (T - 1u); cx <= (uint32_t)res; cx += T) {
+        if (!touched(cx)) continue;
+        corner((int)cx, cy > 0 ? s1[cy - 1] : 0, cy > 0 ? s1[cy] : 0, s1[cy], s1[cy + 1]);
+      }
+    }
+  }
+  if (valid && (ax != 0.f || ay != 0.f)) {
+    const uint32_t t = g.offset[l] + e;
+    float2 d = dtable[t];
+    d.x += ax;
+    d.y += ay;
     dtable[t] = d;
   }
 }
@@ -364,8 +447,8 @@ int build_points_plan(const PtyGrid& g, const PtyPoints& pts, void* plan, hipStr
   const int nt = L.n_starts > g.n_levels * nax ? L.n_starts : g.n_levels * nax;
   hipLaunchKernelGGL(k_plan_axes, dim3((nt + 255) / 256), dim3(256), 0, stream, g, L, pts.axis0, pts.axis1,
                      (int*)(p + L.s0), (int*)(p + L.s1), (float*)(p + L.fx), (float*)(p + L.fy));
-  hipLaunchKernelGGL(k_plan_buckets, dim3(g.n_levels), dim3(64), 0, stream, g, L, (const int*)(p + L.s1),
-                     (int*)(p + L.bstart), (int*)(p + L.bcy));
+  hipLaunchKernelGGL(k_plan_buckets, dim3(g.n_levels), dim3(64), 0, stream, g, L, (const int*)(p + L.s0),
+                     (const int*)(p + L.s1), (int*)(p + L.bstart), (int*)(p + L.bcy), (uint32_t*)(p + L.mask));
   return check_launch("points_plan_build");
 }
 
@@ -398,8 +481,17 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
   if (na > 0)
     hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, P, gfeat,
                        A);
-  hipLaunchKernelGGL(k_table_owner, dim3((g.n_entries + 255) / 256), dim3(256), 0, stream, g, L, pts, P, gfeat,
-                     (const float2*)A, (float2*)dtable);
+  uint32_t n_dense_entries = 0, hashed_chunks = 0;
+  for (int l = 0; l < g.n_levels; ++l) {
+    if ((g.dense_mask >> l) & 1u) n_dense_entries = g.offset[l] + g.size[l];
+    else hashed_chunks += (g.size[l] + 255) / 256;
+  }
+  if (n_dense_entries > 0)
+    hipLaunchKernelGGL(k_table_dense, dim3((n_dense_entries + 255) / 256), dim3(256), 0, stream, g, L, P,
+                       (const float2*)A, n_dense_entries, (float2*)dtable);
+  if (hashed_chunks > 0)
+    hipLaunchKernelGGL(k_table_hashed, dim3(hashed_chunks), dim3(256), 0, stream, g, L, pts, P, gfeat,
+                       (float2*)dtable);
   return check_launch("table_bwd");
 }
 

@@ -167,6 +167,7 @@ class FusedReconstruction:
                 self.far[b] = (d, _lib.workspace(q("pty_farfield_workspace", d), dev))
         self.epoch = 0
         self._timers = None
+        self._grad_clear = True   # FlatParams starts with zero gradients
 
     # ------------------------------------------------------------------ timing
     def enable_timing(self, names):
@@ -221,9 +222,9 @@ class FusedReconstruction:
         m = self.model
         b = batch_idx.numel()
         d, ws_far = self.far[b]
-        C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
+        if not self._grad_clear:
+            C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
         self.forward_fields()
-        C("pty_fill", self.dobj.data_ptr(), 2 * self.dobj.numel(), 0.0, s)
         self._mark("farfield", True)
         C("pty_farfield_fwd_bwd", d, self.obj.data_ptr(), self.probe_n.data_ptr(), self.stack.data_ptr(),
           batch_idx.data_ptr(), self.tl.data_ptr(), self.loss.data_ptr(), self.dobj.data_ptr(),
@@ -250,12 +251,47 @@ class FusedReconstruction:
           self.prb_dtable.data_ptr(), self.prb_pdesc, self.dprb_mlp.data_ptr(), self.ws_prb.data_ptr(),
           self.ws_prb.numel(), s)
         if update:
+            # Adam clears the trainable gradients it consumes; frozen ones are
+            # cleared with them when they exist
             C("pty_adam_tick", self.hyper.data_ptr(), s)
             C("pty_adam", self.flat.data.data_ptr(), self.flat.grad.data_ptr(), self.m.data_ptr(),
-              self.v.data_ptr(), self.flat.n_train, self.hyper.data_ptr(), self.beta1, self.beta2, self.eps, s)
+              self.v.data_ptr(), self.flat.n_train, self.hyper.data_ptr(), self.beta1, self.beta2, self.eps, 1, s)
+            n_frozen = self.flat.grad.numel() - self.flat.n_train
+            if n_frozen:
+                C("pty_fill", self.flat.grad.data_ptr() + 4 * self.flat.n_train, n_frozen, 0.0, s)
+        self._grad_clear = update
         return self.loss[:b]
 
-    def run_epoch(self, generator: torch.Generator | None = None):
+    # ------------------------------------------------------------------ graph
+    def capture(self):
+        """Capture one full training step (batch size ``batch_size``) in a HIP graph.
+
+        Every launch of the step goes to the capture stream; the batch indices
+        are read from a persistent device buffer that ``graph_step`` fills
+        before each replay.  The graph holds no memset nodes (clears are fill
+        kernels) and no host synchronisation.
+        """
+        self.batch_buf = torch.zeros(self.batch_size, dtype=torch.int32, device=self.device)
+        # one eager pass without the update loads every kernel before capture
+        self.step(self.batch_buf, update=False)
+        _lib.call("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, _lib.stream_ptr(self.device))
+        self._grad_clear = True
+        torch.cuda.synchronize(self.device)
+        timers, self._timers = self._timers, None
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.step(self.batch_buf, update=True)
+        self._timers = timers
+        torch.cuda.synchronize(self.device)
+        return self.graph
+
+    def graph_step(self, batch_idx: torch.Tensor):
+        """Replay the captured step on ``batch_idx`` (int32, device, length batch_size)."""
+        self.batch_buf.copy_(batch_idx, non_blocking=True)
+        self.graph.replay()
+        return self.loss
+
+    def run_epoch(self, generator: torch.Generator | None = None, use_graph: bool = False):
         """One epoch: randperm over frames, ceil(frames/batch) steps (Recovery.py:243-350)."""
         lr = self.lr0 * self.gamma ** (self.epoch // self.decay_step)
         if self.epoch == 0 or self.epoch % self.decay_step == 0:
@@ -263,7 +299,11 @@ class FusedReconstruction:
         perm = torch.randperm(self.frames, device=self.device, generator=generator).to(torch.int32)
         total = torch.zeros((), dtype=torch.float32, device=self.device)
         for start in range(0, self.frames, self.batch_size):
-            total += self.step(perm[start:start + self.batch_size]).sum()
+            bidx = perm[start:start + self.batch_size]
+            if use_graph and bidx.numel() == self.batch_size:
+                total += self.graph_step(bidx).sum()
+            else:
+                total += self.step(bidx).sum()
         self.epoch += 1
         return total
 

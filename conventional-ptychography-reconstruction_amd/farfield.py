@@ -100,7 +100,7 @@ class FarFieldLossFn(torch.autograd.Function):
         H, W = o.shape
         d = far_descriptor(M, N, H, W, bidx.numel(), stack.shape[0], loss_type)
         loss = torch.empty(bidx.numel(), dtype=torch.float32, device=o.device)
-        dobj = torch.zeros_like(o)
+        dobj = torch.empty_like(o)
         dprobe = torch.empty_like(p)
         ws = _lib.workspace(_lib.query("pty_farfield_workspace", d), o.device)
         _lib.call("pty_farfield_fwd_bwd", d, o.data_ptr(), p.data_ptr(), stack.data_ptr(), bidx.data_ptr(),

@@ -181,8 +181,10 @@ int pty_stack_prepare(const float* stack_mnf, int32_t M, int32_t N, int32_t fram
                       hipStream_t stream);
 size_t pty_farfield_workspace(const PtyFar* d);
 /* loss[b] = per-pattern loss (mean over pixels) of position batch_idx[b];
- * dobject (c64 canvas) += d(sum_b loss[b])/d object; dprobe (c64 [M,N]) =
- * d(sum_b loss[b])/d probe_n (overwritten).  tl is [frames, 2] = (tlY, tlX). */
+ * dobject (c64 canvas) = d(sum_b loss[b])/d object and dprobe (c64 [M,N]) =
+ * d(sum_b loss[b])/d probe_n, both overwritten (the whole canvas is written,
+ * zero outside the windows).  tl is [frames, 2] = (tlY, tlX).  The result is
+ * deterministic (fixed-order sums, no atomics). */
 int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* probe_n,
                          const float* stack, const int32_t* batch_idx, const int32_t* tl,
                          float* loss, float* dobject, float* dprobe, void* work,
@@ -195,9 +197,11 @@ int pty_farfield_intensity(const PtyFar* d, const float* object, const float* pr
 /* ---- Adam (torch.optim.Adam, amsgrad=False, weight_decay=0; Recovery.py:238, 321) ---- */
 /* hyper (device, 2 floats): [lr, step]; bias corrections 1 - beta^step are
  * formed on the device.  pty_adam_tick adds 1 to step (call once per step
- * before the pty_adam calls of that step). */
-int pty_adam(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
-             const float* hyper, float beta1, float beta2, float eps, hipStream_t stream);
+ * before the pty_adam calls of that step).  clear_grad != 0 zeroes grad after
+ * use (saves the separate clear of the next step). */
+int pty_adam(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+             const float* hyper, float beta1, float beta2, float eps, int32_t clear_grad,
+             hipStream_t stream);
 int pty_adam_tick(float* hyper, hipStream_t stream);
 
 /* fill kernel (graph-safe replacement for hipMemsetAsync) */
