@@ -37,7 +37,7 @@ MLP_FLOP_FWD_PER_POINT = 18688
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
-def build_problem(pkg, device, seed=0):
+def build_problem(pkg, device, seed=0, precision="fp32"):
     import numpy as np
     import torch
     inst = pkg.synthetic.make_instance(seed=seed, **WORKLOAD)
@@ -45,7 +45,7 @@ def build_problem(pkg, device, seed=0):
     torch.manual_seed(seed)
     model = pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M,
                                   log2_hashmap_size=LOG2T, hidden_omega_0=1.0, trainable_omega0=True,
-                                  use_residual=True,
+                                  use_residual=True, mlp_precision=precision,
                                   probe_initial=torch.from_numpy(inst.probe.astype(np.complex64)))
     return inst, model.to(device)
 
@@ -282,6 +282,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32", help="MLP GEMM precision")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", choices=["throughput", "nrmse"], default="throughput")
     ap.add_argument("--workload", default="cfg0", help="nrmse mode: cfg0 | cfg1 | small")
@@ -300,7 +301,7 @@ def main():
     import __graft_entry__
     pkg = __graft_entry__.load_package()
     dev = torch.device("cuda:0")
-    inst, model = build_problem(pkg, dev)
+    inst, model = build_problem(pkg, dev, precision=args.precision)
     eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
                                   loss_type="smooth_L1_loss", lr=1e-3, batch_size=BATCH)
     frames = eng.frames
@@ -366,7 +367,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.precision == "fp32" else "f32 (MLP GEMMs bf16, fp32 accumulate)",
         "data": "synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson 1e5 photons)",
         "config": {"workload": "canvas1024_inr512_probe256_scan20x20_batch10_log2T18_smoothL1",
                    "steps_per_iter": 40, "global_batch": BATCH, "positions": frames,

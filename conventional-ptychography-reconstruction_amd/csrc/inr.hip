@@ -25,138 +25,11 @@
 //   g_W_real += g_v x^T (MFMA with points as K; operands staged through LDS,
 //   o-major with a padded stride), g_c += sum g_v, g_x = W_real^T g_v (MFMA)
 //   layer 0: g_u0 = w0 Im(g_w0), g_w0 += u0 Im(g_w0), g_W0, g_b0, g_h = W0^T g_u0
-//   hash grid: dtable[corner] += w_c * g_h (atomics)
-#include "hashgrid.h"
+//   feature gradients g_h are written level-major for the table kernels
+//   (hashbwd.hip)
+#include "inr_common.h"
 
 namespace pty {
-
-typedef float v16f __attribute__((ext_vector_type(16)));
-
-constexpr int WS = 33;  // padded LDS row stride (floats)
-constexpr int kWaves = 4;
-constexpr int kThreads = 64 * kWaves;
-
-// C-layout row of value v for lane half h
-__device__ __forceinline__ int crow(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
-// feature held by lane half h as value s (layer-0 B operand): level 2(s>>1)+h, feature s&1
-__device__ __forceinline__ int kappa0(int s, int h) { return 4 * (s >> 1) + 2 * h + (s & 1); }
-// inverse of crow, composed with kappa0: feature index of M row rho in g_h = W0^T g_u0
-__device__ __forceinline__ int phi0(int rho) { return kappa0(4 * (rho >> 3) + (rho & 3), (rho >> 2) & 1); }
-
-__device__ __forceinline__ v16f mfma(float a, float b, v16f c) {
-  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
-}
-
-// exp(i*w*x) for real x: (cos wx, sin wx).  Range-reduced to revolutions in
-// [-0.5, 0.5] before the hardware sin/cos.
-__device__ __forceinline__ void cis(float theta, float& c, float& s) {
-  float r = theta * 0.15915494309189535f;
-  r = r - rintf(r);
-  s = __builtin_amdgcn_sinf(r);
-  c = __builtin_amdgcn_cosf(r);
-}
-
-__device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
-
-template <int H>
-struct MlpLds {
-  float w0[32 * WS];   // W0[o][f], f zero-padded to 32
-  float b0[32];
-  float ar[H][32 * WS];
-  float ai[H][32 * WS];
-  float nai[H][32 * WS];
-  float cr[H][32];
-  float ci[H][32];
-  float om[H + 1];
-  float wor[32], woi[32];
-  float bo[2];
-};
-
-template <int H>
-__device__ void load_weights(MlpLds<H>& s, const PtyMlp& m) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int t = tid; t < 32 * 32; t += nt) {
-    const int o = t >> 5, f = t & 31;
-    s.w0[o * WS + f] = f < m.n_in ? m.w0[o * m.n_in + f] : 0.f;
-    for (int k = 0; k < H; ++k) {
-      const float2 a = ((const float2*)m.wh[k])[t];
-      s.ar[k][o * WS + f] = a.x;
-      s.ai[k][o * WS + f] = a.y;
-      s.nai[k][o * WS + f] = -a.y;
-    }
-  }
-  for (int t = tid; t < 32; t += nt) {
-    s.b0[t] = m.b0[t];
-    for (int k = 0; k < H; ++k) {
-      s.cr[k][t] = m.bh[k][2 * t];
-      s.ci[k][t] = m.bh[k][2 * t + 1];
-    }
-    s.wor[t] = m.wo[2 * t];
-    s.woi[t] = m.wo[2 * t + 1];
-  }
-  if (tid <= H) s.om[tid] = *m.omega[tid];
-  if (tid == 0) { s.bo[0] = m.bo[0]; s.bo[1] = m.bo[1]; }
-}
-
-// Features of the lane's 8 levels: feat[s] = feature kappa0(s, h).
-__device__ __forceinline__ void encode_lane(const PtyGrid& g, float x0, float x1, int h,
-                                            const float2* __restrict__ table, float feat[16]) {
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int l = 2 * m + h;
-    float2 f = make_float2(0.f, 0.f);
-    if (l < g.n_levels) f = level_feature(g, l, x0, x1, table);
-    feat[2 * m] = f.x;
-    feat[2 * m + 1] = f.y;
-  }
-}
-
-// Layer 0: u0[v] (unit crow(v,h)) = b0 + W0 feat.
-template <int H>
-__device__ __forceinline__ v16f layer0(const MlpLds<H>& s, const float feat[16], int j, int h) {
-  v16f acc;
-#pragma unroll
-  for (int v = 0; v < 16; ++v) acc[v] = s.b0[crow(v, h)];
-#pragma unroll
-  for (int st = 0; st < 16; ++st) acc = mfma(s.w0[j * WS + kappa0(st, h)], feat[st], acc);
-  return acc;
-}
-
-// Hidden layer k: (vr, vi)[v] for unit crow(v,h), input x[s] in kappa1 order
-// (s < 16: Re z of unit crow(s,h); s >= 16: Im z of unit crow(s-16,h)).
-template <int H>
-__device__ __forceinline__ void hidden_fwd(const MlpLds<H>& s, int k, const float zr[16], const float zi[16], int j,
-                                           int h, v16f& vr, v16f& vi) {
-#pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    vr[v] = s.cr[k][crow(v, h)];
-    vi[v] = s.ci[k][crow(v, h)];
-  }
-#pragma unroll
-  for (int st = 0; st < 16; ++st) {
-    const int u = crow(st, h);
-    vr = mfma(s.ar[k][j * WS + u], zr[st], vr);
-    vi = mfma(s.ai[k][j * WS + u], zr[st], vi);
-  }
-#pragma unroll
-  for (int st = 0; st < 16; ++st) {
-    const int u = crow(st, h);
-    vr = mfma(s.nai[k][j * WS + u], zi[st], vr);
-    vi = mfma(s.ar[k][j * WS + u], zi[st], vi);
-  }
-}
-
-__device__ __forceinline__ void gabor_act(float w, const v16f& vr, const v16f& vi, float zr[16], float zi[16]) {
-#pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    float c, sn;
-    cis(w * vr[v], c, sn);
-    const float mag = exp_f(-w * vi[v]);
-    zr[v] = mag * c;
-    zi[v] = mag * sn;
-  }
-}
-
 // ============================================================== forward
 template <int H>
 __global__ __launch_bounds__(kThreads) void k_inr_fwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
@@ -246,38 +119,12 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __r
 }
 
 // ============================================================== backward
-// Flat gradient layout of one workgroup partial (floats):
-//   w0 [32*32] | b0 [32] | per hidden k: A [32*32*2] (re,im) | c [32*2] | omega [H+1] | wo [32*2] | bo [2]
-template <int H>
-struct GradLayout {
-  static constexpr int w0 = 0;
-  static constexpr int b0 = 1024;
-  static constexpr int A(int k) { return 1056 + k * 2112; }
-  static constexpr int c(int k) { return 1056 + k * 2112 + 2048; }
-  static constexpr int om = 1056 + H * 2112;
-  static constexpr int wo = om + H + 1;
-  static constexpr int bo = wo + 64;
-  static constexpr int total = bo + 2;
-};
-
 template <int H>
 struct BwdStage {
   float X[H][64 * WS];   // z_k (k < H) o-major: rows 0..31 Re, 32..63 Im, column = point
   float F[32 * WS];      // features, feature-major
   float G[64 * WS];      // gradient staging
 };
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
 
 template <int H>
 __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
@@ -560,7 +407,8 @@ static int validate_mlp(const PtyMlp* m, bool need_all) {
   PTY_REQUIRE(m != nullptr, PTY_EINVAL, "mlp descriptor is NULL");
   PTY_REQUIRE(m->n_in >= 1 && m->n_in <= 32, PTY_EUNSUP, "mlp: n_in %d outside 1..32", m->n_in);
   PTY_REQUIRE(m->n_hidden >= 1 && m->n_hidden <= 2, PTY_EUNSUP, "mlp: hidden_layers %d outside 1..2", m->n_hidden);
-  PTY_REQUIRE(m->precision == PTY_PREC_FP32, PTY_EUNSUP, "mlp: precision %d not built", m->precision);
+  PTY_REQUIRE(m->precision == PTY_PREC_FP32 || m->precision == PTY_PREC_BF16, PTY_EUNSUP,
+              "mlp: precision %d not built", m->precision);
   if (need_all) {
     PTY_REQUIRE(m->w0 && m->b0 && m->wo && m->bo, PTY_EINVAL, "mlp: null weight pointer");
     for (int k = 0; k < m->n_hidden; ++k)
@@ -611,7 +459,9 @@ extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* tab
   PTY_REQUIRE(table && out, PTY_EINVAL, "inr_fwd: null pointer");
   if (pts->n == 0) return PTY_OK;
   const int blocks = fwd_blocks(pts->n);
-  if (mlp->n_hidden == 1)
+  if (mlp->precision == PTY_PREC_BF16)
+    launch_inr_fwd_bf16(*g, *mlp, (const float2*)table, *pts, (float2*)out, blocks, stream);
+  else if (mlp->n_hidden == 1)
     hipLaunchKernelGGL(k_inr_fwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, *pts,
                        (float2*)out);
   else
@@ -656,7 +506,9 @@ extern "C" int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* gr
   char* tab_work = (char*)gfeat + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n);
   const int total = (int)grad_total(mlp);
   float* partial2 = partial + (size_t)blocks * total;
-  if (mlp->n_hidden == 1)
+  if (mlp->precision == PTY_PREC_BF16)
+    launch_inr_bwd_bf16(*g, *mlp, (const float2*)table, gfeat, *pts, (const float2*)dout, partial, blocks, stream);
+  else if (mlp->n_hidden == 1)
     hipLaunchKernelGGL(k_inr_bwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, gfeat,
                        *pts, (const float2*)dout, partial);
   else

@@ -1,0 +1,175 @@
+// Shared pieces of the fused INR kernels (FP32 and BF16 MFMA variants).
+//
+// Layout.  A wavefront owns a tile of 32 points; lane l = (j = l&31, h = l>>5)
+// holds point j.  Every layer is computed transposed, Y^T = W X^T, with the
+// points as the MFMA N dimension.  The C layout of the 32x32 MFMAs gives lane
+// (j, h) rows crow(v, h) = 8(v>>2) + 4h + (v&3), v < 16, of column j; with the
+// K index of the next weight matrix permuted to match (kappa maps), that
+// register set is the next layer's B operand, so activations stay in
+// registers through the forward.
+#pragma once
+#include "hashgrid.h"
+
+namespace pty {
+
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int WS = 33;  // padded LDS row stride (floats)
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+
+// C-layout row of value v for lane half h
+__device__ __forceinline__ int crow(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
+// feature held by lane half h as value s (layer-0 B operand): level 2(s>>1)+h, feature s&1
+__device__ __forceinline__ int kappa0(int s, int h) { return 4 * (s >> 1) + 2 * h + (s & 1); }
+// inverse of crow, composed with kappa0: feature index of M row rho in g_h = W0^T g_u0
+__device__ __forceinline__ int phi0(int rho) { return kappa0(4 * (rho >> 3) + (rho & 3), (rho >> 2) & 1); }
+
+__device__ __forceinline__ v16f mfma(float a, float b, v16f c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// exp(i*w*x) for real x: (cos wx, sin wx).  Range-reduced to revolutions in
+// [-0.5, 0.5] before the hardware sin/cos.
+__device__ __forceinline__ void cis(float theta, float& c, float& s) {
+  float r = theta * 0.15915494309189535f;
+  r = r - rintf(r);
+  s = __builtin_amdgcn_sinf(r);
+  c = __builtin_amdgcn_cosf(r);
+}
+
+__device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
+template <int H>
+struct MlpLds {
+  float w0[32 * WS];   // W0[o][f], f zero-padded to 32
+  float b0[32];
+  float ar[H][32 * WS];
+  float ai[H][32 * WS];
+  float nai[H][32 * WS];
+  float cr[H][32];
+  float ci[H][32];
+  float om[H + 1];
+  float wor[32], woi[32];
+  float bo[2];
+};
+
+template <int H>
+__device__ void load_weights(MlpLds<H>& s, const PtyMlp& m) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int t = tid; t < 32 * 32; t += nt) {
+    const int o = t >> 5, f = t & 31;
+    s.w0[o * WS + f] = f < m.n_in ? m.w0[o * m.n_in + f] : 0.f;
+    for (int k = 0; k < H; ++k) {
+      const float2 a = ((const float2*)m.wh[k])[t];
+      s.ar[k][o * WS + f] = a.x;
+      s.ai[k][o * WS + f] = a.y;
+      s.nai[k][o * WS + f] = -a.y;
+    }
+  }
+  for (int t = tid; t < 32; t += nt) {
+    s.b0[t] = m.b0[t];
+    for (int k = 0; k < H; ++k) {
+      s.cr[k][t] = m.bh[k][2 * t];
+      s.ci[k][t] = m.bh[k][2 * t + 1];
+    }
+    s.wor[t] = m.wo[2 * t];
+    s.woi[t] = m.wo[2 * t + 1];
+  }
+  if (tid <= H) s.om[tid] = *m.omega[tid];
+  if (tid == 0) { s.bo[0] = m.bo[0]; s.bo[1] = m.bo[1]; }
+}
+
+// Features of the lane's 8 levels: feat[s] = feature kappa0(s, h).
+__device__ __forceinline__ void encode_lane(const PtyGrid& g, float x0, float x1, int h,
+                                            const float2* __restrict__ table, float feat[16]) {
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int l = 2 * m + h;
+    float2 f = make_float2(0.f, 0.f);
+    if (l < g.n_levels) f = level_feature(g, l, x0, x1, table);
+    feat[2 * m] = f.x;
+    feat[2 * m + 1] = f.y;
+  }
+}
+
+// Layer 0: u0[v] (unit crow(v,h)) = b0 + W0 feat.
+template <int H>
+__device__ __forceinline__ v16f layer0(const MlpLds<H>& s, const float feat[16], int j, int h) {
+  v16f acc;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc[v] = s.b0[crow(v, h)];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) acc = mfma(s.w0[j * WS + kappa0(st, h)], feat[st], acc);
+  return acc;
+}
+
+// Hidden layer k: (vr, vi)[v] for unit crow(v,h), input x[s] in kappa1 order
+// (s < 16: Re z of unit crow(s,h); s >= 16: Im z of unit crow(s-16,h)).
+template <int H>
+__device__ __forceinline__ void hidden_fwd(const MlpLds<H>& s, int k, const float zr[16], const float zi[16], int j,
+                                           int h, v16f& vr, v16f& vi) {
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    vr[v] = s.cr[k][crow(v, h)];
+    vi[v] = s.ci[k][crow(v, h)];
+  }
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int u = crow(st, h);
+    vr = mfma(s.ar[k][j * WS + u], zr[st], vr);
+    vi = mfma(s.ai[k][j * WS + u], zr[st], vi);
+  }
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int u = crow(st, h);
+    vr = mfma(s.nai[k][j * WS + u], zi[st], vr);
+    vi = mfma(s.ar[k][j * WS + u], zi[st], vi);
+  }
+}
+
+__device__ __forceinline__ void gabor_act(float w, const v16f& vr, const v16f& vi, float zr[16], float zi[16]) {
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    float c, sn;
+    cis(w * vr[v], c, sn);
+    const float mag = exp_f(-w * vi[v]);
+    zr[v] = mag * c;
+    zi[v] = mag * sn;
+  }
+}
+
+// Flat gradient layout of one workgroup partial (floats):
+//   w0 [32*32] | b0 [32] | per hidden k: A [32*32*2] (re,im) | c [32*2] | omega [H+1] | wo [32*2] | bo [2]
+template <int H>
+struct GradLayout {
+  static constexpr int w0 = 0;
+  static constexpr int b0 = 1024;
+  static constexpr int A(int k) { return 1056 + k * 2112; }
+  static constexpr int c(int k) { return 1056 + k * 2112 + 2048; }
+  static constexpr int om = 1056 + H * 2112;
+  static constexpr int wo = om + H + 1;
+  static constexpr int bo = wo + 64;
+  static constexpr int total = bo + 2;
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// BF16 launchers (inr_bf16.hip)
+void launch_inr_fwd_bf16(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
+                         int blocks, hipStream_t stream);
+void launch_inr_bwd_bf16(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
+                         const float2* dout, float* partial, int blocks, hipStream_t stream);
+
+}  // namespace pty

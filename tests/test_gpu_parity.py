@@ -374,3 +374,33 @@ def test_fused_step_is_deterministic(pkg, dev):
         losses = [e.step(bt).clone() for e in engines]
         assert torch.equal(losses[0], losses[1])
     assert torch.equal(engines[0].flat.data, engines[1].flat.data)
+
+
+@pytest.mark.parametrize("hidden_layers", [1, 2])
+def test_inr_field_bf16(pkg, dev, hidden_layers):
+    """BF16 MFMA path (fp32 accumulate): field and gradients against the FP64 oracle.
+
+    Tolerance: bf16 operands carry 8 mantissa bits (relative 2^-9 per rounding);
+    with features of order 0.1 and omega_0 = 10 the phases move by ~1e-3 rad, so
+    the field is held to 2e-2 and the gradients to 5e-2 relative (norm).
+    """
+    model = _model(pkg, dev, 64, 16, log2T=12, residual=False, hidden_layers=hidden_layers)
+    model.mlp_precision = "bf16"
+    with torch.no_grad():
+        model.xy_hash_encoder.params.uniform_(-0.1, 0.1)
+    f = model._field(model.xy_hash_encoder, model.object_complex_mlp, "object")
+    params = oracle_params(model)
+    ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], oracle_table(model), params["obj_mlp"], 32, 32)
+    err_f = rel_err(f.reshape(32, 32), ref)
+    assert err_f < 2e-2, err_f
+    g = torch.randn(f.shape, dtype=torch.complex64, device=dev)
+    (torch.real(torch.conj(g) * f)).sum().backward()
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref.reshape(-1))).sum().backward()
+    assert rel_err(model.xy_hash_encoder.params.grad, params["obj_table"].grad) < 5e-2
+    mlp = params["obj_mlp"]
+    theirs = []
+    for (w, b, om) in mlp["layers"]:
+        theirs += [w, b, om]
+    theirs += [mlp["out_weight"], mlp["out_bias"]]
+    for a, b in zip(model.object_complex_mlp.tensors(), theirs):
+        assert rel_err(a.grad, b.grad) < 5e-2, (a.shape, rel_err(a.grad, b.grad))
