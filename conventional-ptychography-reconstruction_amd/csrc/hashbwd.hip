@@ -315,7 +315,8 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
 // LDS together with the rows' point ranges and the level's touched-column
 // bitmask; a candidate (cx, cy) is summed only if column cx is touched.
 constexpr int kRowsLds = 512;
-constexpr int kMaskLds = 4096;
+constexpr int kMaskLds = 2048;
+constexpr int kCand = 16;
 
 __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                       const float2* __restrict__ gfeat,
@@ -323,6 +324,7 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
   __shared__ uint32_t s_h[kRowsLds];
   __shared__ int4 s_ir[kRowsLds];      // (i0, i1) of cell cy-1, (i0, i1) of cell cy
   __shared__ uint32_t s_mask[kMaskLds];
+  __shared__ uint32_t s_cand[kCand][256];
   // blockIdx.x -> (level, chunk) over the hashed levels in order
   int l = 0;
   uint32_t chunk = blockIdx.x;
@@ -350,7 +352,9 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
   const bool mask_lds = words <= kMaskLds;
   if (mask_lds)
     for (int w = threadIdx.x; w < words; w += blockDim.x) s_mask[w] = mk[w];
-  const bool shared_key = b >= 8;  // all 256 entries of the chunk share e >> b
+  // all 256 entries of the chunk share e >> b; a row gives at most per_row columns
+  const int per_row = (int)(((uint32_t)res + T) / T);
+  const bool shared_key = b >= 8 && per_row <= kCand && res < 65536;
   float ax = 0.f, ay = 0.f;
   auto corner = [&](int cx, int i0a, int i1a, int i0b, int i1b) {
 #pragma unroll
@@ -392,14 +396,25 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
         s_ir[r] = make_int4(a0, a1, s1[cy], s1[cy + 1]);
       }
       __syncthreads();
-      if (valid) {
-        for (int r = 0; r < nr; ++r) {
-          for (uint32_t cx = (e ^ s_h[r]) & (T - 1u); cx <= (uint32_t)res; cx += T) {
-            if (!touched(cx)) continue;
-            const int4 ir = s_ir[r];
-            corner((int)cx, ir.x, ir.y, ir.z, ir.w);
+      // Compact this thread's touched candidates (row, cx) into its own LDS
+      // list, then sum them with all lanes busy: at the fine levels only a
+      // few of the rows give a touched column for a given entry.
+      int r = 0;
+      while (true) {
+        int cnt = 0;
+        if (valid) {
+          for (; r < nr && cnt + per_row <= kCand; ++r) {
+            for (uint32_t cx = (e ^ s_h[r]) & (T - 1u); cx <= (uint32_t)res; cx += T)
+              if (touched(cx)) s_cand[cnt++][threadIdx.x] = (r << 16) | cx;
           }
         }
+        for (int k = 0; k < cnt; ++k) {
+          const uint32_t c = s_cand[k][threadIdx.x];
+          const int4 ir = s_ir[c >> 16];
+          corner((int)(c & 0xFFFFu), ir.x, ir.y, ir.z, ir.w);
+        }
+        // the loop is per thread; threads that are done leave it
+        if (!valid || r >= nr) break;
       }
     }
   } else if (valid) {

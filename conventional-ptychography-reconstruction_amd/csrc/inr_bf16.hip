@@ -168,6 +168,30 @@ __device__ __forceinline__ void hidden_bf16(const Bf16Fwd<H>& o, const SmallLds<
   }
 }
 
+// Hidden layer k from the packed bf16 input (4 operands in kappa1 order).
+template <int H>
+__device__ __forceinline__ void hidden_bf16p(const Bf16Fwd<H>& o, const SmallLds<H>& s, int k, const uint4 b[4],
+                                             int lane, int h, v16f& vr, v16f& vi) {
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    vr[v] = s.cr[k][crow(v, h)];
+    vi[v] = s.ci[k][crow(v, h)];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    vr = mfma16(o.a[k][0][q][lane], b[q], vr);
+    vi = mfma16(o.a[k][1][q][lane], b[q], vi);
+  }
+}
+
+// unpack value v (0..15) of the Re (part 0) or Im (part 1) half of a packed operand set
+__device__ __forceinline__ float unpack_val(const uint4 b[4], int part, int v) {
+  const uint4 w = b[2 * part + (v >> 3)];
+  const int t = v & 7;
+  const uint32_t word = (t < 2) ? w.x : (t < 4) ? w.y : (t < 6) ? w.z : w.w;
+  return (t & 1) ? __uint_as_float(word & 0xFFFF0000u) : __uint_as_float(word << 16);
+}
+
 // ============================================================== forward
 template <int H>
 __global__ __launch_bounds__(kThreads) void k_inr_fwd_bf16(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
@@ -268,25 +292,42 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_bf16(PtyGrid g, PtyMlp m, 
     const bool valid = p < pts.n;
     float x0 = 0.5f, x1 = 0.5f;
     if (valid) point_coords(pts, p, x0, x1);
-    // ---------------- recompute forward
-    float feat[16];
-    encode_lane(g, x0, x1, h, table, feat);
+    // ---------------- recompute forward; keep only the packed bf16 inputs of
+    // every layer (the pre-activations are recomputed where needed)
+    uint4 pf[2];
+    {
+      float feat[16];
+      encode_lane(g, x0, x1, h, table, feat);
 #pragma unroll
-    for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * BS + j] = f2bf(feat[v]);
-    const v16f u0 = layer0_bf16<H>(o, s, feat, lane, h);
+      for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * BS + j] = f2bf(feat[v]);
+      pf[0] = pack8(feat);
+      pf[1] = pack8(feat + 8);
+    }
     float zr[16], zi[16];
+    {
+      v16f u0;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
-    v16f vr[H], vi[H];
+      for (int v = 0; v < 16; ++v) u0[v] = s.b0[crow(v, h)];
+      u0 = mfma16(o.w0[0][lane], pf[0], u0);
+      u0 = mfma16(o.w0[1][lane], pf[1], u0);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
+    }
+    uint4 px[H][4];
 #pragma unroll
     for (int k = 0; k < H; ++k) {
+      px[k][0] = pack8(zr);
+      px[k][1] = pack8(zr + 8);
+      px[k][2] = pack8(zi);
+      px[k][3] = pack8(zi + 8);
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         S.X[k][crow(v, h) * BS + j] = f2bf(zr[v]);
         S.X[k][(32 + crow(v, h)) * BS + j] = f2bf(zi[v]);
       }
-      hidden_bf16<H>(o, s, k, zr, zi, lane, h, vr[k], vi[k]);
-      gabor_act(s.om[k + 1], vr[k], vi[k], zr, zi);
+      v16f vr, vi;
+      hidden_bf16p<H>(o, s, k, px[k], lane, h, vr, vi);
+      gabor_act(s.om[k + 1], vr, vi, zr, zi);
     }
     // ---------------- output layer
     float2 go = make_float2(0.f, 0.f);
@@ -313,13 +354,16 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_bf16(PtyGrid g, PtyMlp m, 
 #pragma unroll
     for (int k = H - 1; k >= 0; --k) {
       const float w = s.om[k + 1];
+      // pre-activation of layer k, recomputed from its packed input
+      v16f vr, vi;
+      hidden_bf16p<H>(o, s, k, px[k], lane, h, vr, vi);
       float gvr[16], gvi[16];
       float om_acc = 0.f;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const float gwr = gzr[v] * zr[v] + gzi[v] * zi[v];
         const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];
-        om_acc += gwi * vr[k][v] - gwr * vi[k][v];
+        om_acc += gwi * vr[v] - gwr * vi[v];
         gvr[v] = w * gwi;
         gvi[v] = -w * gwr;
         const int u = crow(v, h);
@@ -353,14 +397,19 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_bf16(PtyGrid g, PtyMlp m, 
       for (int v = 0; v < 16; ++v) {
         gzr[v] = gxr[v];
         gzi[v] = gxi[v];
-        zr[v] = bf2f(S.X[k][crow(v, h) * BS + j]);
-        zi[v] = bf2f(S.X[k][(32 + crow(v, h)) * BS + j]);
+        zr[v] = unpack_val(px[k], 0, v);
+        zi[v] = unpack_val(px[k], 1, v);
       }
       wave_sync();
     }
     // ---------------- layer 0 (real)
     {
       const float w = s.om[0];
+      v16f u0;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) u0[v] = s.b0[crow(v, h)];
+      u0 = mfma16(o.w0[0][lane], pf[0], u0);
+      u0 = mfma16(o.w0[1][lane], pf[1], u0);
       float gu[16];
       float om_acc = 0.f;
 #pragma unroll
