@@ -106,6 +106,113 @@ __device__ float2* fft_lines(float2* a, float2* b, int n, int log2n, int lines, 
   return a;
 }
 
+// ---------------------------------------------------------------- mixed radix
+// Radix plan of a transform length: passes of radix 4 first, then 2, 3, 5 and
+// any remaining prime factors (<= kMaxRadix).  Built on the host.
+constexpr int kMaxPasses = 24;
+constexpr int kMaxRadix = 64;
+
+struct FftPlan {
+  int n;
+  int pow2;                 // 1 when n is a power of two (shift/mask fast path)
+  int log2n;
+  int n_passes;
+  int radix[kMaxPasses];
+};
+
+// Mixed-radix Stockham autosort FFT over `lines` lines in LDS (same contract
+// as fft_lines; tw may point to LDS or global memory and holds
+// exp(-2 pi i k / (n*tws)), k < n*tws).  Radices 2, 3, 4 have dedicated
+// butterflies; other radices use a direct DFT of the R twiddled inputs.
+template <int DIR>
+__device__ float2* fft_lines_mixed(float2* a, float2* b, const FftPlan& P, int lines, int stride,
+                                   const float2* tw, int tws, int tid, int nthreads) {
+  const int n = P.n;
+  int Ns = 1;
+  for (int ps = 0; ps < P.n_passes; ++ps) {
+    const int R = P.radix[ps];
+    const int nb = n / R;
+    const int total = nb * lines;
+    const int twscale = (n / (Ns * R)) * tws;
+    const int wR = (n / R) * tws;          // twiddle stride of the R-point DFT kernel
+    for (int t = tid; t < total; t += nthreads) {
+      const int line = t / nb;
+      const int j = t - line * nb;
+      const float2* src = a + line * stride;
+      float2* dst = b + line * stride;
+      const int k = j % Ns;
+      const int d = (j - k) * R + k;
+      if (R == 4) {
+        float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb], v3 = src[j + 3 * nb];
+        if (k > 0) {
+          const int ti = k * twscale;
+          v1 = cmul(v1, twiddle<DIR>(tw, ti));
+          v2 = cmul(v2, twiddle<DIR>(tw, 2 * ti));
+          v3 = cmul(v3, twiddle<DIR>(tw, 3 * ti));
+        }
+        float2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), d13 = csub(v1, v3);
+        float2 t3 = (DIR < 0) ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
+        dst[d] = cadd(t0, t2);
+        dst[d + Ns] = cadd(t1, t3);
+        dst[d + 2 * Ns] = csub(t0, t2);
+        dst[d + 3 * Ns] = csub(t1, t3);
+      } else if (R == 2) {
+        float2 v0 = src[j], v1 = src[j + nb];
+        if (k > 0) v1 = cmul(v1, twiddle<DIR>(tw, k * twscale));
+        dst[d] = cadd(v0, v1);
+        dst[d + Ns] = csub(v0, v1);
+      } else if (R == 3) {
+        float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb];
+        if (k > 0) {
+          const int ti = k * twscale;
+          v1 = cmul(v1, twiddle<DIR>(tw, ti));
+          v2 = cmul(v2, twiddle<DIR>(tw, 2 * ti));
+        }
+        // y0 = v0+v1+v2; y1,2 = v0 - (v1+v2)/2 -+ i*DIR*(sqrt3/2)(v1-v2)
+        const float2 s12 = cadd(v1, v2), d12 = csub(v1, v2);
+        const float2 m = make_float2(v0.x - 0.5f * s12.x, v0.y - 0.5f * s12.y);
+        const float c = 0.86602540378443865f * (float)DIR;   // sin(2 pi / 3) * DIR
+        const float2 r = make_float2(-c * d12.y, c * d12.x);  // i * c * d12
+        dst[d] = cadd(v0, s12);
+        dst[d + Ns] = cadd(m, r);
+        dst[d + 2 * Ns] = csub(m, r);
+      } else {
+        for (int q = 0; q < R; ++q) {
+          float2 acc = make_float2(0.f, 0.f);
+          for (int r = 0; r < R; ++r) {
+            float2 v = src[j + r * nb];
+            if (k > 0 && r > 0) v = cmul(v, twiddle<DIR>(tw, r * k * twscale));
+            const int e = (r * q) % R;
+            if (e) v = cmul(v, twiddle<DIR>(tw, e * wR));
+            acc = cadd(acc, v);
+          }
+          dst[d + q * Ns] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    float2* tmp = a; a = b; b = tmp;
+    Ns *= R;
+  }
+  return a;
+}
+
+template <int DIR>
+__device__ __forceinline__ float2* fft_any(float2* a, float2* b, const FftPlan& P, int lines, int stride,
+                                           const float2* tw, int tws, int tid, int nthreads) {
+  if (P.pow2) return fft_lines<DIR>(a, b, P.n, P.log2n, lines, stride, tw, tws, tid, nthreads);
+  return fft_lines_mixed<DIR>(a, b, P, lines, stride, tw, tws, tid, nthreads);
+}
+
+// Frequency k of an n-point spectrum in the ds-times larger m-point spectrum
+// of the reference upsample: fftshift (roll n//2), zero pad (m-n)//2 before,
+// ifftshift (roll -(m//2)).  For even n this is k (k < n/2) or k - n + m.
+__device__ __forceinline__ int embed_freq(int k, int n, int m) {
+  int q = (k + n / 2) % n + (m - n) / 2 - m / 2;
+  q %= m;
+  return q < 0 ? q + m : q;
+}
+
 // tw[k] = exp(-2 pi i k / n), k < n, accurate sincospi.
 __device__ __forceinline__ void fill_twiddles(float2* tw, int n, int tid, int nthreads) {
   for (int k = tid; k < n; k += nthreads) {

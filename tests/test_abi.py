@@ -34,8 +34,8 @@ def test_errors_are_reported_without_a_gpu(pkg):
     d.n_features = 3
     with pytest.raises(RuntimeError, match="n_features_per_level must be 2"):
         pkg._lib.call("pty_hashgrid_fwd", d, 1, 1, 1, 1, None)
-    with pytest.raises(RuntimeError, match="power of two"):
-        pkg._lib.call("pty_upsample_fwd", 1, 12, 2, 1, 1, 1 << 30, None)
+    with pytest.raises(RuntimeError, match="prime factors"):
+        pkg._lib.call("pty_upsample_fwd", 1, 67, 2, 1, 1, 1 << 30, None)   # 67 is prime > 64
     far = pkg.farfield.far_descriptor(64, 64, 256, 256, 10, 100, "FD_loss")
     assert pkg._lib.query("pty_farfield_workspace", far) > 0
     with pytest.raises(ValueError):

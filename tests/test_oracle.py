@@ -264,6 +264,21 @@ def test_fourier_upsample_numpy_and_even_samples(n):
     assert np.allclose(y[::2, ::2], x, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,ds", [(5, 2), (7, 3), (9, 2), (8, 3), (57, 2)])
+def test_fourier_upsample_general_sizes(n, ds):
+    """Odd sizes and ds > 2: fftshift/pad/ifftshift place small frequency k at
+    q = ((k + n//2) % n + (m-n)//2 - m//2) mod m (for odd n the DC bin lands on
+    m-1, a half-cell shift the kernels reproduce)."""
+    rng = np.random.default_rng(n * 10 + ds)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    m = n * ds
+    y = orc.fourier_upsample(torch.from_numpy(x)[None, None], ds)[0, 0].numpy()
+    q = np.array([((k + n // 2) % n + (m - n) // 2 - m // 2) % m for k in range(n)])
+    Y = np.zeros((m, m), complex)
+    Y[np.ix_(q, q)] = np.fft.fft2(x)
+    assert np.allclose(y, np.fft.ifft2(Y) * ds * ds, atol=1e-12)
+
+
 def test_fourier_upsample_adjoint_formula():
     n, ds = 8, 2
     m = n * ds
