@@ -1,21 +1,26 @@
 """Throughput of the fused INR-ptychography training step on one MI355X.
 
-Workload (the reference-style "plant-like" reconstruction): canvas 1024x1024
-(INR evaluated on 512x512 and Fourier-upsampled), probe 256x256, 20x20 scan
-with 40 px step and +/-4 px jitter (400 positions), 1e5 photons per pattern
-with Poisson noise, HashGrid 16 levels x 2 features with log2_hashmap_size
-18, complex-exp MLP 64 wide with 2 hidden layers (omega 10 / 1), initProbe +
-use_residual, smooth-L1 loss, Adam lr 1e-3, batch 10, FP32.  Synthetic
-seeded data (no dataset is shipped with the reference).
+Default workload (the reference-style "plant-like" reconstruction): canvas
+1024x1024 (INR evaluated on 512x512 and Fourier-upsampled), probe 256x256,
+20x20 scan with 40 px step and +/-4 px jitter (400 positions), 1e5 photons
+per pattern with Poisson noise, HashGrid 16 levels x 2 features with
+log2_hashmap_size 18, complex-exp MLP 64 wide with 2 hidden layers (omega
+10 / 1), initProbe + use_residual, smooth-L1 loss, Adam lr 1e-3, batch 10,
+FP32.  Synthetic seeded data (no dataset is shipped with the reference).
+--workload cfg3 runs the large case (canvas 3648, INR 1824x1824, probe 512,
+50x50 scan, batch 64, FD loss, log2_hashmap_size 20); cfg0 the small one.
 
 A "step" is one optimizer step of Recovery.py:247-321 (whole-canvas INR
-forward, one batch of 10 positions, backward, Adam); the reference's "iters"
-are epochs = ceil(400/10) = 40 steps.  value = iters/s = epochs per second.
+forward, one batch of positions, backward, Adam); the reference's "iters"
+are epochs = ceil(positions/batch) steps.  value = iters/s = epochs per
+second.
 
-Also reported: the roofline of the dominant kernel (object INR backward,
-measured with HIP events on its launch stream inside the timed region) and
-the CPU baseline (the FP64/FP32 oracle restatement on the host cores, a
-bounded sample of steps).
+Also reported: the roofline of the dominant kernel (the MLP backward with
+recompute, k_inr_bwd, launched once for the object grid and once for the
+probe grid per step; HIP events on its launch stream bracket each launch
+inside the timed region), its HBM traffic from the committed rocprofv3 PMC
+summary, and the CPU baseline (the FP32 oracle restatement on the host
+cores, a bounded sample of steps).
 """
 from __future__ import annotations
 
@@ -32,19 +37,106 @@ sys.path.insert(0, str(ROOT))
 WORKLOAD = dict(canvas=1024, M=256, n_side=20, step=40, jitter=4, photons=1e5)
 LOG2T = 18
 BATCH = 10
+# throughput workloads: (generator shapes, log2 hashmap size, batch, loss, name)
+WORKLOADS = {
+    "cfg0": (dict(canvas=256, M=64, n_side=10, step=16, jitter=2, photons=1e4), 14, 10, "smooth_L1_loss",
+             "canvas256_inr128_probe64_scan10x10_batch10_log2T14_smoothL1"),
+    "cfg1": (WORKLOAD, LOG2T, BATCH, "smooth_L1_loss",
+             "canvas1024_inr512_probe256_scan20x20_batch10_log2T18_smoothL1"),
+    "cfg3": (dict(canvas=3648, M=512, n_side=50, step=64, jitter=6, photons=1e6), 20, 64, "FD_loss",
+             "canvas3648_inr1824_probe512_scan50x50_batch64_log2T20_FD"),
+}
 MLP_FLOP_BWD_PER_POINT = 37376      # dX + dW of the complex MLP per point (recompute not counted)
 MLP_FLOP_FWD_PER_POINT = 18688
-FP32_MFMA_PEAK_TFLOPS = 157.3
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}     # MI355X dense
+MLP_BWD_KERNEL = {"fp32": "k_inr_bwd<2>", "bf16": "k_inr_bwd_bf16"}
+PROFILES = ROOT / "profiles"
 
 
-def build_problem(pkg, device, seed=0, precision="fp32"):
+def measured_traffic(workload, precision):
+    """HBM bytes per launch of the MLP-backward kernel from the committed
+    rocprofv3 --pmc summary of this workload (FETCH_SIZE and WRITE_SIZE,
+    scaled by a known-size copy), or None when there is none."""
+    path = PROFILES / f"pmc_mlp_bwd_{workload}_{precision}.json"
+    if not path.exists():
+        return None
+    d = json.loads(path.read_text())
+    return d.get("hbm_bytes_per_launch")
+
+
+def pmc_summary(fetch_csv, write_csv, workload, precision, command):
+    """Write profiles/pmc_mlp_bwd_<workload>_<precision>.{json,md} from two
+    rocprofv3 --pmc counter_collection CSVs (FETCH_SIZE, WRITE_SIZE; KiB per
+    dispatch).  Each counter is scaled by what it reads for the 256 MiB
+    calibration copy (--copy-calibration) against the copy's true bytes."""
+    import csv
+    true_kib = 256 * 1024
+
+    def read(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        cal = [float(r["Counter_Value"]) for r in rows
+               if "copyBuffer" in r["Kernel_Name"] and float(r["Counter_Value"]) > true_kib / 4]
+        if not cal:
+            raise SystemExit(f"{path}: no calibration copy dispatch found")
+        factor = true_kib / (sum(cal) / len(cal))
+        import re
+        key = re.compile(r"\b" + MLP_BWD_KERNEL[precision].split("<")[0] + r"[<(]")
+        # each step launches the kernel for the object grid, then the probe grid
+        per = {"object": [], "probe": []}
+        hits = [r for r in rows if key.search(r["Kernel_Name"])]
+        if not hits or len(hits) % 2:
+            raise SystemExit(f"{path}: expected pairs of {key.pattern} dispatches, found {len(hits)}")
+        hits.sort(key=lambda r: int(r["Dispatch_Id"]))
+        for i, r in enumerate(hits):
+            per["object" if i % 2 == 0 else "probe"].append(float(r["Counter_Value"]) * factor * 1024)
+        return factor, {g: sum(v) / len(v) for g, v in per.items()}, len(hits)
+
+    ff, fetch, nf = read(fetch_csv, "FETCH_SIZE")
+    wf, write, nw = read(write_csv, "WRITE_SIZE")
+    grids = ["object", "probe"]
+    per_launch = {g: {"fetch_bytes": fetch[g], "write_bytes": write[g]} for g in grids}
+    mean = sum(fetch[g] + write[g] for g in grids) / len(grids)
+    out = {"kernel": MLP_BWD_KERNEL[precision], "workload": workload, "precision": precision,
+           "hbm_bytes_per_launch": mean,
+           "note": "mean over the object-grid and probe-grid launches",
+           "per_launch": per_launch, "dispatches": {"FETCH_SIZE": nf, "WRITE_SIZE": nw},
+           "calibration": {"copy_bytes": true_kib * 1024, "fetch_factor": ff, "write_factor": wf},
+           "command": command}
+    stem = PROFILES / f"pmc_mlp_bwd_{workload}_{precision}"
+    stem.with_suffix(".json").write_text(json.dumps(out, indent=1) + "\n")
+    lines = [f"# HBM traffic of {MLP_BWD_KERNEL[precision]} ({workload}, {precision} MLP)", "",
+             "Two rocprofv3 PMC passes (one counter each), run from the repository root on one MI355X:", "",
+             "```", command, "```", "",
+             f"Calibration: a 256 MiB device copy reads FETCH_SIZE x{ff:.4f} and WRITE_SIZE x{wf:.4f} of its "
+             "true bytes; the kernel's counters are scaled by these factors.", "",
+             "| launch | FETCH MB/launch | WRITE MB/launch |", "|---|---|---|"]
+    for g in grids:
+        lines.append(f"| {g} | {fetch[g] / 1e6:.2f} | {write[g] / 1e6:.2f} |")
+    lines += ["", f"Mean HBM bytes per launch (fetch + write): {mean / 1e6:.2f} MB"]
+    stem.with_suffix(".md").write_text("\n".join(lines) + "\n")
+    print(json.dumps(out))
+
+
+def copy_calibration(dev, mib=256, reps=3):
+    """A plain device copy of known size, for scaling the HBM byte counters."""
+    import torch
+    a = torch.empty(mib * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def build_problem(pkg, device, seed=0, precision="fp32", workload="cfg1"):
     import numpy as np
     import torch
-    inst = pkg.synthetic.make_instance(seed=seed, **WORKLOAD)
+    shapes, log2T, _, _, _ = WORKLOADS[workload]
+    inst = pkg.synthetic.make_instance(seed=seed, **shapes)
     M = inst.probe.shape[0]
     torch.manual_seed(seed)
     model = pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M,
-                                  log2_hashmap_size=LOG2T, hidden_omega_0=1.0, trainable_omega0=True,
+                                  log2_hashmap_size=log2T, hidden_omega_0=1.0, trainable_omega0=True,
                                   use_residual=True, mlp_precision=precision,
                                   probe_initial=torch.from_numpy(inst.probe.astype(np.complex64)))
     return inst, model.to(device)
@@ -285,25 +377,42 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32", help="MLP GEMM precision")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", choices=["throughput", "nrmse"], default="throughput")
-    ap.add_argument("--workload", default="cfg0", help="nrmse mode: cfg0 | cfg1 | small")
+    ap.add_argument("--workload", default=None,
+                    help="throughput: cfg1 (default) | cfg0 | cfg3;  nrmse mode: cfg0 (default) | cfg1 | small")
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB device copy (reference dispatch for HBM byte counters)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 --pmc runs of this bench into profiles/ and exit")
+    ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.workload or "cfg1", args.precision, args.pmc_command)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
     if args.mode == "nrmse":
+        args.workload = args.workload or "cfg0"
         nrmse_mode(args)
         return
+    args.workload = args.workload or "cfg1"
+    shapes, _, batch, loss_type, wl_name = WORKLOADS[args.workload]
 
     import torch
     import __graft_entry__
     pkg = __graft_entry__.load_package()
     dev = torch.device("cuda:0")
-    inst, model = build_problem(pkg, dev, precision=args.precision)
+    if args.copy_calibration:
+        copy_calibration(dev)
+    print(f"bench: generating the {wl_name} instance on the host", file=sys.stderr, flush=True)
+    inst, model = build_problem(pkg, dev, precision=args.precision, workload=args.workload)
+    print("bench: instance ready", file=sys.stderr, flush=True)
     eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
-                                  loss_type="smooth_L1_loss", lr=1e-3, batch_size=BATCH)
+                                  loss_type=loss_type, lr=1e-3, batch_size=batch)
+    steps_per_iter = (inst.stack.shape[2] + batch - 1) // batch
     frames = eng.frames
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
@@ -311,18 +420,20 @@ def main():
     state = {"perm": None, "pos": frames}
 
     def next_batch():
-        if state["pos"] >= frames:
+        # full batches only (the trailing partial batch of an epoch is skipped
+        # here so that every timed step has the same shape)
+        if state["pos"] + batch > frames:
             state["perm"] = torch.randperm(frames, device=dev, generator=gen).to(torch.int32)
             state["pos"] = 0
-        b = state["perm"][state["pos"]:state["pos"] + BATCH]
-        state["pos"] += BATCH
+        b = state["perm"][state["pos"]:state["pos"] + batch]
+        state["pos"] += batch
         return b
 
     eng.set_lr(1e-3)
     for _ in range(args.warmup):
         eng.step(next_batch())
     torch.cuda.synchronize()
-    eng.enable_timing(["obj_inr_bwd", "obj_inr_fwd", "farfield"])
+    eng.enable_timing(["obj_mlp_bwd", "prb_mlp_bwd", "obj_inr_fwd", "farfield"])
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
@@ -336,9 +447,15 @@ def main():
     ms_per_step = total_ms / args.steps
     steps_per_s = 1000.0 / ms_per_step
     tim = eng.timings_ms()
-    bwd_ms = sum(tim["obj_inr_bwd"]) / len(tim["obj_inr_bwd"])
+    # the MLP-backward kernel runs twice a step (object grid, probe grid);
+    # average launch duration and average points per launch over both
+    launches = tim["obj_mlp_bwd"] + tim["prb_mlp_bwd"]
+    bwd_ms = sum(launches) / len(launches)
     n_obj = model.downsampled_width * model.downsampled_height
-    achieved = MLP_FLOP_BWD_PER_POINT * n_obj / (bwd_ms * 1e-3) / 1e12
+    n_prb = model.probe_width * model.probe_height
+    n_avg = (n_obj + n_prb) / 2
+    achieved = MLP_FLOP_BWD_PER_POINT * n_avg / (bwd_ms * 1e-3) / 1e12
+    peak = MFMA_PEAK_TFLOPS[args.precision]
     loss_now = float(eng.loss.sum().item())
     # the same steps replayed from one captured HIP graph
     graph_ms = None
@@ -358,7 +475,7 @@ def main():
 
     line = {
         "metric": "wall-s to plain-final object NRMSE + iters/s: HIP fused vs plain, 1 MI355X",
-        "value": steps_per_s / 40.0,
+        "value": steps_per_s / steps_per_iter,
         "unit": "iters/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -369,21 +486,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f32 (MLP GEMMs bf16, fp32 accumulate)",
         "data": "synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson 1e5 photons)",
-        "config": {"workload": "canvas1024_inr512_probe256_scan20x20_batch10_log2T18_smoothL1",
-                   "steps_per_iter": 40, "global_batch": BATCH, "positions": frames,
-                   "parallelism": "single-gpu"},
+        "config": {"workload": wl_name, "steps_per_iter": steps_per_iter, "global_batch": batch,
+                   "positions": frames, "parallelism": "single-gpu"},
         "steps_per_s": steps_per_s,
         "graph_ms_per_step": graph_ms,
         "wall_s_timed": wall,
         "kernel_ms": {k: (sum(v) / len(v) if v else None) for k, v in tim.items()},
         "loss_last_step": loss_now,
-        "roofline": {"bound": "mfma", "kernel": "k_inr_bwd<2> (object, 512x512 points, + grad reduce)",
-                     "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
-                     "algorithmic": f"{MLP_FLOP_BWD_PER_POINT} FLOP/point x {n_obj} points per launch"},
+        "roofline": {"bound": "mfma", "kernel": MLP_BWD_KERNEL[args.precision],
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": measured_traffic(args.workload, args.precision),
+                     "avg_launch_ms": bwd_ms, "launches": len(launches),
+                     "algorithmic": f"{MLP_FLOP_BWD_PER_POINT} FLOP/point x {n_avg:.0f} points per launch "
+                                    f"(mean of the object grid {n_obj} and the probe grid {n_prb})"},
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "cfg1":
         line["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
     print(json.dumps(line))
 

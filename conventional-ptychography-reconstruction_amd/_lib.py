@@ -65,6 +65,10 @@ _SIGNATURES = {
     "pty_inr_bwd_workspace": (c_size_t, [P(PtyGrid), P(PtyMlp), P(PtyPoints)]),
     "pty_inr_bwd": (c_int32, [P(PtyGrid), P(PtyMlp), P(PtyMlp), c_void_p, c_void_p, P(PtyPoints),
                               c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_inr_bwd_mlp": (c_int32, [P(PtyGrid), P(PtyMlp), c_void_p, P(PtyPoints), c_void_p, c_void_p, c_size_t,
+                                  c_void_p]),
+    "pty_inr_bwd_finish": (c_int32, [P(PtyGrid), P(PtyMlp), P(PtyMlp), c_void_p, P(PtyPoints), c_void_p, c_size_t,
+                                     c_void_p]),
     "pty_points_plan_bytes": (c_size_t, [P(PtyGrid), P(PtyPoints)]),
     "pty_points_plan_build": (c_int32, [P(PtyGrid), P(PtyPoints), c_void_p, c_size_t, c_void_p]),
     "pty_mlp_fwd": (c_int32, [P(PtyMlp), c_void_p, c_int64, c_void_p, c_void_p]),

@@ -486,45 +486,88 @@ extern "C" size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, con
          table_bwd_workspace(*g, *pts);
 }
 
+static int check_bwd_args(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts, void* work,
+                          size_t work_bytes) {
+  int rc = validate_grid(g);
+  if (!rc) rc = validate_mlp(mlp, true);
+  if (!rc) rc = validate_points(pts);
+  if (rc) return rc;
+  PTY_REQUIRE(mlp->n_in == g->n_levels * g->n_features, PTY_EINVAL, "mlp n_in != levels*features");
+  if (pts->n == 0) return PTY_OK;
+  PTY_REQUIRE(work && work_bytes >= pty_inr_bwd_workspace(g, mlp, pts), PTY_EWORK, "inr_bwd: workspace too small");
+  return PTY_OK;
+}
+
+// Workspace layout shared by the two stages: per-block MLP gradient partials
+// (+ the chunk sums of the first reduce stage), the feature gradient
+// [n_levels][n] c64, then the table-gradient scratch.
+struct BwdWork {
+  float* partial;
+  float* partial2;
+  float2* gfeat;
+  char* tab_work;
+  int blocks, total;
+};
+
+static BwdWork bwd_work(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts, void* work) {
+  BwdWork b;
+  char* w = (char*)work;
+  b.blocks = bwd_blocks(pts->n);
+  b.total = (int)grad_total(mlp);
+  b.partial = (float*)w;
+  b.partial2 = b.partial + (size_t)b.blocks * b.total;
+  b.gfeat = (float2*)(w + partial_bytes(mlp, pts));
+  b.tab_work = (char*)b.gfeat + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n);
+  return b;
+}
+
+extern "C" int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float* table, const PtyPoints* pts,
+                               const float* dout, void* work, size_t work_bytes, hipStream_t stream) {
+  int rc = check_bwd_args(g, mlp, pts, work, work_bytes);
+  if (rc) return rc;
+  PTY_REQUIRE(table && dout, PTY_EINVAL, "inr_bwd: null pointer");
+  if (pts->n == 0) return PTY_OK;
+  const BwdWork b = bwd_work(g, mlp, pts, work);
+  if (mlp->precision == PTY_PREC_BF16)
+    launch_inr_bwd_bf16(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
+                        stream);
+  else if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_bwd<1>, dim3(b.blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table,
+                       b.gfeat, *pts, (const float2*)dout, b.partial);
+  else
+    hipLaunchKernelGGL(k_inr_bwd<2>, dim3(b.blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table,
+                       b.gfeat, *pts, (const float2*)dout, b.partial);
+  return check_launch("inr_bwd_mlp");
+}
+
+extern "C" int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
+                                  const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream) {
+  int rc = check_bwd_args(g, mlp, pts, work, work_bytes);
+  if (!rc) rc = validate_mlp(grad, true);
+  if (rc) return rc;
+  PTY_REQUIRE(grad->n_in == mlp->n_in && grad->n_hidden == mlp->n_hidden, PTY_EINVAL, "grad descriptor mismatch");
+  PTY_REQUIRE(dtable, PTY_EINVAL, "inr_bwd: null pointer");
+  if (pts->n == 0) return PTY_OK;
+  const BwdWork b = bwd_work(g, mlp, pts, work);
+  hipLaunchKernelGGL(k_inr_grad_reduce1, dim3((b.total + 255) / 256, kReduceChunks), dim3(256), 0, stream,
+                     (const float*)b.partial, b.blocks, b.total, b.partial2);
+  if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((b.total + 255) / 256), dim3(256), 0, stream,
+                       (const float*)b.partial2, *grad);
+  else
+    hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((b.total + 255) / 256), dim3(256), 0, stream,
+                       (const float*)b.partial2, *grad);
+  rc = check_launch("inr_bwd_finish");
+  if (rc) return rc;
+  return launch_table_bwd(*g, *pts, b.gfeat, dtable, b.tab_work, stream);
+}
+
 extern "C" int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table, float* dtable,
                            const PtyPoints* pts, const float* dout, void* work, size_t work_bytes,
                            hipStream_t stream) {
-  int rc = validate_grid(g);
-  if (!rc) rc = validate_mlp(mlp, true);
-  if (!rc) rc = validate_mlp(grad, true);
-  if (!rc) rc = validate_points(pts);
+  int rc = pty_inr_bwd_mlp(g, mlp, table, pts, dout, work, work_bytes, stream);
   if (rc) return rc;
-  PTY_REQUIRE(grad->n_in == mlp->n_in && grad->n_hidden == mlp->n_hidden, PTY_EINVAL, "grad descriptor mismatch");
-  PTY_REQUIRE(mlp->n_in == g->n_levels * g->n_features, PTY_EINVAL, "mlp n_in != levels*features");
-  PTY_REQUIRE(table && dtable && dout, PTY_EINVAL, "inr_bwd: null pointer");
-  if (pts->n == 0) return PTY_OK;
-  PTY_REQUIRE(work && work_bytes >= pty_inr_bwd_workspace(g, mlp, pts), PTY_EWORK, "inr_bwd: workspace too small");
-  const int blocks = bwd_blocks(pts->n);
-  char* w = (char*)work;
-  float* partial = (float*)w;
-  float2* gfeat = (float2*)(w + partial_bytes(mlp, pts));
-  char* tab_work = (char*)gfeat + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n);
-  const int total = (int)grad_total(mlp);
-  float* partial2 = partial + (size_t)blocks * total;
-  if (mlp->precision == PTY_PREC_BF16)
-    launch_inr_bwd_bf16(*g, *mlp, (const float2*)table, gfeat, *pts, (const float2*)dout, partial, blocks, stream);
-  else if (mlp->n_hidden == 1)
-    hipLaunchKernelGGL(k_inr_bwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, gfeat,
-                       *pts, (const float2*)dout, partial);
-  else
-    hipLaunchKernelGGL(k_inr_bwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, gfeat,
-                       *pts, (const float2*)dout, partial);
-  hipLaunchKernelGGL(k_inr_grad_reduce1, dim3((total + 255) / 256, kReduceChunks), dim3(256), 0, stream,
-                     (const float*)partial, blocks, total, partial2);
-  if (mlp->n_hidden == 1)
-    hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)partial2, *grad);
-  else
-    hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)partial2, *grad);
-  rc = check_launch("inr_bwd");
-  if (rc) return rc;
-  return launch_table_bwd(*g, *pts, gfeat, dtable, tab_work, stream);
+  return pty_inr_bwd_finish(g, mlp, grad, dtable, pts, work, work_bytes, stream);
 }
 
 extern "C" int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream) {

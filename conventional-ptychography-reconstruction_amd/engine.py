@@ -172,7 +172,8 @@ class FusedReconstruction:
     # ------------------------------------------------------------------ timing
     def enable_timing(self, names):
         """Record HIP events (on the launch stream) around the named launches
-        ('obj_inr_bwd', 'obj_inr_fwd', 'prb_inr_bwd', 'farfield', 'upsample_adj', ...)."""
+        ('obj_inr_fwd', 'obj_inr_bwd', 'prb_inr_bwd', 'farfield', and 'obj_mlp_bwd' /
+        'prb_mlp_bwd', which time the MLP-backward kernel alone)."""
         self._timers = {n: [] for n in names}
 
     def _mark(self, name, begin):
@@ -215,6 +216,22 @@ class FusedReconstruction:
         C("pty_probe_normalize_fwd", self.probe.data_ptr(), self.M, self.N, self.probe_n.data_ptr(),
           self.stats.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
 
+    def _inr_bwd(self, which, grid, mlp, gmlp, table, dtable, pdesc, dout, ws, s):
+        """INR backward; with timing on for '<which>_mlp_bwd' the MLP stage is
+        bracketed by events on its own (the reduce and table gradient follow)."""
+        C = _lib.call
+        name = which + "_mlp_bwd"
+        if self._timers is None or name not in self._timers:
+            self._mark(which + "_inr_bwd", True)
+            C("pty_inr_bwd", grid, mlp, gmlp, table.data_ptr(), dtable.data_ptr(), pdesc, dout.data_ptr(),
+              ws.data_ptr(), ws.numel(), s)
+            self._mark(which + "_inr_bwd", False)
+            return
+        self._mark(name, True)
+        C("pty_inr_bwd_mlp", grid, mlp, table.data_ptr(), pdesc, dout.data_ptr(), ws.data_ptr(), ws.numel(), s)
+        self._mark(name, False)
+        C("pty_inr_bwd_finish", grid, mlp, gmlp, dtable.data_ptr(), pdesc, ws.data_ptr(), ws.numel(), s)
+
     def step(self, batch_idx: torch.Tensor, update: bool = True):
         """One optimizer step on the positions ``batch_idx`` (int32, device)."""
         s = _lib.stream_ptr(self.device)
@@ -242,14 +259,10 @@ class FusedReconstruction:
               self.ws_res.data_ptr(), self.ws_res.numel(), s)
         C("pty_upsample_adj", self.dobj_up.data_ptr(), self.n_small, self.ds, self.dsmall.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
-        self._mark("obj_inr_bwd", True)
-        C("pty_inr_bwd", self.obj_grid, self.obj_mlp, self.obj_gmlp, m.xy_hash_encoder.params.data_ptr(),
-          self.obj_dtable.data_ptr(), self.obj_pdesc, self.dsmall.data_ptr(), self.ws_obj.data_ptr(),
-          self.ws_obj.numel(), s)
-        self._mark("obj_inr_bwd", False)
-        C("pty_inr_bwd", self.prb_grid, self.prb_mlp, self.prb_gmlp, m.probe_hash_encoder.params.data_ptr(),
-          self.prb_dtable.data_ptr(), self.prb_pdesc, self.dprb_mlp.data_ptr(), self.ws_prb.data_ptr(),
-          self.ws_prb.numel(), s)
+        self._inr_bwd("obj", self.obj_grid, self.obj_mlp, self.obj_gmlp, m.xy_hash_encoder.params,
+                      self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, s)
+        self._inr_bwd("prb", self.prb_grid, self.prb_mlp, self.prb_gmlp, m.probe_hash_encoder.params,
+                      self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, s)
         if update:
             # Adam clears the trainable gradients it consumes; frozen ones are
             # cleared with them when they exist

@@ -10,7 +10,8 @@ with photon scaling and Poisson noise.  Definitions:
 * probe: circular aperture of radius R px with a Gaussian edge (sigma 2 px)
   and quadratic phase a*r^2 rad, centred at (M/2, M/2);
 * scan: n x n raster with a fixed step plus U(-j, +j) jitter, rounded to
-  integer pixels, centred on the canvas;
+  integer pixels, centred on the canvas (clamped to it where the jitter
+  would leave it);
 * measurement: I = |fftshift(fft2(P * O_window))|^2 with one global scale on
   the probe so that mean_j sum I_j = photons, then Poisson noise; stored
   f32 [M, N, frames] (the reference layout).
@@ -63,10 +64,9 @@ def raster_scan(rng: np.random.Generator, n_side: int, step: int, jitter: int, M
     ty = ty - ty.min()
     tx = tx - tx.min()
     ext_y, ext_x = ty.max() + M, tx.max() + M
-    if ext_y > canvas or ext_x > canvas:
-        raise ValueError(f"scan extent {ext_y}x{ext_x} exceeds canvas {canvas}")
-    ty = ty + (canvas - ext_y) // 2
-    tx = tx + (canvas - ext_x) // 2
+    # centred on the canvas; jitter that would leave it is clamped
+    ty = np.clip(ty + (canvas - ext_y) // 2, 0, canvas - M)
+    tx = np.clip(tx + (canvas - ext_x) // 2, 0, canvas - M)
     return ty.astype(np.int64), tx.astype(np.int64)
 
 
@@ -96,6 +96,7 @@ class Instance:
 SHAPES = {
     "cfg0": dict(canvas=256, M=64, n_side=10, step=16, jitter=2, photons=1e4),
     "cfg1": dict(canvas=1024, M=256, n_side=20, step=40, jitter=4, photons=1e5),
+    "cfg3": dict(canvas=3648, M=512, n_side=50, step=64, jitter=6, photons=1e6),
     "tiny": dict(canvas=64, M=16, n_side=4, step=8, jitter=1, photons=1e3),
     "small": dict(canvas=128, M=32, n_side=6, step=14, jitter=1, photons=1e3),
 }

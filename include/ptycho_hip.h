@@ -140,6 +140,16 @@ size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoint
 int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table,
                 float* dtable, const PtyPoints* pts, const float* dout, void* work,
                 size_t work_bytes, hipStream_t stream);
+/* The two stages of pty_inr_bwd, for callers that time or schedule them
+ * apart (no separate counterpart in the reference; together they are the
+ * same backward).  _mlp runs the MLP backward with recompute and leaves the
+ * feature gradient and per-block MLP-gradient partials in `work`; _finish
+ * reduces the partials into *grad and forms the table gradient.  Both take
+ * the same workspace, and _finish must follow _mlp on the same stream. */
+int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float* table, const PtyPoints* pts,
+                    const float* dout, void* work, size_t work_bytes, hipStream_t stream);
+int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
+                       const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream);
 /* Static plan of a regular grid for the table gradient (no counterpart in
  * the reference: it only depends on the hash-grid layout and the axes). */
 size_t pty_points_plan_bytes(const PtyGrid* g, const PtyPoints* pts);
