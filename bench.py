@@ -485,7 +485,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f32 (MLP GEMMs bf16, fp32 accumulate)",
-        "data": "synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson 1e5 photons)",
+        "data": f"synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson {shapes['photons']:.0e} photons)",
         "config": {"workload": wl_name, "steps_per_iter": steps_per_iter, "global_batch": batch,
                    "positions": frames, "parallelism": "single-gpu"},
         "steps_per_s": steps_per_s,

@@ -211,6 +211,71 @@ def test_fused_step_matches_oracle(pkg, dev, loss_type):
     assert rel_err(eng.flat.g(model.k_probe), g["k_probe"]) < tol_p
 
 
+def test_adjoint_central_differences(pkg, dev):
+    """The hand-written adjoint at 32 random parameters against central
+    differences (h = 1e-3, FP64) of the oracle's step loss (smooth-L1, the
+    plant-like setting).  The oracle's own autograd gradient is held to the
+    same differences at 1e-5, which pins both."""
+    inst = pkg.synthetic.make_named("small", seed=3)
+    M = inst.probe.shape[0]
+    model = _model(pkg, dev, inst.canvas, M, log2T=12, residual=True, seed=7,
+                   probe_init=torch.from_numpy(inst.probe.astype(np.complex64)))
+    eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                  loss_type="smooth_L1_loss", batch_size=5)
+    batch = [2, 5, 11, 17, 30]
+    eng.step(torch.tensor(batch, dtype=torch.int32, device=dev), update=False)
+    params = oracle_params(model)
+    _, g_auto = step_reference(model, params, inst, batch, "smooth_L1_loss")
+    table = oracle_table(model)
+    from tests.helpers import oracle_geom
+    geom = oracle_geom(model, inst.tlY, inst.tlX)
+    stack = torch.from_numpy(np.asarray(inst.stack)).to(torch.float64)
+
+    def loss_now():
+        with torch.no_grad():
+            return float(orc.step_loss(params, table, geom, stack, batch, "smooth_L1_loss")[0])
+
+    def real(t):
+        t = t.detach()
+        return torch.view_as_real(t).reshape(-1) if t.is_complex() else t.reshape(-1)
+
+    # (oracle leaf, its autograd gradient, the product parameter) per tensor
+    groups = [[(params["obj_table"], g_auto["obj_table"], model.xy_hash_encoder.params)],
+              [(params["prb_table"], g_auto["prb_table"], model.probe_hash_encoder.params)]]
+    for key, mlp in (("obj_mlp", model.object_complex_mlp), ("prb_mlp", model.probe_complex_mlp)):
+        leaves = []
+        for (w, b, om) in params[key]["layers"]:
+            leaves += [w, b, om]
+        leaves += [params[key]["out_weight"], params[key]["out_bias"]]
+        groups.append(list(zip(leaves, g_auto[key], mlp.tensors())))
+    groups.append([(params["k_probe"], g_auto["k_probe"], model.k_probe)])
+    rng = np.random.default_rng(11)
+    picks = []
+    for grp, count in zip(groups, [12, 6, 6, 6, 2]):
+        for _ in range(count):
+            leaf, ga, prod = grp[int(rng.integers(len(grp)))]
+            picks.append((leaf, ga, prod, int(rng.integers(real(leaf).numel()))))
+    assert len(picks) == 32
+    h = 1e-3
+    fd, ours, auto = [], [], []
+    for leaf, ga, prod, i in picks:
+        flat = real(leaf.data)
+        x0 = flat[i].item()
+        flat[i] = x0 + h
+        lp = loss_now()
+        flat[i] = x0 - h
+        lm = loss_now()
+        flat[i] = x0
+        fd.append((lp - lm) / (2 * h))
+        auto.append(real(ga)[i].item())
+        ours.append(real(eng.flat.g(prod)).cpu().double()[i].item())
+    fd, ours, auto = map(np.asarray, (fd, ours, auto))
+    scale = np.linalg.norm(fd)
+    assert scale > 0
+    assert np.linalg.norm(auto - fd) < 1e-5 * scale, np.abs(auto - fd).max()
+    assert np.linalg.norm(ours - fd) < 2e-3 * scale, (np.linalg.norm(ours - fd) / scale)
+
+
 def test_fused_step_equals_autograd_dropin(pkg, dev):
     """The fused step and the drop-in autograd path (model.forward + loss) agree."""
     inst = pkg.synthetic.make_named("small", seed=2)
