@@ -53,10 +53,12 @@ MLP_BWD_KERNEL = {"fp32": "k_inr_bwd<2>", "bf16": "k_inr_bwd_bf16"}
 PROFILES = ROOT / "profiles"
 
 
-def measured_traffic(workload, precision):
+def measured_traffic(workload, precision, table_precision="fp32"):
     """HBM bytes per launch of the MLP-backward kernel from the committed
     rocprofv3 --pmc summary of this workload (FETCH_SIZE and WRITE_SIZE,
-    scaled by a known-size copy), or None when there is none."""
+    scaled by a known-size copy, f32 tables), or None when there is none."""
+    if table_precision != "fp32":
+        return None
     path = PROFILES / f"pmc_mlp_bwd_{workload}_{precision}.json"
     if not path.exists():
         return None
@@ -128,7 +130,7 @@ def copy_calibration(dev, mib=256, reps=3):
     del a, b
 
 
-def build_problem(pkg, device, seed=0, precision="fp32", workload="cfg1"):
+def build_problem(pkg, device, seed=0, precision="fp32", workload="cfg1", table_precision="fp32"):
     import numpy as np
     import torch
     shapes, log2T, _, _, _ = WORKLOADS[workload]
@@ -137,7 +139,7 @@ def build_problem(pkg, device, seed=0, precision="fp32", workload="cfg1"):
     torch.manual_seed(seed)
     model = pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M,
                                   log2_hashmap_size=log2T, hidden_omega_0=1.0, trainable_omega0=True,
-                                  use_residual=True, mlp_precision=precision,
+                                  use_residual=True, mlp_precision=precision, table_precision=table_precision,
                                   probe_initial=torch.from_numpy(inst.probe.astype(np.complex64)))
     return inst, model.to(device)
 
@@ -375,6 +377,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32", help="MLP GEMM precision")
+    ap.add_argument("--table-precision", choices=["fp32", "fp16"], default="fp32",
+                    help="hash-table storage the kernels gather from (f32 master kept for Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", choices=["throughput", "nrmse"], default="throughput")
     ap.add_argument("--workload", default=None,
@@ -408,7 +412,8 @@ def main():
     if args.copy_calibration:
         copy_calibration(dev)
     print(f"bench: generating the {wl_name} instance on the host", file=sys.stderr, flush=True)
-    inst, model = build_problem(pkg, dev, precision=args.precision, workload=args.workload)
+    inst, model = build_problem(pkg, dev, precision=args.precision, workload=args.workload,
+                                table_precision=args.table_precision)
     print("bench: instance ready", file=sys.stderr, flush=True)
     eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
                                   loss_type=loss_type, lr=1e-3, batch_size=batch)
@@ -484,7 +489,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f32 (MLP GEMMs bf16, fp32 accumulate)",
+        "dtype": "f32" + ("" if args.precision == "fp32" else " (MLP GEMMs bf16, fp32 accumulate)")
+                 + ("" if args.table_precision == "fp32" else " (hash tables fp16)"),
         "data": f"synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson {shapes['photons']:.0e} photons)",
         "config": {"workload": wl_name, "steps_per_iter": steps_per_iter, "global_batch": batch,
                    "positions": frames, "parallelism": "single-gpu"},
@@ -495,7 +501,7 @@ def main():
         "loss_last_step": loss_now,
         "roofline": {"bound": "mfma", "kernel": MLP_BWD_KERNEL[args.precision],
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(args.workload, args.precision),
+                     "frac": achieved / peak, "traffic": measured_traffic(args.workload, args.precision, args.table_precision),
                      "avg_launch_ms": bwd_ms, "launches": len(launches),
                      "algorithmic": f"{MLP_FLOP_BWD_PER_POINT} FLOP/point x {n_avg:.0f} points per launch "
                                     f"(mean of the object grid {n_obj} and the probe grid {n_prb})"},

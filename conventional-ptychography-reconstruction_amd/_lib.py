@@ -12,12 +12,13 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libptycho_hip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_LEVELS = 16
 MAX_HIDDEN = 4
 
 LOSS_CODES = {"smooth_L1_loss": 0, "FD_loss": 1, "Poisson_likelihood_loss": 2}
 PREC_CODES = {"fp32": 0, "bf16": 1}
+TABLE_CODES = {"fp32": 0, "fp16": 1}
 
 
 class PtyGrid(ctypes.Structure):
@@ -26,7 +27,7 @@ class PtyGrid(ctypes.Structure):
         ("log2_hashmap_size", c_int32),
         ("scale", c_float * MAX_LEVELS), ("res", c_uint32 * MAX_LEVELS),
         ("size", c_uint32 * MAX_LEVELS), ("offset", c_uint32 * MAX_LEVELS),
-        ("dense_mask", c_uint32), ("n_entries", c_uint32),
+        ("dense_mask", c_uint32), ("n_entries", c_uint32), ("table_precision", c_uint32),
     ]
 
 
@@ -94,6 +95,7 @@ _SIGNATURES = {
                            c_float, c_int32, c_void_p]),
     "pty_adam_tick": (c_int32, [c_void_p, c_void_p]),
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
+    "pty_cast_f16": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
 }
 
 _lib = None

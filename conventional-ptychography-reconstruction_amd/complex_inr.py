@@ -19,7 +19,7 @@ import torch
 import torch.nn as nn
 
 from . import _lib
-from .hashgrid import HashGridEncoding
+from .hashgrid import HashGridEncoding, table_storage
 from .levels import GridSpec
 
 
@@ -183,31 +183,34 @@ class InrFieldFn(torch.autograd.Function):
     """Complex field [n] = ComplexMLP.net(HashGrid(points)), fused on the device."""
 
     @staticmethod
-    def forward(ctx, spec: GridSpec, points, precision: str, table, *ts):
+    def forward(ctx, spec: GridSpec, points, precision: str, table_precision: str, table, *ts):
         _lib.require_cuda(table, *ts)
         ts = tuple(t.contiguous() for t in ts)
         out = torch.empty(points.n, dtype=torch.complex64, device=table.device)
-        _lib.call("pty_inr_fwd", spec.descriptor(), mlp_descriptor(ts, spec.n_output_dims, precision),
-                  table.data_ptr(), points.descriptor(spec), out.data_ptr(), _lib.stream_ptr(table.device))
-        ctx.spec, ctx.points, ctx.precision = spec, points, precision
-        ctx.save_for_backward(table, *ts)
+        storage = table_storage(table, table_precision)
+        _lib.call("pty_inr_fwd", spec.descriptor(table_precision), mlp_descriptor(ts, spec.n_output_dims, precision),
+                  storage.data_ptr(), points.descriptor(spec), out.data_ptr(), _lib.stream_ptr(table.device))
+        ctx.spec, ctx.points, ctx.precision, ctx.table_precision = spec, points, precision, table_precision
+        ctx.n_table = table.numel()
+        ctx.save_for_backward(storage, *ts)
         return out
 
     @staticmethod
     def backward(ctx, gout):
-        table, *ts = ctx.saved_tensors
+        storage, *ts = ctx.saved_tensors
         spec, points, precision = ctx.spec, ctx.points, ctx.precision
+        grid = spec.descriptor(ctx.table_precision)
         gout = gout.resolve_conj().contiguous()
         grads = [torch.zeros_like(t) for t in ts]
-        dtable = torch.zeros_like(table)
+        dtable = torch.zeros(ctx.n_table, dtype=torch.float32, device=storage.device)
         desc = mlp_descriptor(ts, spec.n_output_dims, precision)
         gdesc = mlp_descriptor(grads, spec.n_output_dims, precision)
         pdesc = points.descriptor(spec)
-        nbytes = _lib.query("pty_inr_bwd_workspace", spec.descriptor(), desc, pdesc)
-        ws = _lib.workspace(nbytes, table.device)
-        _lib.call("pty_inr_bwd", spec.descriptor(), desc, gdesc, table.data_ptr(), dtable.data_ptr(), pdesc,
-                  gout.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr(table.device))
-        return (None, None, None, dtable, *grads)
+        nbytes = _lib.query("pty_inr_bwd_workspace", grid, desc, pdesc)
+        ws = _lib.workspace(nbytes, storage.device)
+        _lib.call("pty_inr_bwd", grid, desc, gdesc, storage.data_ptr(), dtable.data_ptr(), pdesc,
+                  gout.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr(storage.device))
+        return (None, None, None, None, dtable, *grads)
 
 
 class UpsampleFn(torch.autograd.Function):
@@ -281,7 +284,7 @@ class ComplexINRModel2D(nn.Module):
                  probe_height=None, n_levels=16, n_features_per_level=2, log2_hashmap_size=19, base_resolution=16,
                  per_level_scale=1.5, first_omega_0=10.0, hidden_omega_0=10.0, hidden_features=64,
                  hidden_layers=2, trainable_omega0=False, use_residual=False, object_initial=None,
-                 probe_initial=None, mlp_precision: str = "fp32"):
+                 probe_initial=None, mlp_precision: str = "fp32", table_precision: str = "fp32"):
         super().__init__()
         self.update_probe = update_probe
         self.use_residual = use_residual
@@ -298,8 +301,12 @@ class ComplexINRModel2D(nn.Module):
             "base_resolution": base_resolution,
             "per_level_scale": per_level_scale,
         }
-        self.xy_hash_encoder = HashGridEncoding(n_input_dims=2, encoding_config=encoding_config)
-        self.probe_hash_encoder = (HashGridEncoding(n_input_dims=2, encoding_config=encoding_config.copy())
+        if table_precision not in _lib.TABLE_CODES:
+            raise ValueError(f"table_precision must be one of {tuple(_lib.TABLE_CODES)}")
+        tdtype = torch.float16 if table_precision == "fp16" else None
+        self.xy_hash_encoder = HashGridEncoding(n_input_dims=2, encoding_config=encoding_config, dtype=tdtype)
+        self.probe_hash_encoder = (HashGridEncoding(n_input_dims=2, encoding_config=encoding_config.copy(),
+                                                    dtype=tdtype)
                                    if update_probe else None)
         self.object_complex_mlp = ComplexMLP(self.num_features, hidden_features, hidden_layers, 1,
                                              first_omega_0, hidden_omega_0, trainable_omega0)
@@ -377,8 +384,8 @@ class ComplexINRModel2D(nn.Module):
 
     def _field(self, encoder: HashGridEncoding, mlp: ComplexMLP, which: str):
         mlp.check_supported()
-        return InrFieldFn.apply(encoder.spec, self.grid_points(which), self.mlp_precision, encoder.params,
-                                *mlp.tensors())
+        return InrFieldFn.apply(encoder.spec, self.grid_points(which), self.mlp_precision, encoder.table_precision,
+                                encoder.params, *mlp.tensors())
 
     def fourier_upsample(self, x, scale_factor):
         return fourier_upsample(x, scale_factor)

@@ -28,6 +28,24 @@ __global__ void k_fill(float* __restrict__ p, int64_t n, float v) {
     p[i] = v;
 }
 
+// dst = half(src), round to nearest even; four values per thread when the
+// pointers allow 16-byte loads and 8-byte stores, else one.
+__global__ void k_cast_f16_vec(const float4* __restrict__ src, int64_t n4, uint2* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 v = src[i];
+    __half2 a = __floats2half2_rn(v.x, v.y), b = __floats2half2_rn(v.z, v.w);
+    uint2 o;
+    o.x = *reinterpret_cast<uint32_t*>(&a);
+    o.y = *reinterpret_cast<uint32_t*>(&b);
+    dst[i] = o;
+  }
+}
+
+__global__ void k_cast_f16(const float* __restrict__ src, int64_t lo, int64_t n, __half* __restrict__ dst) {
+  for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = __float2half_rn(src[i]);
+}
+
 // out = k * x + initial  (k is a device scalar; complex x, initial, out)
 __global__ void k_residual_fwd(const float* __restrict__ k, const float2* __restrict__ x,
                                const float2* __restrict__ init, int64_t n, float2* __restrict__ out) {
@@ -162,6 +180,22 @@ extern "C" int pty_fill(float* p, int64_t n, float value, hipStream_t stream) {
   if (n == 0) return PTY_OK;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, n, value);
   return check_launch("fill");
+}
+
+extern "C" int pty_cast_f16(const float* src, int64_t n, void* dst, hipStream_t stream) {
+  PTY_REQUIRE(n >= 0 && (n == 0 || (src && dst)), PTY_EINVAL, "cast_f16: null pointer");
+  if (n == 0) return PTY_OK;
+  int64_t done = 0;
+  if (((uintptr_t)src & 15) == 0 && ((uintptr_t)dst & 7) == 0 && n >= 4) {
+    const int64_t n4 = n / 4;
+    hipLaunchKernelGGL(k_cast_f16_vec, dim3(grid_for(n4, 256)), dim3(256), 0, stream, (const float4*)src, n4,
+                       (uint2*)dst);
+    done = n4 * 4;
+  }
+  if (done < n)
+    hipLaunchKernelGGL(k_cast_f16, dim3(grid_for(n - done, 256)), dim3(256), 0, stream, src, done, n,
+                       (__half*)dst);
+  return check_launch("cast_f16");
 }
 
 extern "C" int pty_residual_fwd(const float* k, const float* x, const float* initial, int64_t n, float* out,

@@ -1,6 +1,7 @@
 // Shared device/host helpers for the gfx950 ptychography kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdarg.h>

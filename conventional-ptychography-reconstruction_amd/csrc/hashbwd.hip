@@ -352,6 +352,8 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
   const bool mask_lds = words <= kMaskLds;
   if (mask_lds)
     for (int w = threadIdx.x; w < words; w += blockDim.x) s_mask[w] = mk[w];
+  // the mask is read by every thread below (both paths)
+  __syncthreads();
   // all 256 entries of the chunk share e >> b; a row gives at most per_row columns
   const int per_row = (int)(((uint32_t)res + T) / T);
   const bool shared_key = b >= 8 && per_row <= kCand && res < 65536;

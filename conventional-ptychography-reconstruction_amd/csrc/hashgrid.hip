@@ -44,6 +44,8 @@ int validate_grid(const PtyGrid* g) {
   PTY_REQUIRE(g->n_levels >= 1 && g->n_levels <= PTY_MAX_LEVELS, PTY_EUNSUP,
               "n_levels %d outside 1..%d", g->n_levels, PTY_MAX_LEVELS);
   PTY_REQUIRE(g->n_features == 2, PTY_EUNSUP, "n_features_per_level must be 2 (got %d)", g->n_features);
+  PTY_REQUIRE(g->table_precision == PTY_TABLE_F32 || g->table_precision == PTY_TABLE_F16, PTY_EUNSUP,
+              "table_precision %u is neither PTY_TABLE_F32 nor PTY_TABLE_F16", g->table_precision);
   PTY_REQUIRE(g->log2_hashmap_size >= 3 && g->log2_hashmap_size <= 28, PTY_EUNSUP,
               "log2_hashmap_size %d outside 3..28", g->log2_hashmap_size);
   for (int l = 0; l < g->n_levels; ++l) {

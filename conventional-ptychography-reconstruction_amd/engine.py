@@ -122,8 +122,15 @@ class FusedReconstruction:
         nin = model.num_features
         self.obj_spec = model.xy_hash_encoder.spec
         self.prb_spec = model.probe_hash_encoder.spec
-        self.obj_grid = self.obj_spec.descriptor()
-        self.prb_grid = self.prb_spec.descriptor()
+        enc_o, enc_p = model.xy_hash_encoder, model.probe_hash_encoder
+        self.obj_grid = enc_o.descriptor()
+        self.prb_grid = enc_p.descriptor()
+        # tables the kernels read: the f32 master itself, or an FP16 copy
+        # refreshed at the start of every forward
+        self.obj_tab = (enc_o.params if enc_o.table_precision == "fp32" else
+                        torch.empty(enc_o.params.numel(), dtype=torch.float16, device=dev))
+        self.prb_tab = (enc_p.params if enc_p.table_precision == "fp32" else
+                        torch.empty(enc_p.params.numel(), dtype=torch.float16, device=dev))
         self.obj_pts = model.grid_points("object")
         self.prb_pts = model.grid_points("probe")
         self.obj_pdesc = self.obj_pts.descriptor(self.obj_spec)
@@ -195,12 +202,19 @@ class FusedReconstruction:
     def set_lr(self, lr: float):
         _lib.call("pty_fill", self.hyper.data_ptr(), 1, float(lr), _lib.stream_ptr(self.device))
 
+    def _refresh_tables(self, s):
+        m = self.model
+        for enc, tab in ((m.xy_hash_encoder, self.obj_tab), (m.probe_hash_encoder, self.prb_tab)):
+            if tab.dtype == torch.float16:
+                _lib.call("pty_cast_f16", enc.params.data_ptr(), enc.params.numel(), tab.data_ptr(), s)
+
     def forward_fields(self):
         s = _lib.stream_ptr(self.device)
         C = _lib.call
         m = self.model
+        self._refresh_tables(s)
         self._mark("obj_inr_fwd", True)
-        C("pty_inr_fwd", self.obj_grid, self.obj_mlp, m.xy_hash_encoder.params.data_ptr(), self.obj_pdesc,
+        C("pty_inr_fwd", self.obj_grid, self.obj_mlp, self.obj_tab.data_ptr(), self.obj_pdesc,
           self.obj_small.data_ptr(), s)
         self._mark("obj_inr_fwd", False)
         C("pty_upsample_fwd", self.obj_small.data_ptr(), self.n_small, self.ds, self.obj_up.data_ptr(),
@@ -208,7 +222,7 @@ class FusedReconstruction:
         if m.has_object_residual:
             C("pty_residual_fwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), m.object_initial.data_ptr(),
               self.obj_up.numel(), self.obj.data_ptr(), s)
-        C("pty_inr_fwd", self.prb_grid, self.prb_mlp, m.probe_hash_encoder.params.data_ptr(), self.prb_pdesc,
+        C("pty_inr_fwd", self.prb_grid, self.prb_mlp, self.prb_tab.data_ptr(), self.prb_pdesc,
           self.prb_mlp_out.data_ptr(), s)
         if m.has_probe_residual:
             C("pty_residual_fwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), m.probe_initial.data_ptr(),
@@ -259,9 +273,9 @@ class FusedReconstruction:
               self.ws_res.data_ptr(), self.ws_res.numel(), s)
         C("pty_upsample_adj", self.dobj_up.data_ptr(), self.n_small, self.ds, self.dsmall.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
-        self._inr_bwd("obj", self.obj_grid, self.obj_mlp, self.obj_gmlp, m.xy_hash_encoder.params,
+        self._inr_bwd("obj", self.obj_grid, self.obj_mlp, self.obj_gmlp, self.obj_tab,
                       self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, s)
-        self._inr_bwd("prb", self.prb_grid, self.prb_mlp, self.prb_gmlp, m.probe_hash_encoder.params,
+        self._inr_bwd("prb", self.prb_grid, self.prb_mlp, self.prb_gmlp, self.prb_tab,
                       self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, s)
         if update:
             # Adam clears the trainable gradients it consumes; frozen ones are

@@ -46,8 +46,11 @@ class GridSpec:
     def n_output_dims(self) -> int:
         return self.n_levels * self.n_features
 
-    def descriptor(self) -> _lib.PtyGrid:
+    def descriptor(self, table_precision: str = "fp32") -> _lib.PtyGrid:
+        """C descriptor; ``table_precision`` names the storage the kernels read
+        ('fp32' params, or 'fp16' for the half copy made by pty_cast_f16)."""
         d = _lib.PtyGrid()
+        d.table_precision = _lib.TABLE_CODES[table_precision]
         d.abi_version = _lib.ABI_VERSION
         d.n_levels = self.n_levels
         d.n_features = self.n_features

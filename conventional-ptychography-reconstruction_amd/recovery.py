@@ -5,7 +5,8 @@ sections, float/int coercion lists, enum validation), :80-158 (defaults,
 init_type logic), :201-222 (model construction), :227-350 (training loop:
 per-epoch randperm, ceil(frames/batch) Adam steps, StepLR per epoch).
 Additive keys: ``canvas_size`` (explicit square canvas; the scan is centred
-on it), ``mlp_precision`` ("fp32" | "bf16"), ``seed``.
+on it), ``mlp_precision`` ("fp32" | "bf16"), ``table_precision`` ("fp32" |
+"fp16", the hash-table storage the kernels gather from), ``seed``.
 """
 from __future__ import annotations
 
@@ -108,7 +109,7 @@ def build(cfg: dict, data: dict, device="cuda", object_initial=None):
         per_level_scale=cfg.get("per_level_scale", 1.5), first_omega_0=cfg.get("first_omega_0", 10.0),
         hidden_omega_0=cfg.get("hidden_omega_0", 1.0), hidden_features=cfg.get("hidden_features", 64),
         hidden_layers=cfg.get("hidden_layers", 2), trainable_omega0=cfg.get("trainable_omega0", True),
-        mlp_precision=cfg.get("mlp_precision", "fp32"),
+        mlp_precision=cfg.get("mlp_precision", "fp32"), table_precision=cfg.get("table_precision", "fp32"),
     ).to(device)
     stack = torch.as_tensor(np.asarray(data["imRaw"]), dtype=torch.float32).to(device)
     engine = FusedReconstruction(model, stack, tlY, tlX, loss_type=cfg.get("loss_type", "smooth_L1_loss"),

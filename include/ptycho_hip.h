@@ -31,7 +31,7 @@ extern "C" {
 
 typedef struct ihipStream_t* hipStream_t;
 
-#define PTY_ABI_VERSION 2
+#define PTY_ABI_VERSION 3
 #define PTY_MAX_LEVELS 16
 #define PTY_MAX_HIDDEN 4
 
@@ -45,6 +45,8 @@ enum {
 
 enum { PTY_LOSS_SMOOTH_L1 = 0, PTY_LOSS_FD = 1, PTY_LOSS_POISSON = 2 };
 enum { PTY_PREC_FP32 = 0, PTY_PREC_BF16 = 1 };
+/* Hash-table storage read by the forward kernels. */
+enum { PTY_TABLE_F32 = 0, PTY_TABLE_F16 = 1 };
 
 /* Multiresolution hash-grid layout (2-D input, F = 2 features per level).
  * Built on the host from (n_levels, log2_hashmap_size, base_resolution,
@@ -64,6 +66,10 @@ typedef struct PtyGrid {
   uint32_t offset[PTY_MAX_LEVELS];
   uint32_t dense_mask;
   uint32_t n_entries;
+  /* Storage of the table that forward kernels read: PTY_TABLE_F32 (float
+   * [n_entries*2]) or PTY_TABLE_F16 (half [n_entries*2], the FP16 copy that
+   * pty_cast_f16 makes of the f32 master).  Gradients are always f32. */
+  uint32_t table_precision;
 } PtyGrid;
 
 /* Complex-exponential MLP (ComplexMLP with hidden_features = 64, i.e. 32
@@ -216,6 +222,12 @@ int pty_adam_tick(float* hyper, hipStream_t stream);
 
 /* fill kernel (graph-safe replacement for hipMemsetAsync) */
 int pty_fill(float* p, int64_t n, float value, hipStream_t stream);
+
+/* dst [n] half = round-to-nearest-even(src [n] f32): the FP16 table copy the
+ * forward kernels read with PTY_TABLE_F16 (tcnn's forward casts its f32
+ * parameters to half the same way, complex_inr.py:138-145 uses its default
+ * half precision). */
+int pty_cast_f16(const float* src, int64_t n, void* dst, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,31 @@ def test_errors_are_reported_without_a_gpu(pkg):
         pkg.farfield.far_descriptor(64, 64, 256, 256, 10, 100, "l2")
 
 
+def test_table_precision_plumbing(pkg):
+    """FP16 table storage: descriptor code, encoder dtype mapping, host checks."""
+    import pytest
+    import torch
+    spec = pkg.level_table(16, 2, 14, 16, 1.5)
+    assert spec.descriptor().table_precision == 0
+    assert spec.descriptor("fp16").table_precision == 1
+    cfg = {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2, "log2_hashmap_size": 10,
+           "base_resolution": 16, "per_level_scale": 1.5}
+    assert pkg.HashGridEncoding(2, cfg).table_precision == "fp32"
+    enc = pkg.HashGridEncoding(2, cfg, dtype=torch.float16)
+    assert enc.table_precision == "fp16" and enc.descriptor().table_precision == 1
+    assert enc.params.dtype == torch.float32          # the f32 master stays the parameter
+    with pytest.raises(ValueError):
+        pkg.HashGridEncoding(2, cfg, dtype=torch.bfloat16)
+    with pytest.raises(ValueError):
+        pkg.ComplexINRModel2D(32, 32, 2, probe_width=16, probe_height=16, table_precision="fp8")
+    d = spec.descriptor()
+    d.table_precision = 7
+    with pytest.raises(RuntimeError, match="table_precision"):
+        pkg._lib.call("pty_hashgrid_fwd", d, 1, 1, 1, 1, None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        pkg._lib.call("pty_cast_f16", None, 8, None, None)
+
+
 def test_model_registration_order(pkg):
     """named_parameters order of the reference (checkpoint format, utils/io.py:11-14)."""
     import torch

@@ -15,12 +15,22 @@ from tests.helpers import oracle_params, oracle_table, rel_err, step_reference
 pytestmark = pytest.mark.gpu
 
 
-def _model(pkg, dev, canvas, M, log2T=12, residual=True, probe_init=None, seed=0, hidden_layers=2):
+def _model(pkg, dev, canvas, M, log2T=12, residual=True, probe_init=None, seed=0, hidden_layers=2,
+           table_precision="fp32"):
     torch.manual_seed(seed)
     init = probe_init if probe_init is not None else torch.ones(M, M, dtype=torch.complex64)
     return pkg.ComplexINRModel2D(canvas, canvas, 2, probe_width=M, probe_height=M, log2_hashmap_size=log2T,
                                  hidden_omega_0=1.0, trainable_omega0=True, use_residual=residual,
-                                 probe_initial=init if residual else None, hidden_layers=hidden_layers).to(dev)
+                                 probe_initial=init if residual else None, hidden_layers=hidden_layers,
+                                 table_precision=table_precision).to(dev)
+
+
+def _round_tables_to_fp16(params):
+    """Oracle leaves holding the tables as the FP16 kernels see them."""
+    for key in ("obj_table", "prb_table"):
+        t = params[key].detach().to(torch.float16).to(torch.float64)
+        params[key] = t.clone().requires_grad_(True)
+    return params
 
 
 # ---------------------------------------------------------------- hash grid
@@ -78,6 +88,51 @@ def test_inr_field_fwd_bwd(pkg, dev, hidden_layers):
     theirs += [mlp["out_weight"], mlp["out_bias"]]
     for a, b in zip(ours, theirs):
         assert rel_err(a.grad, b.grad) < 1e-4, (a.shape, rel_err(a.grad, b.grad))
+
+
+def test_inr_field_fp16_table(pkg, dev):
+    """FP16 table storage: the kernels gather half entries and interpolate in
+    f32, so against the oracle evaluated at the half-rounded table the field
+    and every gradient keep the FP32 bars; against the unrounded table the
+    difference is of the order of half rounding."""
+    model = _model(pkg, dev, 64, 16, log2T=12, residual=False, table_precision="fp16")
+    with torch.no_grad():
+        model.xy_hash_encoder.params.uniform_(-1, 1)
+    f = model._field(model.xy_hash_encoder, model.object_complex_mlp, "object")
+    params = oracle_params(model)
+    table = oracle_table(model)
+    exact = orc.inr_field(model.xy_coords.cpu(), params["obj_table"].detach(), table, params["obj_mlp"], 32, 32)
+    params = _round_tables_to_fp16(params)
+    ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], table, params["obj_mlp"], 32, 32)
+    assert rel_err(f.reshape(32, 32), ref) < 2e-6
+    assert 1e-5 < rel_err(f.reshape(32, 32), exact) < 1e-2
+    g = torch.randn(f.shape, dtype=torch.complex64, device=dev)
+    (torch.real(torch.conj(g) * f)).sum().backward()
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref.reshape(-1))).sum().backward()
+    assert model.xy_hash_encoder.params.grad.dtype == torch.float32
+    assert rel_err(model.xy_hash_encoder.params.grad, params["obj_table"].grad) < 1e-4
+    mlp = params["obj_mlp"]
+    theirs = []
+    for (w, b, om) in mlp["layers"]:
+        theirs += [w, b, om]
+    theirs += [mlp["out_weight"], mlp["out_bias"]]
+    for a, b in zip(model.object_complex_mlp.tensors(), theirs):
+        assert rel_err(a.grad, b.grad) < 1e-4, (a.shape, rel_err(a.grad, b.grad))
+
+
+def test_hash_encode_fp16_table(pkg, dev):
+    enc = pkg.HashGridEncoding(2, {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2,
+                                   "log2_hashmap_size": 14, "base_resolution": 16, "per_level_scale": 1.5},
+                               dtype=torch.float16).to(dev)
+    assert enc.table_precision == "fp16" and enc.params.dtype == torch.float32
+    with torch.no_grad():
+        enc.params.uniform_(-1, 1)
+    x = torch.rand(3000, 2, generator=torch.Generator().manual_seed(4))
+    out = enc(x.to(dev))
+    table = orc.level_table(16, 14, 16, 1.5)
+    rounded = enc.params.detach().cpu().to(torch.float16).to(torch.float64)
+    ref = orc.hashgrid_encode(x.numpy(), rounded, table)     # f32 coordinates, as the kernel sees them
+    assert rel_err(out, ref) < 1e-6
 
 
 def test_mlp_forward_from_features(pkg, dev):
@@ -211,6 +266,29 @@ def test_fused_step_matches_oracle(pkg, dev, loss_type):
     assert rel_err(eng.flat.g(model.k_probe), g["k_probe"]) < tol_p
 
 
+def test_fused_step_fp16_tables(pkg, dev):
+    """Fused step with FP16 table storage against the oracle at the
+    half-rounded tables (FP32 bars of the fused-step test)."""
+    inst = pkg.synthetic.make_named("small", seed=1)
+    M = inst.probe.shape[0]
+    model = _model(pkg, dev, inst.canvas, M, log2T=12, residual=True, table_precision="fp16",
+                   probe_init=torch.from_numpy(inst.probe.astype(np.complex64)))
+    with torch.no_grad():   # table values where half rounding is visible
+        model.xy_hash_encoder.params.uniform_(-0.3, 0.3)
+        model.probe_hash_encoder.params.uniform_(-0.3, 0.3)
+    eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                  loss_type="smooth_L1_loss", batch_size=5)
+    batch = [0, 7, 13, 22, 35]
+    loss = eng.step(torch.tensor(batch, dtype=torch.int32, device=dev), update=False).sum().item()
+    params = _round_tables_to_fp16(oracle_params(model))
+    ref_loss, g = step_reference(model, params, inst, batch, "smooth_L1_loss")
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss), (loss, ref_loss)
+    assert rel_err(eng.flat.g(model.xy_hash_encoder.params), g["obj_table"]) < 2e-3
+    assert rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"]) < 2e-3
+    for a, b in zip(model.object_complex_mlp.tensors(), g["obj_mlp"]):
+        assert rel_err(eng.flat.g(a), b) < 2e-3, ("obj", a.shape, rel_err(eng.flat.g(a), b))
+
+
 def test_adjoint_central_differences(pkg, dev):
     """The hand-written adjoint at 32 random parameters against central
     differences (h = 1e-3, FP64) of the oracle's step loss (smooth-L1, the
@@ -336,9 +414,13 @@ def test_full_size_properties(pkg, dev):
         assert abs(I[b].double().sum() / (256 * 256) - ew) < 1e-5 * ew
 
 
-@pytest.mark.parametrize("log2T", [10, 14, 18])
+@pytest.mark.parametrize("log2T", [10, 12, 14, 18])
 def test_table_gradient_paths_agree(pkg, dev, log2T):
-    """Regular-grid owner-computes (cached plan / plan built per call) == explicit-coords scatter == oracle."""
+    """Regular-grid owner-computes (cached plan / plan built per call) == explicit-coords scatter == oracle.
+
+    log2T 10 and 12 have hashed levels with fewer than 2^8 columns per hash
+    bucket (the per-entry path of k_table_hashed), 14 and 18 the shared-bucket
+    path; the grid path runs three times and must give the same bits."""
     torch.manual_seed(3)
     model = _model(pkg, dev, 128, 16, log2T=log2T, residual=False)
     with torch.no_grad():
@@ -350,7 +432,7 @@ def test_table_gradient_paths_agree(pkg, dev, log2T):
     coords = pkg.CoordPoints(model.xy_coords.to(dev))
     g = torch.randn(grid.n, dtype=torch.complex64, device=dev)
     outs = []
-    for pts, with_spec in ((grid, True), (grid, False), (coords, False)):
+    for pts, with_spec in ((grid, True), (grid, True), (grid, False), (coords, False)):
         pdesc = pts.descriptor(spec if with_spec else None)
         grads = [torch.zeros_like(t) for t in ts]
         dtab = torch.zeros_like(table)
@@ -360,8 +442,8 @@ def test_table_gradient_paths_agree(pkg, dev, log2T):
         pkg._lib.call("pty_inr_bwd", spec.descriptor(), desc, gdesc, table.data_ptr(), dtab.data_ptr(), pdesc,
                       g.data_ptr(), ws.data_ptr(), ws.numel(), pkg._lib.stream_ptr(dev))
         outs.append(dtab.clone())
-    assert torch.equal(outs[0], outs[1])              # deterministic, same kernels
-    assert rel_err(outs[0], outs[2]) < 1e-5           # gather == scatter
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])   # deterministic
+    assert rel_err(outs[0], outs[3]) < 1e-5           # gather == scatter
     params = oracle_params(model)
     ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], oracle_table(model), params["obj_mlp"], 64, 64)
     (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref.reshape(-1))).sum().backward()
