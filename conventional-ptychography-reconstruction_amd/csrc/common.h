@@ -121,10 +121,69 @@ struct FftPlan {
   int radix[kMaxPasses];
 };
 
+// One Stockham pass of prime radix R (5 <= R <= 31) over `lines` lines: each
+// thread loads its R inputs into registers, applies the stage twiddles and
+// forms the R-point DFT from the conjugate pairs a_r = v_r + v_{R-r},
+// b_r = v_r - v_{R-r}:  y_q = v_0 + sum_r a_r cos(2 pi rq/R) + i DIR sum_r b_r sin(2 pi rq/R)
+// and y_{R-q} the same with the sine term negated (q, r = 1..(R-1)/2).
+template <int R, int DIR>
+__device__ void fft_pass_prime(const float2* __restrict__ a, float2* __restrict__ b, int n, int Ns, int lines,
+                               int stride, const float2* tw, int tws, int tid, int nthreads) {
+  constexpr int Hh = (R - 1) / 2;
+  float cs[R], sn[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) sincospif(2.0f * (float)e / (float)R, &sn[e], &cs[e]);
+  const int nb = n / R;
+  const int total = nb * lines;
+  const int twscale = (n / (Ns * R)) * tws;
+  for (int t = tid; t < total; t += nthreads) {
+    const int line = t / nb;
+    const int j = t - line * nb;
+    const float2* src = a + line * stride;
+    float2* dst = b + line * stride;
+    const int k = j % Ns;
+    const int d = (j - k) * R + k;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+    if (k > 0) {
+      const int ti = k * twscale;
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, r * ti));
+    }
+    float2 pa[Hh], pb[Hh];
+    float2 y0 = v[0];
+#pragma unroll
+    for (int r = 1; r <= Hh; ++r) {
+      pa[r - 1] = cadd(v[r], v[R - r]);
+      pb[r - 1] = csub(v[r], v[R - r]);
+      y0 = cadd(y0, pa[r - 1]);
+    }
+    dst[d] = y0;
+#pragma unroll
+    for (int q = 1; q <= Hh; ++q) {
+      float2 tq = v[0], uq = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int r = 1; r <= Hh; ++r) {
+        const int e = (r * q) % R;
+        tq.x += pa[r - 1].x * cs[e];
+        tq.y += pa[r - 1].y * cs[e];
+        uq.x += pb[r - 1].x * sn[e];
+        uq.y += pb[r - 1].y * sn[e];
+      }
+      // i * DIR * u
+      const float2 iu = (DIR > 0) ? make_float2(-uq.y, uq.x) : make_float2(uq.y, -uq.x);
+      dst[d + q * Ns] = cadd(tq, iu);
+      dst[d + (R - q) * Ns] = csub(tq, iu);
+    }
+  }
+}
+
 // Mixed-radix Stockham autosort FFT over `lines` lines in LDS (same contract
 // as fft_lines; tw may point to LDS or global memory and holds
 // exp(-2 pi i k / (n*tws)), k < n*tws).  Radices 2, 3, 4 have dedicated
-// butterflies; other radices use a direct DFT of the R twiddled inputs.
+// butterflies, primes 5..31 the register pass above; larger primes use a
+// direct DFT of the R twiddled inputs.
 template <int DIR>
 __device__ float2* fft_lines_mixed(float2* a, float2* b, const FftPlan& P, int lines, int stride,
                                    const float2* tw, int tws, int tid, int nthreads) {
@@ -132,65 +191,82 @@ __device__ float2* fft_lines_mixed(float2* a, float2* b, const FftPlan& P, int l
   int Ns = 1;
   for (int ps = 0; ps < P.n_passes; ++ps) {
     const int R = P.radix[ps];
-    const int nb = n / R;
-    const int total = nb * lines;
-    const int twscale = (n / (Ns * R)) * tws;
-    const int wR = (n / R) * tws;          // twiddle stride of the R-point DFT kernel
-    for (int t = tid; t < total; t += nthreads) {
-      const int line = t / nb;
-      const int j = t - line * nb;
-      const float2* src = a + line * stride;
-      float2* dst = b + line * stride;
-      const int k = j % Ns;
-      const int d = (j - k) * R + k;
-      if (R == 4) {
-        float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb], v3 = src[j + 3 * nb];
-        if (k > 0) {
-          const int ti = k * twscale;
-          v1 = cmul(v1, twiddle<DIR>(tw, ti));
-          v2 = cmul(v2, twiddle<DIR>(tw, 2 * ti));
-          v3 = cmul(v3, twiddle<DIR>(tw, 3 * ti));
-        }
-        float2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), d13 = csub(v1, v3);
-        float2 t3 = (DIR < 0) ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
-        dst[d] = cadd(t0, t2);
-        dst[d + Ns] = cadd(t1, t3);
-        dst[d + 2 * Ns] = csub(t0, t2);
-        dst[d + 3 * Ns] = csub(t1, t3);
-      } else if (R == 2) {
-        float2 v0 = src[j], v1 = src[j + nb];
-        if (k > 0) v1 = cmul(v1, twiddle<DIR>(tw, k * twscale));
-        dst[d] = cadd(v0, v1);
-        dst[d + Ns] = csub(v0, v1);
-      } else if (R == 3) {
-        float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb];
-        if (k > 0) {
-          const int ti = k * twscale;
-          v1 = cmul(v1, twiddle<DIR>(tw, ti));
-          v2 = cmul(v2, twiddle<DIR>(tw, 2 * ti));
-        }
-        // y0 = v0+v1+v2; y1,2 = v0 - (v1+v2)/2 -+ i*DIR*(sqrt3/2)(v1-v2)
-        const float2 s12 = cadd(v1, v2), d12 = csub(v1, v2);
-        const float2 m = make_float2(v0.x - 0.5f * s12.x, v0.y - 0.5f * s12.y);
-        const float c = 0.86602540378443865f * (float)DIR;   // sin(2 pi / 3) * DIR
-        const float2 r = make_float2(-c * d12.y, c * d12.x);  // i * c * d12
-        dst[d] = cadd(v0, s12);
-        dst[d + Ns] = cadd(m, r);
-        dst[d + 2 * Ns] = csub(m, r);
-      } else {
-        for (int q = 0; q < R; ++q) {
-          float2 acc = make_float2(0.f, 0.f);
-          for (int r = 0; r < R; ++r) {
-            float2 v = src[j + r * nb];
-            if (k > 0 && r > 0) v = cmul(v, twiddle<DIR>(tw, r * k * twscale));
-            const int e = (r * q) % R;
-            if (e) v = cmul(v, twiddle<DIR>(tw, e * wR));
-            acc = cadd(acc, v);
+    switch (R) {
+      case 5: fft_pass_prime<5, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 7: fft_pass_prime<7, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 11: fft_pass_prime<11, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 13: fft_pass_prime<13, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 17: fft_pass_prime<17, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 19: fft_pass_prime<19, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 23: fft_pass_prime<23, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 29: fft_pass_prime<29, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      case 31: fft_pass_prime<31, DIR>(a, b, n, Ns, lines, stride, tw, tws, tid, nthreads); goto next;
+      default: break;
+    }
+    {
+      const int nb = n / R;
+      const int total = nb * lines;
+      const int twscale = (n / (Ns * R)) * tws;
+      const int wR = (n / R) * tws;          // twiddle stride of the R-point DFT kernel
+      for (int t = tid; t < total; t += nthreads) {
+        const int line = t / nb;
+        const int j = t - line * nb;
+        const float2* src = a + line * stride;
+        float2* dst = b + line * stride;
+        const int k = j % Ns;
+        const int d = (j - k) * R + k;
+        if (R == 4) {
+          float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb], v3 = src[j + 3 * nb];
+          if (k > 0) {
+            const int ti = k * twscale;
+            v1 = cmul(v1, twiddle<DIR>(tw, ti));
+            v2 = cmul(v2, twiddle<DIR>(tw, 2 * ti));
+            v3 = cmul(v3, twiddle<DIR>(tw, 3 * ti));
           }
-          dst[d + q * Ns] = acc;
+          float2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), d13 = csub(v1, v3);
+          float2 t3 = (DIR < 0) ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
+          dst[d] = cadd(t0, t2);
+          dst[d + Ns] = cadd(t1, t3);
+          dst[d + 2 * Ns] = csub(t0, t2);
+          dst[d + 3 * Ns] = csub(t1, t3);
+        } else if (R == 2) {
+          float2 v0 = src[j], v1 = src[j + nb];
+          if (k > 0) v1 = cmul(v1, twiddle<DIR>(tw, k * twscale));
+          dst[d] = cadd(v0, v1);
+          dst[d + Ns] = csub(v0, v1);
+        } else if (R == 3) {
+          float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb];
+          if (k > 0) {
+            const int ti = k * twscale;
+            v1 = cmul(v1, twiddle<DIR>(tw, ti));
+            v2 = cmul(v2, twiddle<DIR>(tw, 2 * ti));
+          }
+          // y0 = v0+v1+v2; y1,2 = v0 - (v1+v2)/2 -+ i*DIR*(sqrt3/2)(v1-v2)
+          const float2 s12 = cadd(v1, v2), d12 = csub(v1, v2);
+          const float2 m = make_float2(v0.x - 0.5f * s12.x, v0.y - 0.5f * s12.y);
+          const float c = 0.86602540378443865f * (float)DIR;   // sin(2 pi / 3) * DIR
+          const float2 r = make_float2(-c * d12.y, c * d12.x);  // i * c * d12
+          dst[d] = cadd(v0, s12);
+          dst[d + Ns] = cadd(m, r);
+          dst[d + 2 * Ns] = csub(m, r);
+        } else {
+          for (int q = 0; q < R; ++q) {
+            float2 acc = make_float2(0.f, 0.f);
+            int e = 0;                         // (r * q) mod R, stepped
+            for (int r = 0; r < R; ++r) {
+              float2 v = src[j + r * nb];
+              if (k > 0 && r > 0) v = cmul(v, twiddle<DIR>(tw, r * k * twscale));
+              if (e) v = cmul(v, twiddle<DIR>(tw, e * wR));
+              acc = cadd(acc, v);
+              e += q;
+              if (e >= R) e -= R;
+            }
+            dst[d + q * Ns] = acc;
+          }
         }
       }
     }
+  next:
     __syncthreads();
     float2* tmp = a; a = b; b = tmp;
     Ns *= R;

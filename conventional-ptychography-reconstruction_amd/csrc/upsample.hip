@@ -5,17 +5,19 @@
 // frequency q(k) = ((k + n//2) % n + (m-n)//2 - m//2) mod m (embed_freq; for
 // even n: k below n/2, k - n + m above, Nyquist on the negative side; for odd
 // n the DC bin lands on q = m-1 exactly as in the reference); every other big
-// frequency is zero.  The 2-D transform is three passes of batched LDS FFTs
-// (mixed radix: any n whose prime factors are <= 64):
-//   U1 rows   : X1 = FFT_n along rows of small                      [n, n]
-//   U2 columns: Z  = IFFT_m(embed_q(FFT_n(X1 column)))              [m, n]
-//   U3 rows   : big = IFFT_m(embed_q(Z row)) / n^2                  [m, m]
-// Adjoint (what autograd of the reference computes), net scale 1:
-//   A1 rows   : W1 = crop_q(FFT_m(gbig row))                        [m, n]
-//   A2 columns: W2 = IFFT_n(crop_q(FFT_m(W1 column)))               [n, n]
-//   A3 rows   : gsmall = IFFT_n(W2 row) / n^2                       [n, n]
-// Twiddles live in LDS when the line buffers leave room, else in a global
-// table in the workspace (filled at the start of every call).
+// frequency is zero.  The map is separable: with the per-line operator
+// U = IFFT_m o embed_q o FFT_n (n samples in, m out),
+//   big = U_rows(U_cols(small)) / n^2,
+// and its adjoint (what autograd of the reference computes, net scale 1) is
+//   gsmall = V_rows(V_cols(gbig)) / n^2,  V = IFFT_n o crop_q o FFT_m.
+// Each stage is one pass of batched LDS FFTs over contiguous rows (mixed
+// radix: any n whose prime factors are <= 64), so columns are handled by
+// LDS-tiled transposes around a row pass:
+//   forward: T(small) -> U rows -> T -> U rows (scaled) -> big
+//   adjoint: gbig -> V rows -> T -> V rows (scaled) -> T -> gsmall
+// Twiddles (exp(-2 pi i k/m), the n-point transforms read every ds-th)
+// live in LDS when the line buffers leave room, else in a global table in
+// the workspace (filled at the start of every call).
 #include "common.h"
 
 namespace pty {
@@ -40,72 +42,38 @@ __global__ void k_twiddle_table(float2* tw, int n) {
   }
 }
 
-// ---------------------------------------------------------------- row kernels
-//   MODE 0: out = FFT_n(in)                         (U1)
-//   MODE 1: out = IFFT_m(embed(in)) * scale         (U3; in rows of n, out rows of m)
-//   MODE 2: out = crop(FFT_m(in))                   (A1; in rows of m, out rows of n)
-//   MODE 3: out = IFFT_n(in) * scale                (A3)
-template <int MODE>
-__global__ __launch_bounds__(256) void k_rows(const float2* __restrict__ in, float2* __restrict__ out, FftPlan Pn,
-                                              FftPlan Pm, int rows, int lines, float scale,
-                                              const float2* __restrict__ gtw) {
-  extern __shared__ float2 smem[];
-  const int n = Pn.n, m = Pm.n;
-  const FftPlan& P = (MODE == 0 || MODE == 3) ? Pn : Pm;
-  const int L = P.n;  // transform length
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const float2* tw;
-  float2* a;
-  if (gtw) {
-    tw = gtw;
-    a = smem;
-  } else {
-    fill_twiddles(smem, L, tid, nt);
-    tw = smem;
-    a = smem + L;
-  }
-  float2* b = a + lines * L;
-  const int r0 = blockIdx.x * lines;
-  const int nl = min(lines, rows - r0);
-  for (int t = tid; t < lines * L; t += nt) {
-    const int li = t / L, c = t - li * L;
-    float2 v = make_float2(0.f, 0.f);
-    if (li < nl) {
-      const int r = r0 + li;
-      if (MODE == 1) {
-        const int k = crop_freq(c, n, m);   // small frequency placed at big frequency c
-        if (k >= 0) v = in[(int64_t)r * n + k];
-      } else {
-        v = in[(int64_t)r * L + c];
-      }
-    }
-    a[t] = v;
+// out [C, R] = in [R, C]^T (c64), 32 x 32 tiles through LDS; grid
+// (ceil(C/32), ceil(R/32)), 256 threads.
+__global__ __launch_bounds__(256) void k_transpose(const float2* __restrict__ in, float2* __restrict__ out, int R,
+                                                   int C) {
+  __shared__ float2 tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    if (r < R && c < C) tile[k][tx] = in[(int64_t)r * C + c];
   }
   __syncthreads();
-  float2* res = (MODE == 0 || MODE == 2) ? fft_any<-1>(a, b, P, lines, L, tw, 1, tid, nt)
-                                         : fft_any<+1>(a, b, P, lines, L, tw, 1, tid, nt);
-  const int W = (MODE == 2) ? n : L;  // output row length
-  for (int t = tid; t < lines * W; t += nt) {
-    const int li = t / W, c = t - li * W;
-    if (li >= nl) continue;
-    const int r = r0 + li;
-    float2 v = (MODE == 2) ? res[li * L + embed_freq(c, n, m)] : res[li * L + c];
-    if (MODE == 1 || MODE == 3) v = cscale(v, scale);
-    out[(int64_t)r * W + c] = v;
+#pragma unroll
+  for (int k = ty; k < 32; k += 8) {
+    const int orow = c0 + k, ocol = r0 + tx;
+    if (orow < C && ocol < R) out[(int64_t)orow * R + ocol] = tile[tx][k];
   }
 }
 
-// ------------------------------------------------------------- column kernels
-//   MODE 0 (U2): in [n, n] -> out [m, n]: IFFT_m(embed(FFT_n(col)))
-//   MODE 1 (A2): in [m, n] -> out [n, n]: IFFT_n(crop(FFT_m(col)))
+// One upsample stage over `rows` contiguous lines, `lines` per workgroup:
+//   MODE 0 (U): in rows of n -> out rows of m: IFFT_m(embed(FFT_n(row))) * scale
+//   MODE 1 (V): in rows of m -> out rows of n: IFFT_n(crop(FFT_m(row))) * scale
+// LDS: [twiddles m] + two ping-pong buffers of lines x m.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ in, float2* __restrict__ out, FftPlan Pn,
-                                              FftPlan Pm, int C, const float2* __restrict__ gtw) {
+__global__ __launch_bounds__(256) void k_up_rows(const float2* __restrict__ in, float2* __restrict__ out, FftPlan Pn,
+                                                 FftPlan Pm, int rows, int lines, float scale,
+                                                 const float2* __restrict__ gtw) {
   extern __shared__ float2 smem[];
-  const int n = Pn.n, m = Pm.n;
-  const int S = m + 1;  // padded line stride in LDS
+  const int n = Pn.n, m = Pm.n, ds = m / n;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const float2* tw;     // exp(-2 pi i k/m), k < m; the n-point FFT uses stride m/n
+  const float2* tw;
   float2* a;
   if (gtw) {
     tw = gtw;
@@ -115,41 +83,40 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ in, flo
     tw = smem;
     a = smem + m;
   }
-  float2* b = a + C * S;
-  const int c0 = blockIdx.x * C;
-  const int rows_in = (MODE == 0) ? n : m;
-  for (int t = tid; t < rows_in * C; t += nt) {
-    const int r = t / C, c = t - r * C;
-    a[c * S + r] = in[(int64_t)r * n + c0 + c];
+  float2* b = a + lines * m;
+  const int r0 = blockIdx.x * lines;
+  const int nl = min(lines, rows - r0);
+  const int Lin = (MODE == 0) ? n : m;
+  for (int t = tid; t < lines * Lin; t += nt) {
+    const int li = t / Lin, c = t - li * Lin;
+    a[li * m + c] = (li < nl) ? in[(int64_t)(r0 + li) * Lin + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
+  float2* res;
   if (MODE == 0) {
-    float2* f = fft_any<-1>(a, b, Pn, C, S, tw, m / n, tid, nt);
+    float2* f = fft_any<-1>(a, b, Pn, lines, m, tw, ds, tid, nt);
     float2* g = (f == a) ? b : a;
-    for (int t = tid; t < C * m; t += nt) {
-      const int c = t / m, q = t - c * m;
+    for (int t = tid; t < lines * m; t += nt) {
+      const int li = t / m, q = t - li * m;
       const int k = crop_freq(q, n, m);
-      g[c * S + q] = (k >= 0) ? f[c * S + k] : make_float2(0.f, 0.f);
+      g[li * m + q] = (k >= 0) ? f[li * m + k] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    float2* h = fft_any<+1>(g, f, Pm, C, S, tw, 1, tid, nt);
-    for (int t = tid; t < m * C; t += nt) {
-      const int r = t / C, c = t - r * C;
-      out[(int64_t)r * n + c0 + c] = h[c * S + r];
-    }
+    res = fft_any<+1>(g, f, Pm, lines, m, tw, 1, tid, nt);
   } else {
-    float2* f = fft_any<-1>(a, b, Pm, C, S, tw, 1, tid, nt);
+    float2* f = fft_any<-1>(a, b, Pm, lines, m, tw, 1, tid, nt);
     float2* g = (f == a) ? b : a;
-    for (int t = tid; t < C * n; t += nt) {
-      const int c = t / n, k = t - c * n;
-      g[c * S + k] = f[c * S + embed_freq(k, n, m)];
+    for (int t = tid; t < lines * n; t += nt) {
+      const int li = t / n, k = t - li * n;
+      g[li * m + k] = f[li * m + embed_freq(k, n, m)];
     }
     __syncthreads();
-    float2* h = fft_any<+1>(g, f, Pn, C, S, tw, m / n, tid, nt);
-    for (int t = tid; t < n * C; t += nt) {
-      const int r = t / C, c = t - r * C;
-      out[(int64_t)r * n + c0 + c] = h[c * S + r];
-    }
+    res = fft_any<+1>(g, f, Pn, lines, m, tw, ds, tid, nt);
+  }
+  const int Lout = (MODE == 0) ? m : n;
+  for (int t = tid; t < lines * Lout; t += nt) {
+    const int li = t / Lout, c = t - li * Lout;
+    if (li < nl) out[(int64_t)(r0 + li) * Lout + c] = cscale(res[li * m + c], scale);
   }
 }
 
@@ -170,23 +137,18 @@ int make_fft_plan(int n, FftPlan& P) {
 }
 
 struct UpLaunch {
-  int lines_n, lines_m;   // rows per workgroup for n- and m-length row kernels
-  bool lds_tw_n, lds_tw_m, lds_tw_c;
-  int C;                  // columns per workgroup
+  int lines;     // lines per workgroup of the row stages
+  bool lds_tw;   // twiddle table in LDS (else the global table)
 };
 
-static int rows_per_wg(int L) { return L >= 1024 ? 1 : 1024 / L; }
-
 static UpLaunch up_launch(int n, int m) {
+  (void)n;
   UpLaunch u;
-  u.lines_n = rows_per_wg(n);
-  u.lines_m = rows_per_wg(m);
-  u.lds_tw_n = sizeof(float2) * (size_t)(n + 2 * u.lines_n * n) <= kLdsBudget;
-  u.lds_tw_m = sizeof(float2) * (size_t)(m + 2 * u.lines_m * m) <= kLdsBudget;
-  int C = 16;
-  while (C > 1 && (sizeof(float2) * (size_t)(m + 2 * C * (m + 1)) > kLdsBudget || n % C)) C >>= 1;
-  u.C = C;
-  u.lds_tw_c = sizeof(float2) * (size_t)(m + 2 * C * (m + 1)) <= kLdsBudget;
+  int lines = 1;
+  while (lines < 16 && lines * 2 * m <= 2048 && sizeof(float2) * (size_t)(m + 2 * (lines * 2) * m) <= kLdsBudget)
+    lines *= 2;
+  u.lines = lines;
+  u.lds_tw = sizeof(float2) * (size_t)(m + 2 * lines * m) <= kLdsBudget;
   return u;
 }
 
@@ -197,34 +159,27 @@ static int check_sizes(int32_t n, int32_t ds, FftPlan& Pn, FftPlan& Pm) {
   PTY_REQUIRE(make_fft_plan(n, Pn) == PTY_OK && make_fft_plan(n * ds, Pm) == PTY_OK, PTY_EUNSUP,
               "upsample: sizes %d / %d need prime factors <= %d", n, n * ds, kMaxRadix);
   // one line of m (ping-pong) must fit the LDS budget
-  PTY_REQUIRE(sizeof(float2) * 2 * (size_t)(n * ds + 1) <= kLdsBudget, PTY_EUNSUP, "upsample: canvas too large");
+  PTY_REQUIRE(sizeof(float2) * 2 * (size_t)(n * ds) <= kLdsBudget, PTY_EUNSUP, "upsample: canvas too large");
   return PTY_OK;
 }
 
 template <int MODE>
-static void launch_rows(const float2* in, float2* out, const FftPlan& Pn, const FftPlan& Pm, int rows, float scale,
-                        const UpLaunch& u, const float2* gtw_n, const float2* gtw_m, hipStream_t s) {
-  const bool on_n = (MODE == 0 || MODE == 3);
-  const int L = on_n ? Pn.n : Pm.n;
-  const int lines = on_n ? u.lines_n : u.lines_m;
-  const bool lds_tw = on_n ? u.lds_tw_n : u.lds_tw_m;
-  const size_t shm = sizeof(float2) * (size_t)((lds_tw ? L : 0) + 2 * lines * L);
-  const int blocks = (rows + lines - 1) / lines;
-  hipLaunchKernelGGL(k_rows<MODE>, dim3(blocks), dim3(256), shm, s, in, out, Pn, Pm, rows, lines, scale,
-                     lds_tw ? nullptr : (on_n ? gtw_n : gtw_m));
-}
-
-template <int MODE>
-static void launch_cols(const float2* in, float2* out, const FftPlan& Pn, const FftPlan& Pm, const UpLaunch& u,
-                        const float2* gtw_m, hipStream_t s) {
+static void launch_up_rows(const float2* in, float2* out, const FftPlan& Pn, const FftPlan& Pm, int rows,
+                           float scale, const UpLaunch& u, const float2* gtw, hipStream_t s) {
   const int m = Pm.n;
-  const size_t shm = sizeof(float2) * (size_t)((u.lds_tw_c ? m : 0) + 2 * u.C * (m + 1));
-  hipLaunchKernelGGL(k_cols<MODE>, dim3(Pn.n / u.C), dim3(256), shm, s, in, out, Pn, Pm, u.C,
-                     u.lds_tw_c ? nullptr : gtw_m);
+  const size_t shm = sizeof(float2) * (size_t)((u.lds_tw ? m : 0) + 2 * u.lines * m);
+  const int blocks = (rows + u.lines - 1) / u.lines;
+  hipLaunchKernelGGL(k_up_rows<MODE>, dim3(blocks), dim3(256), shm, s, in, out, Pn, Pm, rows, u.lines, scale,
+                     u.lds_tw ? nullptr : gtw);
 }
 
+static void launch_transpose(const float2* in, float2* out, int R, int C, hipStream_t s) {
+  hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32), dim3(256), 0, s, in, out, R, C);
+}
+
+// workspace: t_nn [n, n], y [m, n], yt [n, m], twiddles [m]
 struct UpWork {
-  size_t x1, z, tw_n, tw_m, total;
+  size_t t_nn, y, yt, tw, total;
 };
 
 static UpWork up_work(int n, int ds) {
@@ -232,17 +187,12 @@ static UpWork up_work(int n, int ds) {
   UpWork w;
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
-  w.x1 = take(sizeof(float2) * (size_t)n * n);
-  w.z = take(sizeof(float2) * m * n);
-  w.tw_n = take(sizeof(float2) * (size_t)n);
-  w.tw_m = take(sizeof(float2) * m);
+  w.t_nn = take(sizeof(float2) * (size_t)n * n);
+  w.y = take(sizeof(float2) * m * n);
+  w.yt = take(sizeof(float2) * m * n);
+  w.tw = take(sizeof(float2) * m);
   w.total = off;
   return w;
-}
-
-static void fill_global_twiddles(float2* tw_n, int n, float2* tw_m, int m, const UpLaunch& u, hipStream_t s) {
-  if (!u.lds_tw_n) hipLaunchKernelGGL(k_twiddle_table, dim3((n + 255) / 256), dim3(256), 0, s, tw_n, n);
-  if (!u.lds_tw_m || !u.lds_tw_c) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, s, tw_m, m);
 }
 
 }  // namespace pty
@@ -262,14 +212,15 @@ extern "C" int pty_upsample_fwd(const float* small, int32_t n, int32_t ds, float
   const int m = n * ds;
   const UpLaunch u = up_launch(n, m);
   char* w = (char*)work;
-  float2* X1 = (float2*)(w + W.x1);
-  float2* Z = (float2*)(w + W.z);
-  float2* twn = (float2*)(w + W.tw_n);
-  float2* twm = (float2*)(w + W.tw_m);
-  fill_global_twiddles(twn, n, twm, m, u, stream);
-  launch_rows<0>((const float2*)small, X1, Pn, Pm, n, 1.f, u, twn, twm, stream);
-  launch_cols<0>(X1, Z, Pn, Pm, u, twm, stream);
-  launch_rows<1>(Z, (float2*)big, Pn, Pm, m, 1.f / ((float)n * (float)n), u, twn, twm, stream);
+  float2* sT = (float2*)(w + W.t_nn);   // small^T        [n, n]
+  float2* YT = (float2*)(w + W.yt);     // U along columns, transposed [n, m]
+  float2* Y = (float2*)(w + W.y);       // [m, n]
+  float2* tw = (float2*)(w + W.tw);
+  if (!u.lds_tw) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, stream, tw, m);
+  launch_transpose((const float2*)small, sT, n, n, stream);
+  launch_up_rows<0>(sT, YT, Pn, Pm, n, 1.f, u, tw, stream);
+  launch_transpose(YT, Y, n, m, stream);
+  launch_up_rows<0>(Y, (float2*)big, Pn, Pm, m, 1.f / ((float)n * (float)n), u, tw, stream);
   return check_launch("upsample_fwd");
 }
 
@@ -284,13 +235,14 @@ extern "C" int pty_upsample_adj(const float* gbig, int32_t n, int32_t ds, float*
   const int m = n * ds;
   const UpLaunch u = up_launch(n, m);
   char* w = (char*)work;
-  float2* W1 = (float2*)(w + W.z);     // [m, n]
-  float2* W2 = (float2*)(w + W.x1);    // [n, n]
-  float2* twn = (float2*)(w + W.tw_n);
-  float2* twm = (float2*)(w + W.tw_m);
-  fill_global_twiddles(twn, n, twm, m, u, stream);
-  launch_rows<2>((const float2*)gbig, W1, Pn, Pm, m, 1.f, u, twn, twm, stream);
-  launch_cols<1>(W1, W2, Pn, Pm, u, twm, stream);
-  launch_rows<3>(W2, (float2*)gsmall, Pn, Pm, n, 1.f / ((float)n * (float)n), u, twn, twm, stream);
+  float2* Wr = (float2*)(w + W.y);      // V along rows            [m, n]
+  float2* WT = (float2*)(w + W.yt);     // its transpose           [n, m]
+  float2* gT = (float2*)(w + W.t_nn);   // gsmall^T                [n, n]
+  float2* tw = (float2*)(w + W.tw);
+  if (!u.lds_tw) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, stream, tw, m);
+  launch_up_rows<1>((const float2*)gbig, Wr, Pn, Pm, m, 1.f, u, tw, stream);
+  launch_transpose(Wr, WT, m, n, stream);
+  launch_up_rows<1>(WT, gT, Pn, Pm, n, 1.f / ((float)n * (float)n), u, tw, stream);
+  launch_transpose(gT, (float2*)gsmall, n, n, stream);
   return check_launch("upsample_adj");
 }
