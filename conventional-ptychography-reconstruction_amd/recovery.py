@@ -53,6 +53,15 @@ def normalize_config(raw: dict) -> dict:
         raise ValueError(f"model_type must be one of {VALID_MODEL_TYPES}, got {model_type!r}")
     if model_type != "inr":
         raise ValueError("model_type 'gaussian_field' is not part of the gfx950 build")
+    if cfg.get("sparsity_weight") not in (None, 0.0):
+        # Recovery.py:309-313 adds weight * haar_wavelet_sparsity_loss(object);
+        # every reference config sets the weight to 0 and the term is not built
+        raise ValueError("sparsity_weight > 0 (Haar wavelet sparsity) is not part of the gfx950 build")
+    for key in ("mlp_precision",):
+        if cfg.get(key, "fp32") not in ("fp32", "bf16"):
+            raise ValueError(f"{key} must be 'fp32' or 'bf16'")
+    if cfg.get("table_precision", "fp32") not in ("fp32", "fp16"):
+        raise ValueError("table_precision must be 'fp32' or 'fp16'")
     return cfg
 
 
