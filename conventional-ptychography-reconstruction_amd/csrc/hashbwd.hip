@@ -16,13 +16,16 @@
 // it:
 //   dense level : local = cx + cy*res (minus size once past the end), so the
 //                 preimages are cx + cy*res in {e, e+size}: at most 4 corners;
+//                 the levels first contract along axis0 (A[i][cx], pass 1)
+//                 so that the coarse levels (many points per cell) stay cheap;
 //   hashed level: local = (cx ^ h(cy)) & (T-1), h(cy) = cy*2654435761 mod 2^32,
 //                 so for each cy the corner is cx = (e ^ h(cy)) & (T-1) (+ k T).
 //                 Only rows cy touched by a grid point can contribute; the
 //                 plan buckets them by the bits of h(cy) above log2(res+1),
-//                 which must equal those of e for cx <= res.
-// Dense levels first contract along axis0 (A[i][cx], pass 1) so that the
-// coarse levels (tens of points per cell) stay cheap.
+//                 which must equal those of e for cx <= res.  Since the grid is
+//                 static, the plan build runs this enumeration once and stores
+//                 every entry's (point, corner) terms as a CSR list (exactly
+//                 4 terms per point and level); each step then gathers them.
 #include "hashgrid.h"
 
 namespace pty {
@@ -38,11 +41,14 @@ struct PlanLayout {
   int off_b[PTY_MAX_LEVELS];    // hashed: bucket starts (nb + 1 ints)
   int off_cy[PTY_MAX_LEVELS];   // hashed: bucketed rows (res + 1 ints capacity)
   int off_m[PTY_MAX_LEVELS];    // hashed: touched-column bitmask, (res + 32) / 32 words
+  int64_t off_co[PTY_MAX_LEVELS];   // hashed: CSR offsets (T + 1 ints)
+  int64_t off_cp[PTY_MAX_LEVELS];   // hashed: CSR terms (4 n ints)
   int n_bucket_ints;
   int n_cy_ints;
   int n_mask_words;
+  int64_t n_co, n_cp;
   // byte offsets
-  size_t s0, s1, fx, fy, bstart, bcy, mask, total;
+  size_t s0, s1, fx, fy, bstart, bcy, mask, csr_o, csr_p, total;
   // dense pass-1 buffer (workspace, not plan): A[l] of n1 * (res + 1) float2
   int64_t off_a[PTY_MAX_LEVELS + 1];
 };
@@ -60,7 +66,8 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
   L.n0 = pts.n_axis0;
   L.n1 = pts.n_axis1;
   int s = 0, bo = 0, co = 0, mo = 0;
-  int64_t a = 0;
+  int64_t a = 0, ro = 0, rp = 0;
+  const int64_t npts = (int64_t)L.n0 * L.n1;
   for (int l = 0; l < g.n_levels; ++l) {
     L.off_s[l] = s;
     s += (int)g.res[l] + 2;
@@ -71,6 +78,8 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
       L.off_b[l] = bo;
       L.off_cy[l] = co;
       L.off_m[l] = mo;
+      L.off_co[l] = ro;
+      L.off_cp[l] = rp;
       a += (int64_t)L.n1 * (g.res[l] + 1);
     } else {
       const int log2T = g.log2_hashmap_size;
@@ -84,8 +93,14 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
       co += (int)g.res[l] + 1;
       L.off_m[l] = mo;
       mo += ((int)g.res[l] + 32) / 32;
+      L.off_co[l] = ro;
+      ro += (int64_t)g.size[l] + 1;
+      L.off_cp[l] = rp;
+      rp += 4 * npts;
     }
   }
+  L.n_co = ro;
+  L.n_cp = rp;
   L.off_a[g.n_levels] = a;
   L.n_starts = s;
   L.n_bucket_ints = bo;
@@ -99,6 +114,8 @@ static PlanLayout plan_layout(const PtyGrid& g, const PtyPoints& pts) {
   L.bstart = off; off += a256(sizeof(int) * (size_t)(bo > 0 ? bo : 1));
   L.bcy = off; off += a256(sizeof(int) * (size_t)(co > 0 ? co : 1));
   L.mask = off; off += a256(sizeof(uint32_t) * (size_t)(mo > 0 ? mo : 1));
+  L.csr_o = off; off += a256(sizeof(int) * (size_t)(ro > 0 ? ro : 1));
+  L.csr_p = off; off += a256(sizeof(uint32_t) * (size_t)(rp > 0 ? rp : 1));
   L.total = off;
   return L;
 }
@@ -111,6 +128,8 @@ struct PlanPtrs {
   const int* bstart;
   const int* bcy;
   const uint32_t* mask;
+  const int* csr_o;
+  const uint32_t* csr_p;
 };
 
 static PlanPtrs plan_ptrs(const PlanLayout& L, const void* plan) {
@@ -123,6 +142,8 @@ static PlanPtrs plan_ptrs(const PlanLayout& L, const void* plan) {
   P.bstart = (const int*)(p + L.bstart);
   P.bcy = (const int*)(p + L.bcy);
   P.mask = (const uint32_t*)(p + L.mask);
+  P.csr_o = (const int*)(p + L.csr_o);
+  P.csr_p = (const uint32_t*)(p + L.csr_p);
   return P;
 }
 
@@ -310,127 +331,155 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
   }
 }
 
-// Hashed levels: a workgroup owns 256 consecutive entries of one level.  When
-// 2^bbits >= 256 they share one bucket of candidate rows, which is staged in
-// LDS together with the rows' point ranges and the level's touched-column
-// bitmask; a candidate (cx, cy) is summed only if column cx is touched.
-constexpr int kRowsLds = 512;
-constexpr int kMaskLds = 2048;
-constexpr int kCand = 16;
-
-__global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
-                                                      const float2* __restrict__ gfeat,
-                                                      float2* __restrict__ dtable) {
-  __shared__ uint32_t s_h[kRowsLds];
-  __shared__ int4 s_ir[kRowsLds];      // (i0, i1) of cell cy-1, (i0, i1) of cell cy
-  __shared__ uint32_t s_mask[kMaskLds];
-  __shared__ uint32_t s_cand[kCand][256];
-  // blockIdx.x -> (level, chunk) over the hashed levels in order
-  int l = 0;
-  uint32_t chunk = blockIdx.x;
-  for (; l < g.n_levels; ++l) {
+// ------------------------------------------------------------------ hashed levels
+// blockIdx.x -> (level, chunk of 256 entries) over the hashed levels in order;
+// false when past the last one.
+__device__ __forceinline__ bool hashed_chunk(const PtyGrid& g, uint32_t bid, int& l, uint32_t& chunk) {
+  chunk = bid;
+  for (l = 0; l < g.n_levels; ++l) {
     if ((g.dense_mask >> l) & 1u) continue;
     const uint32_t nch = (g.size[l] + 255) / 256;
-    if (chunk < nch) break;
+    if (chunk < nch) return true;
     chunk -= nch;
   }
-  if (l >= g.n_levels) return;
+  return false;
+}
+
+// Term of one grid point at one corner: (i, j) point, (dx, dy) corner offset.
+__device__ __forceinline__ uint32_t pack_term(int i, int j, int dx, int dy) {
+  return ((uint32_t)i << 17) | ((uint32_t)j << 2) | ((uint32_t)dy << 1) | (uint32_t)dx;
+}
+
+// Every (point, corner) term of hashed-level entry e, in a fixed order:
+// bucket rows ascending, candidate columns ascending, dx = 0 then 1, points j
+// ascending, then the points i of cell cy-1 (dy = 1) and of cell cy (dy = 0).
+template <class Visit>
+__device__ void hashed_terms(const PtyGrid& g, const PlanLayout& L, const PlanPtrs& P, int l, uint32_t e,
+                             Visit&& visit) {
   const int res = (int)g.res[l];
   const uint32_t T = 1u << g.log2_hashmap_size;
-  const int b = L.bbits[l];
-  const uint32_t e = chunk * 256 + threadIdx.x;
-  const bool valid = e < g.size[l];
   const int* s0 = P.s0 + L.off_s[l];
   const int* s1 = P.s1 + L.off_s[l];
-  const float* fxl = P.fx + (int64_t)l * L.n0;
-  const float* fyl = P.fy + (int64_t)l * L.n1;
-  const float2* gl = gfeat + (int64_t)l * pts.n;
   const int* bs = P.bstart + L.off_b[l];
   const int* rows = P.bcy + L.off_cy[l];
   const uint32_t* mk = P.mask + L.off_m[l];
-  const int words = (res + 32) / 32;
-  const bool mask_lds = words <= kMaskLds;
-  if (mask_lds)
-    for (int w = threadIdx.x; w < words; w += blockDim.x) s_mask[w] = mk[w];
-  // the mask is read by every thread below (both paths)
-  __syncthreads();
-  // all 256 entries of the chunk share e >> b; a row gives at most per_row columns
-  const int per_row = (int)(((uint32_t)res + T) / T);
-  const bool shared_key = b >= 8 && per_row <= kCand && res < 65536;
-  float ax = 0.f, ay = 0.f;
-  auto corner = [&](int cx, int i0a, int i1a, int i0b, int i1b) {
-#pragma unroll
-    for (int dx = 0; dx < 2; ++dx) {
-      const int cxx = cx - dx;
-      if (cxx < 0 || cxx > res) continue;
-      const int j0 = s0[cxx], j1 = s0[cxx + 1];
-      for (int j = j0; j < j1; ++j) {
-        const float wx = dx ? fxl[j] : 1.f - fxl[j];
-        for (int i = i0a; i < i1a; ++i) {          // cell cy-1: dy = 1
-          const float w = wx * fyl[i];
-          const float2 v = gl[(int64_t)i * L.n0 + j];
-          ax += w * v.x;
-          ay += w * v.y;
-        }
-        for (int i = i0b; i < i1b; ++i) {          // cell cy: dy = 0
-          const float w = wx * (1.f - fyl[i]);
-          const float2 v = gl[(int64_t)i * L.n0 + j];
-          ax += w * v.x;
-          ay += w * v.y;
-        }
-      }
-    }
-  };
-  auto touched = [&](uint32_t cx) -> bool {
-    const uint32_t w = mask_lds ? s_mask[cx >> 5] : mk[cx >> 5];
-    return (w >> (cx & 31)) & 1u;
-  };
-  if (shared_key) {
-    const uint32_t key = (chunk * 256) >> b;
-    const int r0 = bs[key], r1 = bs[key + 1];
-    for (int rb = r0; rb < r1; rb += kRowsLds) {
-      const int nr = min(kRowsLds, r1 - rb);
-      __syncthreads();
-      for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-        const int cy = rows[rb + r];
-        s_h[r] = (uint32_t)cy * kPrime;
-        const int a0 = cy > 0 ? s1[cy - 1] : 0, a1 = cy > 0 ? s1[cy] : 0;
-        s_ir[r] = make_int4(a0, a1, s1[cy], s1[cy + 1]);
-      }
-      __syncthreads();
-      // Compact this thread's touched candidates (row, cx) into its own LDS
-      // list, then sum them with all lanes busy: at the fine levels only a
-      // few of the rows give a touched column for a given entry.
-      int r = 0;
-      while (true) {
-        int cnt = 0;
-        if (valid) {
-          for (; r < nr && cnt + per_row <= kCand; ++r) {
-            for (uint32_t cx = (e ^ s_h[r]) & (T - 1u); cx <= (uint32_t)res; cx += T)
-              if (touched(cx)) s_cand[cnt++][threadIdx.x] = (r << 16) | cx;
-          }
-        }
-        for (int k = 0; k < cnt; ++k) {
-          const uint32_t c = s_cand[k][threadIdx.x];
-          const int4 ir = s_ir[c >> 16];
-          corner((int)(c & 0xFFFFu), ir.x, ir.y, ir.z, ir.w);
-        }
-        // the loop is per thread; threads that are done leave it
-        if (!valid || r >= nr) break;
-      }
-    }
-  } else if (valid) {
-    const uint32_t key = e >> b;
-    const int r1 = bs[key + 1];
-    for (int r = bs[key]; r < r1; ++r) {
-      const int cy = rows[r];
-      for (uint32_t cx = (e ^ ((uint32_t)cy * kPrime)) & (T - 1u); cx <= (uint32_t)res; cx += T) {
-        if (!touched(cx)) continue;
-        corner((int)cx, cy > 0 ? s1[cy - 1] : 0, cy > 0 ? s1[cy] : 0, s1[cy], s1[cy + 1]);
+  const uint32_t key = e >> L.bbits[l];
+  const int r1 = bs[key + 1];
+  for (int r = bs[key]; r < r1; ++r) {
+    const int cy = rows[r];
+    const int i0a = cy > 0 ? s1[cy - 1] : 0, i1a = cy > 0 ? s1[cy] : 0;
+    const int i0b = s1[cy], i1b = s1[cy + 1];
+    for (uint32_t cx = (e ^ ((uint32_t)cy * kPrime)) & (T - 1u); cx <= (uint32_t)res; cx += T) {
+      if (!((mk[cx >> 5] >> (cx & 31)) & 1u)) continue;
+      for (int dx = 0; dx < 2; ++dx) {
+        const int cxx = (int)cx - dx;
+        if (cxx < 0 || cxx > res) continue;
+        const int j1 = s0[cxx + 1];
+        for (int j = s0[cxx]; j < j1; ++j) visit(j, dx, i0a, i1a, i0b, i1b);
       }
     }
   }
-  if (valid && (ax != 0.f || ay != 0.f)) {
+}
+
+// plan build, pass 1: csr_o[l][e + 1] = number of terms of entry e
+__global__ __launch_bounds__(256) void k_csr_count(PtyGrid g, PlanLayout L, PlanPtrs P, int* __restrict__ csr_o) {
+  int l;
+  uint32_t chunk;
+  if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;
+  const uint32_t e = chunk * 256 + threadIdx.x;
+  int* off = csr_o + L.off_co[l];
+  if (e == 0) off[0] = 0;
+  if (e >= g.size[l]) return;
+  int cnt = 0;
+  hashed_terms(g, L, P, l, e, [&](int, int, int i0a, int i1a, int i0b, int i1b) {
+    cnt += (i1a - i0a) + (i1b - i0b);
+  });
+  off[e + 1] = cnt;
+}
+
+// plan build, pass 2: inclusive scan of csr_o[l][1..T] (one workgroup per
+// hashed level, 1024 threads, tiles of 1024 with a carried total)
+__global__ __launch_bounds__(1024) void k_csr_scan(PtyGrid g, PlanLayout L, int* __restrict__ csr_o) {
+  __shared__ int s_tmp[1024];
+  __shared__ int s_carry;
+  int l = -1;
+  for (int k = 0, h = 0; k < g.n_levels; ++k) {
+    if ((g.dense_mask >> k) & 1u) continue;
+    if (h++ == (int)blockIdx.x) { l = k; break; }
+  }
+  if (l < 0) return;
+  int* off = csr_o + L.off_co[l] + 1;
+  const int n = (int)g.size[l];
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + (int)threadIdx.x;
+    int v = i < n ? off[i] : 0;
+    s_tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      const int add = threadIdx.x >= (unsigned)d ? s_tmp[threadIdx.x - d] : 0;
+      __syncthreads();
+      s_tmp[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int carry = s_carry;
+    if (i < n) off[i] = s_tmp[threadIdx.x] + carry;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = carry + s_tmp[1023];
+    __syncthreads();
+  }
+}
+
+// plan build, pass 3: the terms themselves, in the order of hashed_terms
+__global__ __launch_bounds__(256) void k_csr_fill(PtyGrid g, PlanLayout L, PlanPtrs P, uint32_t* __restrict__ csr_p) {
+  int l;
+  uint32_t chunk;
+  if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;
+  const uint32_t e = chunk * 256 + threadIdx.x;
+  if (e >= g.size[l]) return;
+  // every point has exactly 4 terms per level; the guard keeps a miscount
+  // inside this level's region
+  const int64_t cap = 4 * (int64_t)L.n0 * L.n1;
+  uint32_t* base = csr_p + L.off_cp[l];
+  int64_t pos = P.csr_o[L.off_co[l] + e];
+  hashed_terms(g, L, P, l, e, [&](int j, int dx, int i0a, int i1a, int i0b, int i1b) {
+    for (int i = i0a; i < i1a; ++i, ++pos)
+      if (pos < cap) base[pos] = pack_term(i, j, dx, 1);
+    for (int i = i0b; i < i1b; ++i, ++pos)
+      if (pos < cap) base[pos] = pack_term(i, j, dx, 0);
+  });
+}
+
+// Hashed-level table gradient: one thread per entry sums its CSR terms
+// w(fx_j, dx) * w(fy_i, dy) * g[l][i n0 + j] in plan order.
+__global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
+                                                      const float2* __restrict__ gfeat,
+                                                      float2* __restrict__ dtable) {
+  int l;
+  uint32_t chunk;
+  if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;
+  const uint32_t e = chunk * 256 + threadIdx.x;
+  if (e >= g.size[l]) return;
+  const int* off = P.csr_o + L.off_co[l];
+  const uint32_t* terms = P.csr_p + L.off_cp[l];
+  const float* fxl = P.fx + (int64_t)l * L.n0;
+  const float* fyl = P.fy + (int64_t)l * L.n1;
+  const float2* gl = gfeat + (int64_t)l * pts.n;
+  const int cap = 4 * L.n0 * L.n1;
+  const int k1 = min(off[e + 1], cap);
+  float ax = 0.f, ay = 0.f;
+  for (int k = off[e]; k < k1; ++k) {
+    const uint32_t c = terms[k];
+    const int i = (int)(c >> 17), j = (int)((c >> 2) & 0x7FFFu);
+    const float fx = fxl[j], fy = fyl[i];
+    const float wx = (c & 1u) ? fx : 1.f - fx;
+    const float w = wx * ((c & 2u) ? fy : 1.f - fy);
+    const float2 v = gl[(int64_t)i * L.n0 + j];
+    ax += w * v.x;
+    ay += w * v.y;
+  }
+  if (ax != 0.f || ay != 0.f) {
     const uint32_t t = g.offset[l] + e;
     float2 d = dtable[t];
     d.x += ax;
@@ -458,6 +507,9 @@ size_t points_plan_bytes(const PtyGrid& g, const PtyPoints& pts) {
 }
 
 int build_points_plan(const PtyGrid& g, const PtyPoints& pts, void* plan, hipStream_t stream) {
+  // CSR terms pack the point as 15-bit (i, j)
+  PTY_REQUIRE(pts.n_axis0 <= 32768 && pts.n_axis1 <= 32768, PTY_EUNSUP,
+              "regular grid %d x %d exceeds 32768 per axis", pts.n_axis0, pts.n_axis1);
   const PlanLayout L = plan_layout(g, pts);
   char* p = (char*)plan;
   const int nax = L.n0 > L.n1 ? L.n0 : L.n1;
@@ -466,6 +518,19 @@ int build_points_plan(const PtyGrid& g, const PtyPoints& pts, void* plan, hipStr
                      (int*)(p + L.s0), (int*)(p + L.s1), (float*)(p + L.fx), (float*)(p + L.fy));
   hipLaunchKernelGGL(k_plan_buckets, dim3(g.n_levels), dim3(64), 0, stream, g, L, (const int*)(p + L.s0),
                      (const int*)(p + L.s1), (int*)(p + L.bstart), (int*)(p + L.bcy), (uint32_t*)(p + L.mask));
+  uint32_t hashed_chunks = 0;
+  int n_hashed = 0;
+  for (int l = 0; l < g.n_levels; ++l)
+    if (!((g.dense_mask >> l) & 1u)) {
+      hashed_chunks += (g.size[l] + 255) / 256;
+      ++n_hashed;
+    }
+  if (n_hashed > 0) {
+    const PlanPtrs P = plan_ptrs(L, plan);
+    hipLaunchKernelGGL(k_csr_count, dim3(hashed_chunks), dim3(256), 0, stream, g, L, P, (int*)(p + L.csr_o));
+    hipLaunchKernelGGL(k_csr_scan, dim3(n_hashed), dim3(1024), 0, stream, g, L, (int*)(p + L.csr_o));
+    hipLaunchKernelGGL(k_csr_fill, dim3(hashed_chunks), dim3(256), 0, stream, g, L, P, (uint32_t*)(p + L.csr_p));
+  }
   return check_launch("points_plan_build");
 }
 
