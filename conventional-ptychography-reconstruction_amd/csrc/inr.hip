@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
     if (valid) point_coords(pts, p, x0, x1);
     // ---------------- recompute forward
     float feat[16];
-    encode_lane(g, x0, x1, h, table, feat);
+    encode_lane_batched(g, x0, x1, h, table, feat);
 #pragma unroll
     for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * WS + j] = feat[v];
     const v16f u0 = layer0<H>(s, feat, j, h);

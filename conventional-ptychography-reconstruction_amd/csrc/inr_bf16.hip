@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_bf16(PtyGrid g, PtyMlp m, 
     uint4 pf[2];
     {
       float feat[16];
-      encode_lane(g, x0, x1, h, table, feat);
+      encode_lane_batched(g, x0, x1, h, table, feat);
 #pragma unroll
       for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * BS + j] = f2bf(feat[v]);
       pf[0] = pack8(feat);

@@ -81,7 +81,9 @@ __device__ void load_weights(MlpLds<H>& s, const PtyMlp& m) {
   if (tid == 0) { s.bo[0] = m.bo[0]; s.bo[1] = m.bo[1]; }
 }
 
-// Features of the lane's 8 levels: feat[s] = feature kappa0(s, h).
+// Features of the lane's 8 levels: feat[s] = feature kappa0(s, h), one level
+// at a time (few registers; the forward kernels hide the gather latency with
+// other waves).
 __device__ __forceinline__ void encode_lane(const PtyGrid& g, float x0, float x1, int h,
                                             const float2* __restrict__ table, float feat[16]) {
 #pragma unroll
@@ -91,6 +93,61 @@ __device__ __forceinline__ void encode_lane(const PtyGrid& g, float x0, float x1
     if (l < g.n_levels) f = level_feature(g, l, x0, x1, table);
     feat[2 * m] = f.x;
     feat[2 * m + 1] = f.y;
+  }
+}
+
+// The same features with all 32 corner indices formed first and the 32
+// gathers issued back to back, so that their latencies overlap (the backward
+// kernels run one wave per SIMD and have no other wave to hide them); the
+// interpolation order per level is that of level_feature.
+__device__ __forceinline__ void encode_lane_batched(const PtyGrid& g, float x0, float x1, int h,
+                                                    const float2* __restrict__ table, float feat[16]) {
+  uint32_t idx[8][4];
+  float fx[8], fy[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int l = 2 * m + h;
+    if (l < g.n_levels) {
+      const LevelPos p = level_pos(g, l, x0, x1);
+      corner_indices(g, l, p, idx[m]);
+      fx[m] = p.fx;
+      fy[m] = p.fy;
+    } else {
+      idx[m][0] = idx[m][1] = idx[m][2] = idx[m][3] = 0u;   // in bounds; the result is discarded
+      fx[m] = fy[m] = 0.f;
+    }
+  }
+  float2 t[8][4];
+  if (g.table_precision == PTY_TABLE_F16) {
+    const __half2* t16 = reinterpret_cast<const __half2*>(table);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[m][c] = __half22float2(t16[idx[m][c]]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[m][c] = table[idx[m][c]];
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int l = 2 * m + h;
+    float2 acc = make_float2(0.f, 0.f);
+    if (l < g.n_levels) {
+      LevelPos p;
+      p.fx = fx[m];
+      p.fy = fy[m];
+      float w[4];
+      corner_weights(p, w);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        acc.x += w[c] * t[m][c].x;
+        acc.y += w[c] * t[m][c].y;
+      }
+    }
+    feat[2 * m] = acc.x;
+    feat[2 * m + 1] = acc.y;
   }
 }
 
