@@ -57,6 +57,67 @@ __device__ __forceinline__ uint32_t level_index(const PtyGrid& g, int l, uint32_
   return g.offset[l] + local;
 }
 
+// Constants of one level for lanes that may sit on different levels: la and
+// lb are compile-time level indices (both candidates are then uniform kernel
+// arguments, read through the scalar path) and the lane picks one with a
+// select instead of indexing the descriptor per lane.
+struct LevelC {
+  float scale;
+  uint32_t res, size, offset, dense;
+};
+
+// readfirstlane makes each candidate a value of its own: without it the
+// compiler turns select(load a, load b) into a per-lane load of a selected
+// address.
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ LevelC level_c(const PtyGrid& g, int la, int lb, bool pick_b) {
+  LevelC c;
+  const uint32_t sa = uniform_u32(__float_as_uint(g.scale[la])), sb = uniform_u32(__float_as_uint(g.scale[lb]));
+  const uint32_t ra = uniform_u32(g.res[la]), rb = uniform_u32(g.res[lb]);
+  const uint32_t za = uniform_u32(g.size[la]), zb = uniform_u32(g.size[lb]);
+  const uint32_t oa = uniform_u32(g.offset[la]), ob = uniform_u32(g.offset[lb]);
+  const uint32_t dm = uniform_u32(g.dense_mask);
+  c.scale = __uint_as_float(pick_b ? sb : sa);
+  c.res = pick_b ? rb : ra;
+  c.size = pick_b ? zb : za;
+  c.offset = pick_b ? ob : oa;
+  c.dense = (dm >> (pick_b ? lb : la)) & 1u;
+  return c;
+}
+
+__device__ __forceinline__ LevelPos level_pos_c(const LevelC& c, float x0, float x1) {
+#pragma clang fp contract(off)
+  float p0 = x0 * c.scale + 0.5f;
+  float p1 = x1 * c.scale + 0.5f;
+  float f0 = floorf(p0), f1 = floorf(p1);
+  LevelPos r;
+  r.gx = (uint32_t)(int)f0;
+  r.gy = (uint32_t)(int)f1;
+  r.fx = p0 - f0;
+  r.fy = p1 - f1;
+  return r;
+}
+
+// level_index with the level's constants in registers; both index forms are
+// computed and selected (the full modulo only runs for inputs outside [0, 1]).
+__device__ __forceinline__ uint32_t level_index_c(const LevelC& c, uint32_t cx, uint32_t cy) {
+  uint32_t dl = cx + cy * c.res;
+  if (dl >= c.size) {
+    if (dl < 2u * c.size) dl -= c.size;
+    else dl %= c.size;
+  }
+  const uint32_t hl = (cx ^ (cy * 2654435761u)) & (c.size - 1u);
+  return c.offset + (c.dense ? dl : hl);
+}
+
+__device__ __forceinline__ void corner_indices_c(const LevelC& c, const LevelPos& p, uint32_t idx[4]) {
+  idx[0] = level_index_c(c, p.gx, p.gy);
+  idx[1] = level_index_c(c, p.gx + 1u, p.gy);
+  idx[2] = level_index_c(c, p.gx, p.gy + 1u);
+  idx[3] = level_index_c(c, p.gx + 1u, p.gy + 1u);
+}
+
 __device__ __forceinline__ void corner_weights(const LevelPos& p, float w[4]) {
   const float ax = 1.f - p.fx, ay = 1.f - p.fy;
   w[0] = ax * ay;

@@ -86,13 +86,35 @@ __device__ void load_weights(MlpLds<H>& s, const PtyMlp& m) {
 // other waves).
 __device__ __forceinline__ void encode_lane(const PtyGrid& g, float x0, float x1, int h,
                                             const float2* __restrict__ table, float feat[16]) {
+  const bool f16 = g.table_precision == PTY_TABLE_F16;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int l = 2 * m + h;
-    float2 f = make_float2(0.f, 0.f);
-    if (l < g.n_levels) f = level_feature(g, l, x0, x1, table);
-    feat[2 * m] = f.x;
-    feat[2 * m + 1] = f.y;
+    float2 acc = make_float2(0.f, 0.f);
+    if (l < g.n_levels) {
+      const LevelC c = level_c(g, 2 * m, 2 * m + 1, h != 0);
+      const LevelPos p = level_pos_c(c, x0, x1);
+      uint32_t idx[4];
+      float w[4];
+      corner_indices_c(c, p, idx);
+      corner_weights(p, w);
+      float2 t[4];
+      if (f16) {
+        const __half2* t16 = reinterpret_cast<const __half2*>(table);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = __half22float2(t16[idx[k]]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = table[idx[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        acc.x += w[k] * t[k].x;
+        acc.y += w[k] * t[k].y;
+      }
+    }
+    feat[2 * m] = acc.x;
+    feat[2 * m + 1] = acc.y;
   }
 }
 
@@ -108,8 +130,9 @@ __device__ __forceinline__ void encode_lane_batched(const PtyGrid& g, float x0, 
   for (int m = 0; m < 8; ++m) {
     const int l = 2 * m + h;
     if (l < g.n_levels) {
-      const LevelPos p = level_pos(g, l, x0, x1);
-      corner_indices(g, l, p, idx[m]);
+      const LevelC c = level_c(g, 2 * m, 2 * m + 1, h != 0);
+      const LevelPos p = level_pos_c(c, x0, x1);
+      corner_indices_c(c, p, idx[m]);
       fx[m] = p.fx;
       fy[m] = p.fy;
     } else {
