@@ -48,8 +48,9 @@ WORKLOADS = {
 }
 MLP_FLOP_BWD_PER_POINT = 37376      # dX + dW of the complex MLP per point (recompute not counted)
 MLP_FLOP_FWD_PER_POINT = 18688
-MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}     # MI355X dense
-MLP_BWD_KERNEL = {"fp32": "k_inr_bwd<2>", "bf16": "k_inr_bwd_bf16"}
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}     # MI355X dense
+MLP_BWD_KERNEL = {"fp32": "k_inr_bwd<2>", "bf16": "k_inr_bwd_lp<pty::FmtBf16, 2>",
+                  "fp8": "k_inr_bwd_lp<pty::FmtFp8, 2>"}
 PROFILES = ROOT / "profiles"
 
 
@@ -82,7 +83,7 @@ def pmc_summary(fetch_csv, write_csv, workload, precision, command):
             raise SystemExit(f"{path}: no calibration copy dispatch found")
         factor = true_kib / (sum(cal) / len(cal))
         import re
-        key = re.compile(r"\b" + MLP_BWD_KERNEL[precision].split("<")[0] + r"[<(]")
+        key = re.compile(r"\b" + re.escape(MLP_BWD_KERNEL[precision]) + r"\(")
         # each step launches the kernel for the object grid, then the probe grid
         per = {"object": [], "probe": []}
         hits = [r for r in rows if key.search(r["Kernel_Name"])]
@@ -308,6 +309,192 @@ def plain_reconstruct(model, inst, epochs, loss_type="smooth_L1_loss", lr=1e-3, 
     return time.perf_counter() - t0
 
 
+# ----------------------------------------------------------------------------
+# precision sweep: fused variants (MLP fp32 / bf16 / fp8 x table fp32 / fp16)
+# against one FP64 run of the same math (plain PyTorch, batched far field, on
+# the GPU), every run on the same seeded batch sequence.
+# ----------------------------------------------------------------------------
+def _fp64_params(model):
+    import torch
+    c128 = torch.complex128
+
+    def leaf(t, dtype):
+        return t.detach().to(dtype).clone().requires_grad_(True)
+
+    out = {"obj_table": leaf(model.xy_hash_encoder.params, torch.float64),
+           "prb_table": leaf(model.probe_hash_encoder.params, torch.float64)}
+    for key, mlp in (("obj", model.object_complex_mlp), ("prb", model.probe_complex_mlp)):
+        layers = list(mlp.net)
+        lay = []
+        for i, l in enumerate(layers[:-1]):
+            lay.append((leaf(l.linear.weight, torch.float64 if i == 0 else c128),
+                        leaf(l.linear.bias, torch.float64 if i == 0 else c128), leaf(l.omega_0, torch.float64)))
+        out[key] = {"layers": lay, "wo": leaf(layers[-1].weight, c128), "bo": leaf(layers[-1].bias, c128)}
+    out["k_probe"] = leaf(model.k_probe, torch.float64)
+    return out
+
+
+def _fp64_leaves(P):
+    leaves = [P["obj_table"], P["prb_table"], P["k_probe"]]
+    for key in ("obj", "prb"):
+        for (w, b, om) in P[key]["layers"]:
+            leaves += [w, b, om]
+        leaves += [P[key]["wo"], P[key]["bo"]]
+    return leaves
+
+
+def _fp64_hashgrid(coords, table, spec):
+    """Hash-grid features in FP64; positions as two rounded f32 operations."""
+    import torch
+    x = coords.float()
+    tab = table.view(-1, 2)
+    outs = []
+    for l in range(spec.n_levels):
+        sc = torch.tensor(spec.scale[l], dtype=torch.float32, device=x.device)
+        pos = x * sc
+        pos = pos + 0.5
+        g = torch.floor(pos)
+        f = (pos - g).double()
+        gi = g.to(torch.int64)
+        acc = 0.0
+        for c in range(4):
+            cx = gi[:, 0] + (c & 1)
+            cy = gi[:, 1] + ((c >> 1) & 1)
+            if spec.dense[l]:
+                idx = (cx + cy * spec.res[l]) % spec.size[l]
+            else:
+                idx = ((cx & 0xFFFFFFFF) ^ ((cy * 2654435761) & 0xFFFFFFFF)) % spec.size[l]
+            wx = f[:, 0] if c & 1 else 1 - f[:, 0]
+            wy = f[:, 1] if c & 2 else 1 - f[:, 1]
+            acc = acc + (wx * wy)[:, None] * tab[spec.offset[l] + idx]
+        outs.append(acc)
+    return torch.cat(outs, 1)
+
+
+def _fp64_mlp(h, M):
+    import torch
+    import torch.nn.functional as F
+    z = h
+    for (w, b, om) in M["layers"]:
+        z = torch.exp(1j * om * F.linear(z, w, b))
+    return F.linear(z, M["wo"], M["bo"])
+
+
+def _fp64_step_loss(model, P, inp_bmn, tlY, tlX, idx, loss_type):
+    import torch
+    o = _fp64_mlp(_fp64_hashgrid(model.xy_coords, P["obj_table"], model.xy_hash_encoder.spec), P["obj"])
+    o = o.reshape(model.downsampled_width, model.downsampled_height)
+    o = _plain_upsample(o[None, None], model.downsample_factor)[0, 0]
+    p = _fp64_mlp(_fp64_hashgrid(model.probe_coords, P["prb_table"], model.probe_hash_encoder.spec), P["prb"])
+    p = P["k_probe"] * p.reshape(model.probe_width, model.probe_height) + model.probe_initial.to(torch.complex128)
+    M, N = p.shape
+    with torch.no_grad():   # Recovery.py:262-278 (device-side, no host syncs)
+        a2 = p.abs() ** 2
+        tot = a2.sum()
+        mn = torch.tensor([M, N], device=p.device, dtype=torch.float64)
+        cp = torch.trunc(mn / 2 - mn * torch.stack((a2.sum(1).cumsum(0).mean(), a2.sum(0).cumsum(0).mean())) / tot
+                         + 1).long()
+    p = torch.roll(p, (-int(cp[0]), -int(cp[1])), (0, 1))
+    p = p * torch.sqrt(M * N / (p.abs() ** 2).sum())
+    wins = torch.stack([o[tlY[i]:tlY[i] + M, tlX[i]:tlX[i] + N] for i in idx])
+    I = torch.abs(torch.fft.fftshift(torch.fft.fft2(p * wins), dim=(-2, -1))) ** 2
+    meas = inp_bmn[idx]
+    eps = 1e-8
+    if loss_type == "smooth_L1_loss":
+        d = I - meas
+        ad = d.abs()
+        per = torch.where(ad < 1, 0.5 * d * d, ad - 0.5).mean(dim=(-2, -1))
+    elif loss_type == "FD_loss":
+        dW = torch.sqrt(meas + eps) - torch.sqrt(I + eps)
+        Rx = torch.roll(dW, -1, -1) - dW
+        Ry = torch.roll(dW, -1, -2) - dW
+        per = torch.sqrt(Rx * Rx + Ry * Ry + eps).mean(dim=(-2, -1))
+    else:
+        per = (I - meas * torch.log(I + eps)).mean(dim=(-2, -1))
+    return per.sum(), o, p
+
+
+def sweep_mode(args):
+    """Object / probe NRMSE and loss of every precision variant against an FP64 run."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    shapes, log2T, batch, _, wl_name = WORKLOADS[args.workload]
+    loss_type = args.loss or "Poisson_likelihood_loss"
+    inst = pkg.synthetic.make_instance(seed=0, **shapes)
+    M = inst.probe.shape[0]
+    frames = inst.stack.shape[2]
+    mask = pkg.metrics.coverage_mask(inst.tlY, inst.tlX, M, M, inst.obj.shape, min_count=2)
+    rng = np.random.default_rng(1)
+    batches = [[perm[b:b + batch] for b in range(0, frames, batch)]
+               for perm in (rng.permutation(frames) for _ in range(args.epochs))]
+
+    def make(mlp_p="fp32", tab_p="fp32"):
+        torch.manual_seed(0)
+        return pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M,
+                                     log2_hashmap_size=log2T, hidden_omega_0=1.0, trainable_omega0=True,
+                                     use_residual=True, mlp_precision=mlp_p, table_precision=tab_p,
+                                     probe_initial=torch.from_numpy(inst.probe.astype(np.complex64))).to(dev)
+
+    def nrmse(a, b, shift):
+        return float(pkg.metrics.aligned_nrmse(np.asarray(a), np.asarray(b), mask if a.shape == mask.shape else None,
+                                               max_shift=shift))
+
+    # FP64 reference (Adam in FP64, same batches)
+    print(f"sweep: FP64 reference, {args.epochs} epochs of {len(batches[0])} steps", file=sys.stderr, flush=True)
+    model64 = make()
+    P = _fp64_params(model64)
+    opt = torch.optim.Adam(_fp64_leaves(P), lr=1e-3)
+    inp = torch.from_numpy(inst.stack).to(dev).permute(2, 0, 1).contiguous().double()
+    t0 = time.perf_counter()
+    ref_losses = []
+    for ep, eb in enumerate(batches):
+        tot = 0.0
+        for idx in eb:
+            opt.zero_grad()
+            loss, _, _ = _fp64_step_loss(model64, P, inp, inst.tlY, inst.tlX, list(idx), loss_type)
+            loss.backward()
+            opt.step()
+            tot += float(loss)
+        ref_losses.append(tot / len(eb))
+        if (ep + 1) % 10 == 0:
+            print(f"sweep: FP64 epoch {ep + 1} loss {ref_losses[-1]:.6g} ({time.perf_counter() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
+    with torch.no_grad():
+        _, o64, p64 = _fp64_step_loss(model64, P, inp, inst.tlY, inst.tlX, [0], loss_type)
+    o64, p64 = o64.cpu().numpy(), p64.cpu().numpy()
+    rows = [{"variant": "fp64 reference", "final_epoch_loss": ref_losses[-1],
+             "object_nrmse_vs_truth": nrmse(o64, inst.obj, 2), "wall_s": time.perf_counter() - t0}]
+    for mlp_p, tab_p in (("fp32", "fp32"), ("bf16", "fp32"), ("fp8", "fp32"),
+                         ("fp32", "fp16"), ("bf16", "fp16"), ("fp8", "fp16")):
+        model = make(mlp_p, tab_p)
+        eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                      loss_type=loss_type, lr=1e-3, batch_size=batch)
+        t0 = time.perf_counter()
+        losses = []
+        for eb in batches:
+            acc = torch.zeros((), device=dev)
+            for idx in eb:
+                acc = acc + eng.step(torch.as_tensor(idx, dtype=torch.int32, device=dev)).sum()
+            losses.append(acc)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        o, p = eng.current_fields()
+        o, p = o.detach().cpu().numpy(), p.detach().cpu().numpy()
+        fl = float(losses[-1]) / len(batches[-1])
+        rows.append({"variant": f"mlp {mlp_p}, table {tab_p}", "final_epoch_loss": fl,
+                     "loss_rel_vs_fp64": abs(fl - ref_losses[-1]) / abs(ref_losses[-1]),
+                     "object_nrmse_vs_fp64": nrmse(o, o64, 0), "probe_nrmse_vs_fp64": nrmse(p, p64, 0),
+                     "object_nrmse_vs_truth": nrmse(o, inst.obj, 2), "wall_s": wall})
+        print(f"sweep: {rows[-1]}", file=sys.stderr, flush=True)
+    print(json.dumps({"mode": "precision sweep", "workload": wl_name, "loss": loss_type, "epochs": args.epochs,
+                      "steps_per_epoch": len(batches[0]), "rows": rows,
+                      "note": "object/probe NRMSE against the FP64 run: global complex scale fitted, no shift; "
+                              "against the truth: shift <= 2 px, phase and scale fitted, coverage mask"}))
+
+
 def nrmse_mode(args):
     """Wall-clock to the plain run's final object NRMSE: plain PyTorch vs fused HIP."""
     import numpy as np
@@ -376,11 +563,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32", help="MLP GEMM precision")
+    ap.add_argument("--precision", choices=["fp32", "bf16", "fp8"], default="fp32", help="MLP GEMM precision")
     ap.add_argument("--table-precision", choices=["fp32", "fp16"], default="fp32",
                     help="hash-table storage the kernels gather from (f32 master kept for Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["throughput", "nrmse"], default="throughput")
+    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep"], default="throughput")
+    ap.add_argument("--loss", default=None, help="sweep mode: loss type (default Poisson_likelihood_loss)")
     ap.add_argument("--workload", default=None,
                     help="throughput: cfg1 (default) | cfg0 | cfg3;  nrmse mode: cfg0 (default) | cfg1 | small")
     ap.add_argument("--epochs", type=int, default=200)
@@ -401,6 +589,10 @@ def main():
     if args.mode == "nrmse":
         args.workload = args.workload or "cfg0"
         nrmse_mode(args)
+        return
+    if args.mode == "sweep":
+        args.workload = args.workload or "cfg1"
+        sweep_mode(args)
         return
     args.workload = args.workload or "cfg1"
     shapes, _, batch, loss_type, wl_name = WORKLOADS[args.workload]
@@ -489,7 +681,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" + ("" if args.precision == "fp32" else " (MLP GEMMs bf16, fp32 accumulate)")
+        "dtype": "f32" + ("" if args.precision == "fp32" else f" (MLP GEMMs {args.precision}, fp32 accumulate)")
                  + ("" if args.table_precision == "fp32" else " (hash tables fp16)"),
         "data": f"synthetic (seeded 1/f^2 object, soft-aperture probe, Poisson {shapes['photons']:.0e} photons)",
         "config": {"workload": wl_name, "steps_per_iter": steps_per_iter, "global_batch": batch,

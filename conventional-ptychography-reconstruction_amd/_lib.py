@@ -12,12 +12,12 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libptycho_hip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_LEVELS = 16
 MAX_HIDDEN = 4
 
 LOSS_CODES = {"smooth_L1_loss": 0, "FD_loss": 1, "Poisson_likelihood_loss": 2}
-PREC_CODES = {"fp32": 0, "bf16": 1}
+PREC_CODES = {"fp32": 0, "bf16": 1, "fp8": 2}
 TABLE_CODES = {"fp32": 0, "fp16": 1}
 
 
@@ -37,7 +37,7 @@ class PtyMlp(ctypes.Structure):
         ("w0", c_void_p), ("b0", c_void_p),
         ("wh", c_void_p * MAX_HIDDEN), ("bh", c_void_p * MAX_HIDDEN),
         ("omega", c_void_p * (MAX_HIDDEN + 1)),
-        ("wo", c_void_p), ("bo", c_void_p),
+        ("wo", c_void_p), ("bo", c_void_p), ("fp8_state", c_void_p),
     ]
 
 
@@ -96,6 +96,7 @@ _SIGNATURES = {
     "pty_adam_tick": (c_int32, [c_void_p, c_void_p]),
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
     "pty_cast_f16": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "pty_fp8_roll": (c_int32, [c_void_p, c_void_p]),
 }
 
 _lib = None

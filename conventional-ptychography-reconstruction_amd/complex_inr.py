@@ -53,6 +53,13 @@ class ComplexMLP(nn.Module):
         net.append(nn.Linear(hidden_features, out_features, dtype=torch.cfloat))
         self.net = nn.Sequential(*net)
         self.precision = "fp32"
+        self._fp8_state = None
+
+    def fp8_state(self, device) -> torch.Tensor:
+        """This MLP's FP8 scaling state on ``device`` (created on first use)."""
+        if self._fp8_state is None or self._fp8_state.device != torch.device(device):
+            self._fp8_state = fp8_state_tensor(device)
+        return self._fp8_state
 
     def tensors(self):
         """[w0, b0, omega0, (A_k, c_k, omega_k)*, wo, bo] in kernel order."""
@@ -85,12 +92,27 @@ class ComplexMLP(nn.Module):
         return mlp_forward(self, coords).real
 
 
-def mlp_descriptor(ts, n_in: int, precision: str = "fp32") -> _lib.PtyMlp:
+FP8_STATE_FLOATS = 520   # PTY_FP8_STATE_FLOATS: 8 scales + 8 x 64 amax sub-slots
+
+
+def fp8_state_tensor(device) -> torch.Tensor:
+    """Delayed-scaling state of an FP8 MLP: 8 scales (1 until a step has
+    recorded amax values), then the zeroed amax sub-slots."""
+    t = torch.zeros(FP8_STATE_FLOATS, dtype=torch.float32, device=device)
+    t[:8] = 1.0
+    return t
+
+
+def mlp_descriptor(ts, n_in: int, precision: str = "fp32", fp8_state: torch.Tensor | None = None) -> _lib.PtyMlp:
     n_hidden = (len(ts) - 5) // 3
     d = _lib.PtyMlp()
     d.n_in = n_in
     d.n_hidden = n_hidden
     d.precision = _lib.PREC_CODES[precision]
+    if precision == "fp8":
+        if fp8_state is None:
+            raise ValueError("an FP8 MLP needs its fp8_state tensor")
+        d.fp8_state = fp8_state.data_ptr()
     d.w0, d.b0 = ts[0].data_ptr(), ts[1].data_ptr()
     d.omega[0] = ts[2].data_ptr()
     for k in range(n_hidden):
@@ -183,14 +205,16 @@ class InrFieldFn(torch.autograd.Function):
     """Complex field [n] = ComplexMLP.net(HashGrid(points)), fused on the device."""
 
     @staticmethod
-    def forward(ctx, spec: GridSpec, points, precision: str, table_precision: str, table, *ts):
+    def forward(ctx, spec: GridSpec, points, precision: str, table_precision: str, fp8_state, table, *ts):
         _lib.require_cuda(table, *ts)
         ts = tuple(t.contiguous() for t in ts)
         out = torch.empty(points.n, dtype=torch.complex64, device=table.device)
         storage = table_storage(table, table_precision)
-        _lib.call("pty_inr_fwd", spec.descriptor(table_precision), mlp_descriptor(ts, spec.n_output_dims, precision),
+        _lib.call("pty_inr_fwd", spec.descriptor(table_precision),
+                  mlp_descriptor(ts, spec.n_output_dims, precision, fp8_state),
                   storage.data_ptr(), points.descriptor(spec), out.data_ptr(), _lib.stream_ptr(table.device))
         ctx.spec, ctx.points, ctx.precision, ctx.table_precision = spec, points, precision, table_precision
+        ctx.fp8_state = fp8_state
         ctx.n_table = table.numel()
         ctx.save_for_backward(storage, *ts)
         return out
@@ -203,14 +227,18 @@ class InrFieldFn(torch.autograd.Function):
         gout = gout.resolve_conj().contiguous()
         grads = [torch.zeros_like(t) for t in ts]
         dtable = torch.zeros(ctx.n_table, dtype=torch.float32, device=storage.device)
-        desc = mlp_descriptor(ts, spec.n_output_dims, precision)
-        gdesc = mlp_descriptor(grads, spec.n_output_dims, precision)
+        desc = mlp_descriptor(ts, spec.n_output_dims, precision, ctx.fp8_state)
+        gdesc = mlp_descriptor(grads, spec.n_output_dims, precision, ctx.fp8_state)
         pdesc = points.descriptor(spec)
         nbytes = _lib.query("pty_inr_bwd_workspace", grid, desc, pdesc)
         ws = _lib.workspace(nbytes, storage.device)
+        stream = _lib.stream_ptr(storage.device)
         _lib.call("pty_inr_bwd", grid, desc, gdesc, storage.data_ptr(), dtable.data_ptr(), pdesc,
-                  gout.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr(storage.device))
-        return (None, None, None, None, dtable, *grads)
+                  gout.data_ptr(), ws.data_ptr(), ws.numel(), stream)
+        if precision == "fp8":
+            # scales of the next step from the amax this step recorded
+            _lib.call("pty_fp8_roll", ctx.fp8_state.data_ptr(), stream)
+        return (None, None, None, None, None, dtable, *grads)
 
 
 class UpsampleFn(torch.autograd.Function):
@@ -384,8 +412,9 @@ class ComplexINRModel2D(nn.Module):
 
     def _field(self, encoder: HashGridEncoding, mlp: ComplexMLP, which: str):
         mlp.check_supported()
+        state = mlp.fp8_state(encoder.params.device) if self.mlp_precision == "fp8" else None
         return InrFieldFn.apply(encoder.spec, self.grid_points(which), self.mlp_precision, encoder.table_precision,
-                                encoder.params, *mlp.tensors())
+                                state, encoder.params, *mlp.tensors())
 
     def fourier_upsample(self, x, scale_factor):
         return fourier_upsample(x, scale_factor)

@@ -407,8 +407,9 @@ static int validate_mlp(const PtyMlp* m, bool need_all) {
   PTY_REQUIRE(m != nullptr, PTY_EINVAL, "mlp descriptor is NULL");
   PTY_REQUIRE(m->n_in >= 1 && m->n_in <= 32, PTY_EUNSUP, "mlp: n_in %d outside 1..32", m->n_in);
   PTY_REQUIRE(m->n_hidden >= 1 && m->n_hidden <= 2, PTY_EUNSUP, "mlp: hidden_layers %d outside 1..2", m->n_hidden);
-  PTY_REQUIRE(m->precision == PTY_PREC_FP32 || m->precision == PTY_PREC_BF16, PTY_EUNSUP,
-              "mlp: precision %d not built", m->precision);
+  PTY_REQUIRE(m->precision == PTY_PREC_FP32 || m->precision == PTY_PREC_BF16 || m->precision == PTY_PREC_FP8,
+              PTY_EUNSUP, "mlp: precision %d not built", m->precision);
+  PTY_REQUIRE(m->precision != PTY_PREC_FP8 || m->fp8_state, PTY_EINVAL, "mlp: FP8 needs fp8_state");
   if (need_all) {
     PTY_REQUIRE(m->w0 && m->b0 && m->wo && m->bo, PTY_EINVAL, "mlp: null weight pointer");
     for (int k = 0; k < m->n_hidden; ++k)
@@ -459,8 +460,8 @@ extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* tab
   PTY_REQUIRE(table && out, PTY_EINVAL, "inr_fwd: null pointer");
   if (pts->n == 0) return PTY_OK;
   const int blocks = fwd_blocks(pts->n);
-  if (mlp->precision == PTY_PREC_BF16)
-    launch_inr_fwd_bf16(*g, *mlp, (const float2*)table, *pts, (float2*)out, blocks, stream);
+  if (mlp->precision != PTY_PREC_FP32)
+    launch_inr_fwd_lowp(*g, *mlp, (const float2*)table, *pts, (float2*)out, blocks, stream);
   else if (mlp->n_hidden == 1)
     hipLaunchKernelGGL(k_inr_fwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, *pts,
                        (float2*)out);
@@ -528,8 +529,8 @@ extern "C" int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float*
   PTY_REQUIRE(table && dout, PTY_EINVAL, "inr_bwd: null pointer");
   if (pts->n == 0) return PTY_OK;
   const BwdWork b = bwd_work(g, mlp, pts, work);
-  if (mlp->precision == PTY_PREC_BF16)
-    launch_inr_bwd_bf16(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
+  if (mlp->precision != PTY_PREC_FP32)
+    launch_inr_bwd_lowp(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
                         stream);
   else if (mlp->n_hidden == 1)
     hipLaunchKernelGGL(k_inr_bwd<1>, dim3(b.blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table,

@@ -246,10 +246,10 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-// BF16 launchers (inr_bf16.hip)
-void launch_inr_fwd_bf16(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
+// BF16 / FP8 launchers (inr_lowp.hip)
+void launch_inr_fwd_lowp(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
                          int blocks, hipStream_t stream);
-void launch_inr_bwd_bf16(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
+void launch_inr_bwd_lowp(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
                          const float2* dout, float* partial, int blocks, hipStream_t stream);
 
 }  // namespace pty

@@ -135,10 +135,17 @@ class FusedReconstruction:
         self.prb_pts = model.grid_points("probe")
         self.obj_pdesc = self.obj_pts.descriptor(self.obj_spec)
         self.prb_pdesc = self.prb_pts.descriptor(self.prb_spec)
-        self.obj_mlp = mlp_descriptor(self.obj_ts, nin, prec)
-        self.prb_mlp = mlp_descriptor(self.prb_ts, nin, prec)
-        self.obj_gmlp = mlp_descriptor([self.flat.g(t) for t in self.obj_ts], nin, prec)
-        self.prb_gmlp = mlp_descriptor([self.flat.g(t) for t in self.prb_ts], nin, prec)
+        # FP8: per-MLP delayed-scaling state, rolled after every backward
+        self.fp8_states = []
+        st_o = st_p = None
+        if prec == "fp8":
+            st_o = model.object_complex_mlp.fp8_state(dev)
+            st_p = model.probe_complex_mlp.fp8_state(dev)
+            self.fp8_states = [st_o, st_p]
+        self.obj_mlp = mlp_descriptor(self.obj_ts, nin, prec, st_o)
+        self.prb_mlp = mlp_descriptor(self.prb_ts, nin, prec, st_p)
+        self.obj_gmlp = mlp_descriptor([self.flat.g(t) for t in self.obj_ts], nin, prec, st_o)
+        self.prb_gmlp = mlp_descriptor([self.flat.g(t) for t in self.prb_ts], nin, prec, st_p)
         self.obj_dtable = self.flat.g(model.xy_hash_encoder.params)
         self.prb_dtable = self.flat.g(model.probe_hash_encoder.params)
 
@@ -175,6 +182,11 @@ class FusedReconstruction:
         self.epoch = 0
         self._timers = None
         self._grad_clear = True   # FlatParams starts with zero gradients
+        if self.fp8_states:
+            # calibration: one step without the update records the amax of
+            # every FP8 tensor, so the first real step has scales (its
+            # gradients are cleared by the next step)
+            self.step(torch.arange(min(self.batch_size, self.frames), dtype=torch.int32, device=dev), update=False)
 
     # ------------------------------------------------------------------ timing
     def enable_timing(self, names):
@@ -277,6 +289,8 @@ class FusedReconstruction:
                       self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, s)
         self._inr_bwd("prb", self.prb_grid, self.prb_mlp, self.prb_gmlp, self.prb_tab,
                       self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, s)
+        for st in self.fp8_states:
+            C("pty_fp8_roll", st.data_ptr(), s)
         if update:
             # Adam clears the trainable gradients it consumes; frozen ones are
             # cleared with them when they exist

@@ -5,7 +5,7 @@ sections, float/int coercion lists, enum validation), :80-158 (defaults,
 init_type logic), :201-222 (model construction), :227-350 (training loop:
 per-epoch randperm, ceil(frames/batch) Adam steps, StepLR per epoch).
 Additive keys: ``canvas_size`` (explicit square canvas; the scan is centred
-on it), ``mlp_precision`` ("fp32" | "bf16"), ``table_precision`` ("fp32" |
+on it), ``mlp_precision`` ("fp32" | "bf16" | "fp8"), ``table_precision`` ("fp32" |
 "fp16", the hash-table storage the kernels gather from), ``seed``.
 """
 from __future__ import annotations
@@ -58,8 +58,8 @@ def normalize_config(raw: dict) -> dict:
         # every reference config sets the weight to 0 and the term is not built
         raise ValueError("sparsity_weight > 0 (Haar wavelet sparsity) is not part of the gfx950 build")
     for key in ("mlp_precision",):
-        if cfg.get(key, "fp32") not in ("fp32", "bf16"):
-            raise ValueError(f"{key} must be 'fp32' or 'bf16'")
+        if cfg.get(key, "fp32") not in ("fp32", "bf16", "fp8"):
+            raise ValueError(f"{key} must be 'fp32', 'bf16' or 'fp8'")
     if cfg.get("table_precision", "fp32") not in ("fp32", "fp16"):
         raise ValueError("table_precision must be 'fp32' or 'fp16'")
     return cfg

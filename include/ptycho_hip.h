@@ -31,9 +31,10 @@ extern "C" {
 
 typedef struct ihipStream_t* hipStream_t;
 
-#define PTY_ABI_VERSION 3
+#define PTY_ABI_VERSION 4
 #define PTY_MAX_LEVELS 16
 #define PTY_MAX_HIDDEN 4
+#define PTY_FP8_STATE_FLOATS 520
 
 enum {
   PTY_OK = 0,
@@ -44,7 +45,7 @@ enum {
 };
 
 enum { PTY_LOSS_SMOOTH_L1 = 0, PTY_LOSS_FD = 1, PTY_LOSS_POISSON = 2 };
-enum { PTY_PREC_FP32 = 0, PTY_PREC_BF16 = 1 };
+enum { PTY_PREC_FP32 = 0, PTY_PREC_BF16 = 1, PTY_PREC_FP8 = 2 };
 /* Hash-table storage read by the forward kernels. */
 enum { PTY_TABLE_F32 = 0, PTY_TABLE_F16 = 1 };
 
@@ -92,6 +93,15 @@ typedef struct PtyMlp {
   float* omega[PTY_MAX_HIDDEN + 1];
   float* wo;
   float* bo;
+  /* PTY_PREC_FP8 only (else ignored, may be NULL): PTY_FP8_STATE_FLOATS
+   * floats of delayed per-tensor scaling state, [0, 8) the scales in use
+   * (224 / amax of the previous step, one binade below the e4m3 maximum 448;
+   * 1 before any), then the amax recorded
+   * by this step's kernels (features, hidden inputs, pre-activation
+   * gradients; 64 sub-slots per tensor, zero-initialised).  The forward and
+   * backward of one step read the same scales; pty_fp8_roll moves the
+   * recorded amax into the scales between steps. */
+  float* fp8_state;
 } PtyMlp;
 
 /* Coordinate source of the INR evaluation: either explicit coords [n, 2]
@@ -161,6 +171,10 @@ int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, 
 size_t pty_points_plan_bytes(const PtyGrid* g, const PtyPoints* pts);
 int pty_points_plan_build(const PtyGrid* g, const PtyPoints* pts, void* plan, size_t plan_bytes,
                           hipStream_t stream);
+/* Delayed-scaling update of an FP8 MLP's state (no counterpart in the
+ * reference, which has no FP8 path): scales = 224 / recorded amax, amax
+ * cleared.  Call once per optimizer step, after the backward. */
+int pty_fp8_roll(float* fp8_state, hipStream_t stream);
 /* ComplexMLP.net over given features [n, n_in] (complex_inr.py:313). */
 int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream);
 

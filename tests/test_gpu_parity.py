@@ -527,6 +527,69 @@ def test_fused_step_is_deterministic(pkg, dev):
     assert torch.equal(engines[0].flat.data, engines[1].flat.data)
 
 
+def _mlp_grads_vs_oracle(model, params):
+    """(name, error) per MLP tensor.  Weights and biases: relative error (norm).
+    omega: omega * dL/domega = Re<G_W, W> + Re<G_b, b> sums terms that cancel,
+    so its error is measured against the Cauchy-Schwarz scale
+    (|G_W| |W| + |G_b| |b|) / |omega| of that identity."""
+    mlp = params["obj_mlp"]
+    ours = model.object_complex_mlp.tensors()
+    out, i = [], 0
+    for li, (w, b, om) in enumerate(mlp["layers"]):
+        out.append((f"W{li}", rel_err(ours[i].grad, w.grad)))
+        out.append((f"b{li}", rel_err(ours[i + 1].grad, b.grad)))
+        scale = (w.grad.abs().pow(2).sum().sqrt() * w.detach().abs().pow(2).sum().sqrt()
+                 + b.grad.abs().pow(2).sum().sqrt() * b.detach().abs().pow(2).sum().sqrt()).item() / abs(om.item())
+        out.append((f"omega{li}", abs(ours[i + 2].grad.item() - om.grad.item()) / scale))
+        i += 3
+    out.append(("Wout", rel_err(ours[i].grad, mlp["out_weight"].grad)))
+    out.append(("bout", rel_err(ours[i + 1].grad, mlp["out_bias"].grad)))
+    return out
+
+
+@pytest.mark.parametrize("hidden_layers", [1, 2])
+def test_inr_field_fp8(pkg, dev, hidden_layers):
+    """FP8 e4m3 MFMA path with per-tensor scales: the delayed-scaling state
+    holds 224 / amax of what the previous pass saw, and field and gradients
+    stay within the error of 3-bit mantissas against the FP64 oracle.
+
+    Tolerance: e4m3 rounds to 2^-4 relative (about 3.6 % RMS per operand);
+    each of the three forward GEMMs perturbs the phases by a few percent of a
+    radian (omega_0 = 10 on layer 0), so the field is held to 1.2e-1 and the
+    parameter gradients (two more GEMMs) to 3e-1."""
+    model = _model(pkg, dev, 64, 16, log2T=12, residual=False, hidden_layers=hidden_layers)
+    model.mlp_precision = "fp8"
+    with torch.no_grad():   # features of order 0.1 as in the BF16 test (omega_0 = 10)
+        model.xy_hash_encoder.params.uniform_(-0.1, 0.1)
+    params = oracle_params(model)
+    table = oracle_table(model)
+    ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], table, params["obj_mlp"], 32, 32)
+    g = torch.randn(32 * 32, dtype=torch.complex64, device=dev)
+    state = model.object_complex_mlp.fp8_state(dev)
+    # pass 1 calibrates the scales (amax recorded, rolled after the backward)
+    f = model._field(model.xy_hash_encoder, model.object_complex_mlp, "object")
+    (torch.real(torch.conj(g) * f)).sum().backward()
+    feats = orc.hashgrid_encode(model.xy_coords.cpu().numpy(), params["obj_table"].detach(), table)
+    s = state.cpu()
+    assert abs(s[0].item() - 224.0 / feats.abs().max().item()) < 1e-5 * s[0].item()
+    assert s.numel() == 520 and (s[8:] == 0).all()
+    assert (s[1:1 + hidden_layers] != 1).all() and (s[3:4 + hidden_layers] != 1).all()
+    for p in model.parameters():
+        p.grad = None
+    f = model._field(model.xy_hash_encoder, model.object_complex_mlp, "object")
+    err_f = rel_err(f.reshape(32, 32), ref)
+    print(f"fp8 H={hidden_layers}: field rel err {err_f:.4f}")
+    assert err_f < 1.2e-1, err_f
+    (torch.real(torch.conj(g) * f)).sum().backward()
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref.reshape(-1))).sum().backward()
+    e_t = rel_err(model.xy_hash_encoder.params.grad, params["obj_table"].grad)
+    errs = _mlp_grads_vs_oracle(model, params)
+    print(f"fp8 H={hidden_layers}: table grad {e_t:.4f}, mlp grads {[round(e, 4) for _, e in errs]}")
+    assert e_t < 3e-1
+    for shape, e in errs:
+        assert e < 3e-1, (shape, e)
+
+
 @pytest.mark.parametrize("hidden_layers", [1, 2])
 def test_inr_field_bf16(pkg, dev, hidden_layers):
     """BF16 MFMA path (fp32 accumulate): field and gradients against the FP64 oracle.

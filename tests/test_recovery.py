@@ -54,7 +54,7 @@ def test_load_config_flattens_and_coerces(pkg, tmp_path):
     (("sparsity_weight: 0.0", "sparsity_weight: 0.5"), "sparsity"),
     (("use_residual: true", "use_residual: true\n  model_type: gaussian_field"), "gaussian_field"),
     (("use_residual: true", "use_residual: true\n  table_precision: fp8"), "table_precision"),
-    (("use_residual: true", "use_residual: true\n  mlp_precision: fp8"), "mlp_precision"),
+    (("use_residual: true", "use_residual: true\n  mlp_precision: fp4"), "mlp_precision"),
 ])
 def test_load_config_rejects(pkg, tmp_path, edit, message):
     with pytest.raises(ValueError, match=message):
