@@ -495,6 +495,99 @@ def sweep_mode(args):
                               "against the truth: shift <= 2 px, phase and scale fitted, coverage mask"}))
 
 
+# ----------------------------------------------------------------------------
+# learned warm start: train the conv net on simulated instances, then compare
+# warm-started and plain (initProbe) reconstructions on held-out seeds by
+# epochs and wall-seconds to the plain run's final object NRMSE.
+# ----------------------------------------------------------------------------
+def warmstart_mode(args):
+    import numpy as np
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    ws = pkg.warmstart
+    dev = torch.device("cuda:0")
+    shapes, log2T, batch, loss_type, wl_name = WORKLOADS[args.workload]
+    shape = {k: shapes[k] for k in ("canvas", "M", "n_side", "step", "jitter", "photons")}
+    M, canvas = shape["M"], shape["canvas"]
+    t0 = time.perf_counter()
+    seeds = list(range(1000, 1000 + args.ws_train))
+    x, y = ws.build_dataset(seeds, device=dev, **shape)
+    t_data = time.perf_counter() - t0
+    print(f"warmstart: {x.shape[0]} training pairs from {len(seeds)} instances ({t_data:.1f} s)",
+          file=sys.stderr, flush=True)
+    torch.manual_seed(0)
+    net = ws.WarmStartNet().to(dev)
+    t0 = time.perf_counter()
+    hist = ws.train(net, x, y, epochs=args.ws_epochs,
+                    log=lambda e, l: print(f"warmstart: net epoch {e} mse {l:.5g}", file=sys.stderr, flush=True))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del x, y
+    net.eval()
+    rows = []
+    for k in range(args.ws_heldout):
+        inst = ws.simulate(900000 + k, jitter_probe=False, device=dev, **shape)
+        truth = inst.obj.cpu().numpy()
+        ty, tx = inst.tlY.cpu().numpy(), inst.tlX.cpu().numpy()
+        mask = pkg.metrics.coverage_mask(ty, tx, M, M, truth.shape, min_count=2)
+        stack = inst.intens.permute(1, 2, 0).contiguous()          # reference layout [M, N, frames]
+
+        def err(o):
+            return pkg.metrics.aligned_nrmse(o.detach().cpu().numpy(), truth, mask, max_shift=2)
+
+        def run(object_initial, target=None):
+            torch.manual_seed(0)
+            model = pkg.ComplexINRModel2D(canvas, canvas, 2, probe_width=M, probe_height=M, log2_hashmap_size=log2T,
+                                          hidden_omega_0=1.0, trainable_omega0=True, use_residual=True,
+                                          object_initial=object_initial,
+                                          probe_initial=inst.probe.to(torch.complex64)).to(dev)
+            eng = pkg.FusedReconstruction(model, stack, ty, tx, loss_type=loss_type, lr=1e-3, batch_size=batch)
+            gen = torch.Generator(device=dev)
+            gen.manual_seed(0)
+            curve, hit = [], None
+            t_start = time.perf_counter()
+            for ep in range(args.epochs):
+                eng.run_epoch(gen)
+                if (ep + 1) % args.check_every == 0 or ep == args.epochs - 1:
+                    torch.cuda.synchronize()
+                    e = err(eng.current_fields()[0])
+                    t = time.perf_counter() - t_start
+                    curve.append((ep + 1, t, e))
+                    if target is not None and hit is None and e <= target:
+                        hit = (ep + 1, t)
+            return curve, hit
+
+        curve_plain, _ = run(None)
+        target = curve_plain[-1][2] * 1.01
+        t0 = time.perf_counter()
+        init = ws.warm_start_object(net, inst.intens, ty, tx, M, canvas)
+        torch.cuda.synchronize()
+        t_net = time.perf_counter() - t0
+        curve_warm, hit = run(init, target)
+        row = {"seed": 900000 + k, "plain_final_nrmse": curve_plain[-1][2], "plain_wall_s": curve_plain[-1][1],
+               "plain_epochs": args.epochs, "warm_initial_nrmse": err(init), "warm_net_s": t_net,
+               "warm_final_nrmse": curve_warm[-1][2],
+               "warm_epochs_to_plain_final": hit[0] if hit else None,
+               "warm_wall_s_to_plain_final": (hit[1] + t_net) if hit else None,
+               "plain_curve": curve_plain, "warm_curve": curve_warm}
+        rows.append(row)
+        print(f"warmstart: seed {row['seed']} plain {row['plain_final_nrmse']:.4f} in {args.epochs} epochs; warm "
+              f"init {row['warm_initial_nrmse']:.4f}, reaches it at epoch {row['warm_epochs_to_plain_final']}",
+              file=sys.stderr, flush=True)
+    hits = [r for r in rows if r["warm_epochs_to_plain_final"]]
+    print(json.dumps({
+        "mode": "learned warm start", "workload": wl_name, "train_instances": len(seeds),
+        "train_pairs": len(seeds) * shape["n_side"] ** 2, "net_epochs": args.ws_epochs, "net_mse": hist,
+        "data_s": t_data, "train_s": t_train, "reconstruction_epochs": args.epochs, "heldout": len(rows),
+        "heldout_reaching_target": len(hits),
+        "mean_epoch_ratio_warm_over_plain": (float(np.mean([r["warm_epochs_to_plain_final"] / args.epochs
+                                                            for r in hits])) if hits else None),
+        "mean_wall_ratio_warm_over_plain": (float(np.mean([r["warm_wall_s_to_plain_final"] / r["plain_wall_s"]
+                                                           for r in hits])) if hits else None),
+        "rows": rows}))
+
+
 def nrmse_mode(args):
     """Wall-clock to the plain run's final object NRMSE: plain PyTorch vs fused HIP."""
     import numpy as np
@@ -567,7 +660,10 @@ def main():
     ap.add_argument("--table-precision", choices=["fp32", "fp16"], default="fp32",
                     help="hash-table storage the kernels gather from (f32 master kept for Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep"], default="throughput")
+    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart"], default="throughput")
+    ap.add_argument("--ws-train", type=int, default=200, help="warmstart: training instances")
+    ap.add_argument("--ws-epochs", type=int, default=8, help="warmstart: net training epochs")
+    ap.add_argument("--ws-heldout", type=int, default=3, help="warmstart: held-out seeds")
     ap.add_argument("--loss", default=None, help="sweep mode: loss type (default Poisson_likelihood_loss)")
     ap.add_argument("--workload", default=None,
                     help="throughput: cfg1 (default) | cfg0 | cfg3;  nrmse mode: cfg0 (default) | cfg1 | small")
@@ -593,6 +689,10 @@ def main():
     if args.mode == "sweep":
         args.workload = args.workload or "cfg1"
         sweep_mode(args)
+        return
+    if args.mode == "warmstart":
+        args.workload = args.workload or "cfg1"
+        warmstart_mode(args)
         return
     args.workload = args.workload or "cfg1"
     shapes, _, batch, loss_type, wl_name = WORKLOADS[args.workload]
