@@ -97,6 +97,9 @@ _SIGNATURES = {
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
     "pty_cast_f16": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "pty_fp8_roll": (c_int32, [c_void_p, c_void_p]),
+    "pty_haar_workspace": (c_size_t, [c_int64, c_int32]),
+    "pty_haar_sparsity": (c_int32, [c_void_p, c_int64, c_int32, c_float, c_float, c_void_p, c_void_p, c_void_p,
+                                    c_size_t, c_void_p]),
 }
 
 _lib = None

@@ -8,6 +8,7 @@ Per step (batch of positions ``batch_idx`` on the device):
   probe   = INR_prb(grid)                      [+ k_p * . + initial_p]
   probe_n = COM-roll + energy-normalise(probe)
   loss_b, dobject, dprobe_n = far field + loss + adjoint over the batch
+  dobject += sparsity_weight * d Haar(object)   (only when the weight is set)
   dprobe  = normalise^T(dprobe_n);  residual^T;  dsmall = Upsample^T(dobject)
   grads   = INR^T (recomputed activations, hash-table atomics)
   Adam over all trainable parameters (one flat launch)
@@ -80,7 +81,7 @@ class FusedReconstruction:
 
     def __init__(self, model: ComplexINRModel2D, stack_mnf: torch.Tensor, tlY, tlX, loss_type="smooth_L1_loss",
                  lr=1e-3, batch_size=10, lr_decay_step=10000, lr_decay_gamma=0.1, betas=(0.9, 0.999),
-                 eps=1e-8):
+                 eps=1e-8, sparsity_weight=0.0):
         dev = model.xy_hash_encoder.params.device
         if dev.type != "cuda":
             raise RuntimeError("FusedReconstruction needs the model on the GPU")
@@ -99,6 +100,12 @@ class FusedReconstruction:
         if self.M != self.N:
             raise ValueError("the fused step needs a square probe")
         check_windows(tlY, tlX, self.M, self.N, self.H, self.W)
+        # Recovery.py:309-313: sparsity_weight * Haar sparsity of the object
+        # joins the objective; at weight 0 the term is skipped (the reference
+        # evaluates it and multiplies by 0)
+        self.sparsity_weight = float(sparsity_weight)
+        if self.sparsity_weight and (self.H % 2 or self.W % 2):
+            raise ValueError("the Haar sparsity term needs an even canvas")
         self.frames = stack_mnf.shape[2]
         self.stack = prepare_stack(stack_mnf.to(dev))
         self.tl = positions_tensor(tlY, tlX, dev)
@@ -166,10 +173,13 @@ class FusedReconstruction:
         self.dprb_mlp = torch.empty(M, N, dtype=c64, device=dev) if model.has_probe_residual else self.dprobe
         self.dsmall = torch.empty(n, n, dtype=c64, device=dev)
         self.loss = torch.zeros(self.batch_size, dtype=torch.float32, device=dev)
+        # unweighted Haar sparsity of the object of the last step (0 when off)
+        self.sparsity = torch.zeros(1, dtype=torch.float32, device=dev)
 
         q = _lib.query
         self.ws_up = _lib.workspace(q("pty_upsample_workspace", n, self.ds), dev)
         self.ws_probe = _lib.workspace(q("pty_probe_workspace", M, N), dev)
+        self.ws_haar = _lib.workspace(q("pty_haar_workspace", H, W) if self.sparsity_weight else 1, dev)
         self.ws_res = _lib.workspace(max(q("pty_residual_workspace", H * W), q("pty_residual_workspace", M * N)),
                                      dev)
         self.ws_obj = _lib.workspace(q("pty_inr_bwd_workspace", self.obj_grid, self.obj_mlp, self.obj_pdesc), dev)
@@ -273,6 +283,9 @@ class FusedReconstruction:
           batch_idx.data_ptr(), self.tl.data_ptr(), self.loss.data_ptr(), self.dobj.data_ptr(),
           self.dprobe_n.data_ptr(), ws_far.data_ptr(), ws_far.numel(), s)
         self._mark("farfield", False)
+        if self.sparsity_weight:
+            C("pty_haar_sparsity", self.obj.data_ptr(), self.H, self.W, 1.0, self.sparsity_weight,
+              self.sparsity.data_ptr(), self.dobj.data_ptr(), self.ws_haar.data_ptr(), self.ws_haar.numel(), s)
         C("pty_probe_normalize_bwd", self.probe.data_ptr(), self.dprobe_n.data_ptr(), self.M, self.N,
           self.stats.data_ptr(), self.dprobe.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
         if m.has_probe_residual:

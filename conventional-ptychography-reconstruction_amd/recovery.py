@@ -53,10 +53,6 @@ def normalize_config(raw: dict) -> dict:
         raise ValueError(f"model_type must be one of {VALID_MODEL_TYPES}, got {model_type!r}")
     if model_type != "inr":
         raise ValueError("model_type 'gaussian_field' is not part of the gfx950 build")
-    if cfg.get("sparsity_weight") not in (None, 0.0):
-        # Recovery.py:309-313 adds weight * haar_wavelet_sparsity_loss(object);
-        # every reference config sets the weight to 0 and the term is not built
-        raise ValueError("sparsity_weight > 0 (Haar wavelet sparsity) is not part of the gfx950 build")
     for key in ("mlp_precision",):
         if cfg.get(key, "fp32") not in ("fp32", "bf16", "fp8"):
             raise ValueError(f"{key} must be 'fp32', 'bf16' or 'fp8'")
@@ -124,7 +120,8 @@ def build(cfg: dict, data: dict, device="cuda", object_initial=None):
     engine = FusedReconstruction(model, stack, tlY, tlX, loss_type=cfg.get("loss_type", "smooth_L1_loss"),
                                  lr=cfg.get("lr", 1e-3), batch_size=cfg.get("batch_size", 10),
                                  lr_decay_step=cfg.get("lr_decay_step", 10000),
-                                 lr_decay_gamma=cfg.get("lr_decay_gamma", 0.1))
+                                 lr_decay_gamma=cfg.get("lr_decay_gamma", 0.1),
+                                 sparsity_weight=cfg.get("sparsity_weight", 0.0) or 0.0)
     geom = dict(tlX=np.asarray(tlX), tlY=np.asarray(tlY), dx=dx, canvas=canvas, M=M, N=N)
     return model, engine, geom
 

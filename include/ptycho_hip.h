@@ -17,6 +17,7 @@
  *   apply_residual                     models/complex_inr.py:316-334
  *   probe COM roll + energy norm       Recovery.py:262-285
  *   per-position far field + losses    Recovery.py:287-307 (utils/optics.py:102-112)
+ *   haar_wavelet_sparsity_loss         utils/loss.py:60-83, Recovery.py:309-313
  *   torch.optim.Adam step              Recovery.py:238-241, 321
  */
 #ifndef PTYCHO_HIP_H
@@ -223,6 +224,19 @@ int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* prob
 int pty_farfield_intensity(const PtyFar* d, const float* object, const float* probe_n,
                            const int32_t* batch_idx, const int32_t* tl, float* intensity,
                            void* work, size_t work_bytes, hipStream_t stream);
+
+/* ---- Haar wavelet sparsity (utils/loss.py:60-83; Recovery.py:309-313) ---- */
+/* object: `rows` x W c64, rows and W even (a stack of images with even
+ * heights may be passed as one image: no 2x2 block crosses an image).
+ * loss[0] = scale * sum over the 2x2 blocks of the real and imaginary parts of
+ * |LH| + |HL| + |HH| (scale 1: reduction='sum'; 1 / (rows/2 * W/2):
+ * reduction='mean'), overwritten.  When dobject != NULL,
+ * dobject += grad_weight * d loss[0] / d object (PyTorch's complex gradient
+ * convention, sign(0) = 0), which is how Recovery.py:310 adds
+ * sparsity_weight * loss to the step's objective.  Deterministic. */
+size_t pty_haar_workspace(int64_t rows, int32_t W);
+int pty_haar_sparsity(const float* object, int64_t rows, int32_t W, float scale, float grad_weight,
+                      float* loss, float* dobject, void* work, size_t work_bytes, hipStream_t stream);
 
 /* ---- Adam (torch.optim.Adam, amsgrad=False, weight_decay=0; Recovery.py:238, 321) ---- */
 /* hyper (device, 2 floats): [lr, step]; bias corrections 1 - beta^step are

@@ -27,6 +27,10 @@ Pinning (what checks each function, see tests/test_oracle.py):
   farfield / losses    numpy.fft, Parseval, F.smooth_l1_loss, closed-form
                        gradients vs autograd, central differences.
   scan_positions_to_pixels  numpy FP64 round-half-even on constructed inputs.
+  haar_wavelet_sparsity_loss  explicit per-block loops, closed forms (impulse,
+                       constant, checkerboard), the orthonormal-Haar energy
+                       identity of the four bands, autograd vs central
+                       differences.
 """
 from __future__ import annotations
 
@@ -290,12 +294,17 @@ def batch_loss(obj, probe_n, stack_mnp, batch_idx, tlY, tlX, loss_type):
     return total
 
 
-def haar_wavelet_sparsity_loss(img):
-    """utils/loss.py:60-83 with reduction='sum'."""
+def haar_wavelet_sparsity_loss(img, reduction="sum"):
+    """utils/loss.py:60-83: L1 of the LH, HL, HH 2x2 Haar sub-bands of the
+    real and imaginary parts over the last two axes ('sum' or 'mean' per
+    band, as torch.sum / torch.mean there)."""
     if torch.is_complex(img):
-        return haar_wavelet_sparsity_loss(img.real) + haar_wavelet_sparsity_loss(img.imag)
-    a, b, c, d = img[0::2, 0::2], img[0::2, 1::2], img[1::2, 0::2], img[1::2, 1::2]
-    return (a - b + c - d).abs().sum() + (a + b - c - d).abs().sum() + (a - b - c + d).abs().sum()
+        return (haar_wavelet_sparsity_loss(img.real, reduction)
+                + haar_wavelet_sparsity_loss(img.imag, reduction))
+    a, b = img[..., 0::2, 0::2], img[..., 0::2, 1::2]
+    c, d = img[..., 1::2, 0::2], img[..., 1::2, 1::2]
+    red = torch.sum if reduction == "sum" else torch.mean
+    return red((a - b + c - d).abs()) + red((a + b - c - d).abs()) + red((a - b - c + d).abs())
 
 
 # ----------------------------------------------------------------------------
@@ -320,7 +329,7 @@ def scan_positions_to_pixels(xlocation, ylocation, probe_shape, wavelength,
 # ----------------------------------------------------------------------------
 
 def step_loss(params: dict, table: dict, geom: dict, stack_mnp, batch_idx, loss_type,
-              dtype=torch.float64):
+              dtype=torch.float64, sparsity_weight: float = 0.0):
     """Loss of one step for parameter dict ``params`` (autograd-ready).
 
     ``params`` keys: obj_table, prb_table, obj_mlp, prb_mlp (dicts with
@@ -336,6 +345,9 @@ def step_loss(params: dict, table: dict, geom: dict, stack_mnp, batch_idx, loss_
                          params.get("k_probe"), params.get("probe_initial"), dtype=dtype)
     pn = probe_center_normalize(p)
     loss = batch_loss(o, pn, stack_mnp, batch_idx, geom["tlY"], geom["tlX"], loss_type)
+    if sparsity_weight:
+        # Recovery.py:309-313: the weighted Haar term joins the objective
+        loss = loss + sparsity_weight * haar_wavelet_sparsity_loss(o)
     return loss, o, pn
 
 

@@ -64,14 +64,14 @@ def flat_grads(params: dict) -> dict:
     return g
 
 
-def step_reference(model, params, inst, batch, loss_type):
+def step_reference(model, params, inst, batch, loss_type, sparsity_weight=0.0):
     """Oracle loss and gradients of one step (Recovery.py:253-316) in FP64."""
     table = oracle_table(model)
     geom = oracle_geom(model, inst.tlY, inst.tlX)
     stack = torch.from_numpy(np.asarray(inst.stack)).to(torch.float64)
-    loss, _, _ = orc.step_loss(params, table, geom, stack, batch, loss_type)
+    loss, _, _ = orc.step_loss(params, table, geom, stack, batch, loss_type, sparsity_weight=sparsity_weight)
     loss.backward()
-    return float(loss), flat_grads(params)
+    return float(loss.detach()), flat_grads(params)
 
 
 def rel_err(a, b) -> float:

@@ -266,6 +266,75 @@ def test_fused_step_matches_oracle(pkg, dev, loss_type):
     assert rel_err(eng.flat.g(model.k_probe), g["k_probe"]) < tol_p
 
 
+@pytest.mark.parametrize("shape", [(2, 2), (6, 10), (64, 96), (1024, 1024), (3, 8, 12)])
+def test_haar_sparsity_loss_and_grad(pkg, dev, shape):
+    """HIP Haar sparsity (loss and adjoint) against the oracle in FP64: the
+    loss to 1e-6 relative, the gradient (sums of band signs) to 1e-6 relative
+    in norm (a band within f32 rounding of 0 may take the other sign)."""
+    rng = np.random.default_rng(sum(shape))
+    z = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    for reduction in ("sum", "mean"):
+        x = torch.from_numpy(z).to(dev).requires_grad_(True)
+        L = pkg.haar_wavelet_sparsity_loss(x, reduction)
+        (2.5 * L).backward()
+        x64 = torch.from_numpy(z).to(torch.complex128).requires_grad_(True)
+        Lr = orc.haar_wavelet_sparsity_loss(x64, reduction)
+        (2.5 * Lr).backward()
+        assert abs(L.item() - Lr.item()) < 1e-6 * abs(Lr.item()), (reduction, L.item(), Lr.item())
+        assert rel_err(x.grad, x64.grad) < 1e-6, reduction
+    # a real image: real gradient, imaginary part ignored
+    r = torch.from_numpy(z.real.copy()).to(dev).requires_grad_(True)
+    pkg.haar_wavelet_sparsity_loss(r).backward()
+    r64 = torch.from_numpy(z.real.astype(np.float64)).requires_grad_(True)
+    orc.haar_wavelet_sparsity_loss(r64).backward()
+    assert r.grad.dtype == torch.float32 and rel_err(r.grad, r64.grad) < 1e-6
+
+
+def test_haar_sparsity_rejects_odd_sizes(pkg, dev):
+    with pytest.raises(RuntimeError, match="odd"):
+        pkg.haar_wavelet_sparsity_loss(torch.zeros(6, 7, dtype=torch.complex64, device=dev))
+
+
+def test_fused_step_with_sparsity_matches_oracle(pkg, dev):
+    """sparsity_weight > 0 (Recovery.py:309-313): the step's objective is the
+    batch loss plus the weighted Haar term of the object.  The object carries
+    a textured object_initial (use_residual) so that its Haar bands are far
+    from 0 and the band signs agree between FP32 and FP64; loss and every
+    gradient against the oracle at the fused-step bars, and the canvas
+    gradient equals the far-field one plus the weighted Haar adjoint."""
+    inst = pkg.synthetic.make_named("small", seed=2)
+    M = inst.probe.shape[0]
+    rng = np.random.default_rng(11)
+    tex = (rng.standard_normal((inst.canvas, inst.canvas)) + 1j * rng.standard_normal((inst.canvas, inst.canvas)))
+    torch.manual_seed(0)
+    model = pkg.ComplexINRModel2D(inst.canvas, inst.canvas, 2, probe_width=M, probe_height=M, log2_hashmap_size=12,
+                                  hidden_omega_0=1.0, trainable_omega0=True, use_residual=True,
+                                  object_initial=torch.from_numpy(tex.astype(np.complex64)),
+                                  probe_initial=torch.from_numpy(inst.probe.astype(np.complex64))).to(dev)
+    params = oracle_params(model)
+    stack = torch.from_numpy(inst.stack).to(dev)
+    bidx = torch.tensor([3, 8, 14, 21, 30], dtype=torch.int32, device=dev)
+    plain = pkg.FusedReconstruction(model, stack, inst.tlY, inst.tlX, batch_size=5)
+    data0 = plain.step(bidx, update=False).sum().item()
+    dobj0 = plain.dobj.clone()
+    w = 0.5
+    eng = pkg.FusedReconstruction(model, stack, inst.tlY, inst.tlX, batch_size=5, sparsity_weight=w)
+    data = eng.step(bidx, update=False).sum().item()
+    assert data == data0
+    total = data + w * eng.sparsity.item()
+    assert abs(total - data) > 0.1 * abs(total), "the Haar term must be visible at this weight"
+    # canvas gradient = far-field gradient + w * Haar adjoint (same FP32 object)
+    o = eng.obj.detach().clone().requires_grad_(True)
+    pkg.haar_wavelet_sparsity_loss(o).backward()
+    assert rel_err(eng.dobj - dobj0, w * o.grad) < 1e-6
+    ref_total, g = step_reference(model, params, inst, bidx.tolist(), "smooth_L1_loss", sparsity_weight=w)
+    assert abs(total - ref_total) < 1e-4 * abs(ref_total), (total, ref_total)
+    assert rel_err(eng.flat.g(model.xy_hash_encoder.params), g["obj_table"]) < 2e-3
+    for a, b in zip(model.object_complex_mlp.tensors(), g["obj_mlp"]):
+        assert rel_err(eng.flat.g(a), b) < 2e-3, ("obj", a.shape, rel_err(eng.flat.g(a), b))
+    assert rel_err(eng.flat.g(model.k_object), g["k_object"]) < 2e-3
+
+
 def test_fused_step_fp16_tables(pkg, dev):
     """Fused step with FP16 table storage against the oracle at the
     half-rounded tables (FP32 bars of the fused-step test)."""

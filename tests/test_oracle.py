@@ -408,6 +408,79 @@ def test_batch_loss_central_difference_on_object():
                 assert abs(fd - g.item()) < 1e-5 * max(1.0, abs(fd)), (lt, y, x, part)
 
 
+# ------------------------------------------------------------------ Haar sparsity
+def _haar_loops(img):
+    """Per-block loops over a real 2-D array, written from the sub-band
+    definitions of utils/loss.py:74-81 (a top-left, b top-right, c
+    bottom-left, d bottom-right)."""
+    tot = 0.0
+    for r in range(img.shape[0] // 2):
+        for s in range(img.shape[1] // 2):
+            a, b = img[2 * r, 2 * s], img[2 * r, 2 * s + 1]
+            c, d = img[2 * r + 1, 2 * s], img[2 * r + 1, 2 * s + 1]
+            tot += abs(a - b + c - d) + abs(a + b - c - d) + abs(a - b - c + d)
+    return tot
+
+
+def test_haar_matches_block_loops_and_closed_forms():
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal((6, 8)) + 1j * rng.standard_normal((6, 8))
+    got = orc.haar_wavelet_sparsity_loss(torch.from_numpy(z)).item()
+    assert got == pytest.approx(_haar_loops(z.real) + _haar_loops(z.imag), rel=1e-12)
+    # constant image: no detail; impulse at a: |lh| = |hl| = |hh| = 1
+    assert orc.haar_wavelet_sparsity_loss(torch.ones(4, 4, dtype=torch.float64)).item() == 0.0
+    imp = torch.zeros(4, 4, dtype=torch.float64)
+    imp[2, 2] = 1.0
+    assert orc.haar_wavelet_sparsity_loss(imp).item() == 3.0
+    # the impulse at d instead: same magnitudes (signs -,-,+)
+    imp = torch.zeros(4, 4, dtype=torch.float64)
+    imp[1, 3] = 2.0
+    assert orc.haar_wavelet_sparsity_loss(imp).item() == 6.0
+    # checkerboard (+1 on even parity): lh = hl = 0, hh = 4 per block
+    cb = torch.from_numpy(np.indices((6, 4)).sum(0) % 2 == 0).to(torch.float64) * 2 - 1
+    assert orc.haar_wavelet_sparsity_loss(cb).item() == 4.0 * 6
+    # imaginary part counts like the real part
+    assert orc.haar_wavelet_sparsity_loss(1j * cb.to(torch.complex128)).item() == 4.0 * 6
+    # 'mean' divides each band's sum by the number of blocks
+    x = torch.from_numpy(z)
+    assert orc.haar_wavelet_sparsity_loss(x, "mean").item() == pytest.approx(got / 12, rel=1e-12)
+    # a leading batch axis stacks images
+    xb = torch.stack([x, 2 * x])
+    assert orc.haar_wavelet_sparsity_loss(xb).item() == pytest.approx(3 * got, rel=1e-12)
+
+
+def test_haar_bands_are_orthogonal():
+    """(ll, lh, hl, hh) / 2 is an orthonormal transform of (a, b, c, d): with
+    ll = a + b + c + d the energies agree, which fails for a wrong sign in any
+    detail band."""
+    rng = np.random.default_rng(6)
+    img = torch.from_numpy(rng.standard_normal((8, 10)))
+    a, b, c, d = img[0::2, 0::2], img[0::2, 1::2], img[1::2, 0::2], img[1::2, 1::2]
+    ll = a + b + c + d
+    lh, hl, hh = a - b + c - d, a + b - c - d, a - b - c + d
+    e_bands = (ll ** 2 + lh ** 2 + hl ** 2 + hh ** 2).sum() / 4
+    assert e_bands.item() == pytest.approx((img ** 2).sum().item(), rel=1e-12)
+    # and the oracle's loss is the L1 of exactly these three detail bands
+    l1 = (lh.abs().sum() + hl.abs().sum() + hh.abs().sum()).item()
+    assert orc.haar_wavelet_sparsity_loss(img).item() == pytest.approx(l1, rel=1e-12)
+
+
+def test_haar_gradient_central_differences():
+    rng = np.random.default_rng(7)
+    z = torch.from_numpy(rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))).requires_grad_(True)
+    orc.haar_wavelet_sparsity_loss(z).backward()
+    eps = 1e-6
+    for (y, x) in [(0, 0), (1, 4), (5, 5), (3, 2)]:
+        for unit in (1.0, 1j):
+            zp = z.detach().clone()
+            zp[y, x] += eps * unit
+            zm = z.detach().clone()
+            zm[y, x] -= eps * unit
+            fd = (orc.haar_wavelet_sparsity_loss(zp) - orc.haar_wavelet_sparsity_loss(zm)).item() / (2 * eps)
+            g = z.grad[y, x].real if unit == 1.0 else z.grad[y, x].imag
+            assert abs(fd - g.item()) < 1e-6
+
+
 # ------------------------------------------------------------------ geometry
 def test_scan_positions_round_half_even():
     dx = 0.675e-6 * 0.0295 / (64 * 2.96e-5)

@@ -51,7 +51,6 @@ def test_load_config_flattens_and_coerces(pkg, tmp_path):
 @pytest.mark.parametrize("edit, message", [
     (("initProbe", "initRandom"), "init_type"),
     (("FD_loss", "l2_loss"), "loss_type"),
-    (("sparsity_weight: 0.0", "sparsity_weight: 0.5"), "sparsity"),
     (("use_residual: true", "use_residual: true\n  model_type: gaussian_field"), "gaussian_field"),
     (("use_residual: true", "use_residual: true\n  table_precision: fp8"), "table_precision"),
     (("use_residual: true", "use_residual: true\n  mlp_precision: fp4"), "mlp_precision"),
@@ -59,6 +58,11 @@ def test_load_config_flattens_and_coerces(pkg, tmp_path):
 def test_load_config_rejects(pkg, tmp_path, edit, message):
     with pytest.raises(ValueError, match=message):
         pkg.recovery.load_config(_write(tmp_path, CONFIG.replace(*edit)))
+
+
+def test_load_config_keeps_sparsity_weight(pkg, tmp_path):
+    cfg = pkg.recovery.load_config(_write(tmp_path, CONFIG.replace("sparsity_weight: 0.0", "sparsity_weight: 5e-3")))
+    assert cfg["sparsity_weight"] == pytest.approx(5e-3) and isinstance(cfg["sparsity_weight"], float)
 
 
 def test_bundle_loader_keys_squeeze_and_gap(pkg, tmp_path):
