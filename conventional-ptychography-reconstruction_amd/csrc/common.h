@@ -13,6 +13,8 @@ namespace pty {
 // ---------------------------------------------------------------- errors
 int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
+// compute units of the current device (256 on MI355X; 256 if the query fails)
+int device_cu_count();
 
 #define PTY_REQUIRE(cond, code, ...) \
   do { if (!(cond)) return ::pty::set_error((code), __VA_ARGS__); } while (0)

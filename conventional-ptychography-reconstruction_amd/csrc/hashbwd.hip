@@ -451,36 +451,59 @@ __global__ __launch_bounds__(256) void k_csr_fill(PtyGrid g, PlanLayout L, PlanP
   });
 }
 
-// Hashed-level table gradient: one thread per entry sums its CSR terms
-// w(fx_j, dx) * w(fy_i, dy) * g[l][i n0 + j] in plan order.
+// Hashed-level table gradient over a chunk of 256 consecutive entries of one
+// level.  The chunk's CSR terms are one contiguous range; the workgroup walks
+// it in windows of kHashWin terms: every thread forms the product
+// w(fx_j, dx) * w(fy_i, dy) * g[l][i n0 + j] of one term (coalesced term
+// loads, neighbouring entries read neighbouring cells of the same rows) into
+// LDS, then each entry's thread adds its own terms from LDS in plan order.
+// The sum order per entry is fixed, so the result is deterministic.
+constexpr int kHashWin = 2048;
+
 __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                       const float2* __restrict__ gfeat,
                                                       float2* __restrict__ dtable) {
+  __shared__ int s_off[257];
+  __shared__ float2 s_val[kHashWin];
   int l;
   uint32_t chunk;
-  if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;
-  const uint32_t e = chunk * 256 + threadIdx.x;
-  if (e >= g.size[l]) return;
-  const int* off = P.csr_o + L.off_co[l];
+  if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;   // uniform across the workgroup
+  const int tid = threadIdx.x;
+  const uint32_t e0 = chunk * 256;
+  const int ne = (int)min(256u, g.size[l] - e0);
+  const int* off = P.csr_o + L.off_co[l] + e0;
   const uint32_t* terms = P.csr_p + L.off_cp[l];
   const float* fxl = P.fx + (int64_t)l * L.n0;
   const float* fyl = P.fy + (int64_t)l * L.n1;
   const float2* gl = gfeat + (int64_t)l * pts.n;
   const int cap = 4 * L.n0 * L.n1;
-  const int k1 = min(off[e + 1], cap);
+  for (int t = tid; t <= ne; t += 256) s_off[t] = min(off[t], cap);
+  __syncthreads();
+  const int k0 = s_off[0], k1 = s_off[ne];
+  const int my0 = tid < ne ? s_off[tid] : 0, my1 = tid < ne ? s_off[tid + 1] : 0;
   float ax = 0.f, ay = 0.f;
-  for (int k = off[e]; k < k1; ++k) {
-    const uint32_t c = terms[k];
-    const int i = (int)(c >> 17), j = (int)((c >> 2) & 0x7FFFu);
-    const float fx = fxl[j], fy = fyl[i];
-    const float wx = (c & 1u) ? fx : 1.f - fx;
-    const float w = wx * ((c & 2u) ? fy : 1.f - fy);
-    const float2 v = gl[(int64_t)i * L.n0 + j];
-    ax += w * v.x;
-    ay += w * v.y;
+  for (int w0 = k0; w0 < k1; w0 += kHashWin) {
+    const int wn = min(kHashWin, k1 - w0);
+    for (int t = tid; t < wn; t += 256) {
+      const uint32_t c = terms[w0 + t];
+      const int i = (int)(c >> 17), j = (int)((c >> 2) & 0x7FFFu);
+      const float fx = fxl[j], fy = fyl[i];
+      const float wx = (c & 1u) ? fx : 1.f - fx;
+      const float w = wx * ((c & 2u) ? fy : 1.f - fy);
+      const float2 v = gl[(int64_t)i * L.n0 + j];
+      s_val[t] = make_float2(w * v.x, w * v.y);
+    }
+    __syncthreads();
+    const int a = max(my0, w0), b = min(my1, w0 + wn);
+    for (int k = a; k < b; ++k) {
+      const float2 v = s_val[k - w0];
+      ax += v.x;
+      ay += v.y;
+    }
+    __syncthreads();
   }
-  if (ax != 0.f || ay != 0.f) {
-    const uint32_t t = g.offset[l] + e;
+  if (tid < ne && (ax != 0.f || ay != 0.f)) {
+    const uint32_t t = g.offset[l] + e0 + tid;
     float2 d = dtable[t];
     d.x += ax;
     d.y += ay;

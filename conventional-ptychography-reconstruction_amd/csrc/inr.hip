@@ -429,10 +429,14 @@ static int validate_points(const PtyPoints* p) {
   return PTY_OK;
 }
 
+// One workgroup per CU (the backward kernels hold one wave per SIMD): every
+// wave walks its tiles in one round, and the per-workgroup set-up and
+// gradient flush are paid once per CU.
 static int bwd_blocks(int64_t n) {
   const int64_t tiles = (n + 31) / 32;
   int64_t b = (tiles + kWaves - 1) / kWaves;
-  if (b > 512) b = 512;
+  const int cus = device_cu_count();
+  if (b > cus) b = cus;
   if (b < 1) b = 1;
   return (int)b;
 }
