@@ -21,10 +21,19 @@
 // rule layer by layer (PyTorch's complex-gradient convention g = dL/dRe +
 // i dL/dIm):
 //   g_z_H = conj(wo) g_o,  g_wo += g_o conj(z_H),  g_bo += g_o
-//   g_w = g_z conj(z),  g_v = -i w g_w,  g_w_k += Im(g_w conj(v))
-//   g_W_real += g_v x^T (MFMA with points as K; operands staged through LDS,
-//   o-major with a padded stride), g_c += sum g_v, g_x = W_real^T g_v (MFMA)
-//   layer 0: g_u0 = w0 Im(g_w0), g_w0 += u0 Im(g_w0), g_W0, g_b0, g_h = W0^T g_u0
+//   g_w = g_z conj(z) and the pre-activation gradient g_v = -i w g_w
+//   P += g_w x^H (MFMA with points as K; operands staged through LDS,
+//   o-major with a padded stride), q += sum g_w,
+//   g_x = A^H g_v = -i w (A^H g_w) (MFMA, then the scalar factor)
+//   layer 0: g_u0 = w0 Im(g_z conj(z0)); P0 += Im(.) h^T, q0 += Im(.),
+//   g_h = w0 W0^T Im(.)
+// The reduce stage turns the totals into the parameter gradients,
+// dA = -i w P, dc = -i w q, dW0 = w0 P0, db0 = w0 q0, and forms the
+// frequency gradients from the homogeneity of each layer in its weights
+// (v = A x + c, so sum g_w conj(v) = <conj(A), P> + <conj(c), q>):
+//   dw = Im(sum conj(A) P + sum conj(c) q),  dw0 = sum W0 P0 + sum b0 q0,
+// which equals sum Im(g_w conj(v)) over the points without keeping the
+// pre-activations and holds for w = 0 as well.
 //   feature gradients g_h are written level-major for the table kernels
 //   (hashbwd.hip)
 #include "inr_common.h"
@@ -153,11 +162,8 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
     }
   }
   float acc_c[H][2];
-  float acc_om[H + 1];
 #pragma unroll
   for (int k = 0; k < H; ++k) acc_c[k][0] = acc_c[k][1] = 0.f;
-#pragma unroll
-  for (int k = 0; k <= H; ++k) acc_om[k] = 0.f;
   float acc_b0 = 0.f, acc_wo = 0.f, acc_bor = 0.f, acc_boi = 0.f;
 
   const int64_t ntiles = (pts.n + 31) / 32;
@@ -169,16 +175,17 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
     const bool valid = p < pts.n;
     float x0 = 0.5f, x1 = 0.5f;
     if (valid) point_coords(pts, p, x0, x1);
-    // ---------------- recompute forward
+    // ---------------- recompute forward (pre-activations are not kept)
     float feat[16];
     encode_lane_batched(g, x0, x1, h, table, feat);
 #pragma unroll
     for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * WS + j] = feat[v];
-    const v16f u0 = layer0<H>(s, feat, j, h);
     float zr[16], zi[16];
+    {
+      const v16f u0 = layer0<H>(s, feat, j, h);
 #pragma unroll
-    for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
-    v16f vr[H], vi[H];
+      for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
+    }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
 #pragma unroll
@@ -186,8 +193,9 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
         S.X[k][crow(v, h) * WS + j] = zr[v];
         S.X[k][(32 + crow(v, h)) * WS + j] = zi[v];
       }
-      hidden_fwd<H>(s, k, zr, zi, j, h, vr[k], vi[k]);
-      gabor_act(s.om[k + 1], vr[k], vi[k], zr, zi);
+      v16f vr, vi;
+      hidden_fwd<H>(s, k, zr, zi, j, h, vr, vi);
+      gabor_act(s.om[k + 1], vr, vi, zr, zi);
     }
     // ---------------- output layer
     float2 go = make_float2(0.f, 0.f);
@@ -215,24 +223,19 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
 #pragma unroll
     for (int k = H - 1; k >= 0; --k) {
       const float w = s.om[k + 1];
-      float gvr[16], gvi[16];
-      float om_acc = 0.f;
+      float gwr[16], gwi[16];
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         // g_w = g_z * conj(z)
-        const float gwr = gzr[v] * zr[v] + gzi[v] * zi[v];
-        const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];
-        om_acc += gwi * vr[k][v] - gwr * vi[k][v];
-        gvr[v] = w * gwi;
-        gvi[v] = -w * gwr;
+        gwr[v] = gzr[v] * zr[v] + gzi[v] * zi[v];
+        gwi[v] = gzi[v] * zr[v] - gzr[v] * zi[v];
         const int u = crow(v, h);
-        S.G[u * WS + j] = gvr[v];
-        S.G[(32 + u) * WS + j] = gvi[v];
+        S.G[u * WS + j] = gwr[v];
+        S.G[(32 + u) * WS + j] = gwi[v];
       }
-      acc_om[k + 1] += om_acc;
       wave_sync();
-      // dW_real += G X^T over the 32 points (K = points); X[k] holds the
-      // input of hidden layer k.
+      // P += G X^H over the 32 points (K = points); X[k] holds the input of
+      // hidden layer k.
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int col = 2 * q + h;
@@ -248,44 +251,41 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
         acc_c[k][0] += a0;
         acc_c[k][1] += a1;
       }
-      // g_x = W_real^T g_v; also fetch the layer input z (C layout) from LDS
-      v16f gxr, gxi;
+      // t = A^H g_w, g_x = -i w t; also fetch the layer input z (C layout)
+      // from LDS
+      v16f txr, txi;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) { gxr[v] = 0.f; gxi[v] = 0.f; }
+      for (int v = 0; v < 16; ++v) { txr[v] = 0.f; txi[v] = 0.f; }
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int u = crow(q, h);
-        gxr = mfma(s.ar[k][u * WS + j], gvr[q], gxr);
-        gxi = mfma(s.nai[k][u * WS + j], gvr[q], gxi);
+        txr = mfma(s.ar[k][u * WS + j], gwr[q], txr);
+        txi = mfma(s.nai[k][u * WS + j], gwr[q], txi);
       }
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int u = crow(q, h);
-        gxr = mfma(s.ai[k][u * WS + j], gvi[q], gxr);
-        gxi = mfma(s.ar[k][u * WS + j], gvi[q], gxi);
+        txr = mfma(s.ai[k][u * WS + j], gwi[q], txr);
+        txi = mfma(s.ar[k][u * WS + j], gwi[q], txi);
       }
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        gzr[v] = gxr[v];
-        gzi[v] = gxi[v];
+        gzr[v] = w * txi[v];
+        gzi[v] = -w * txr[v];
         zr[v] = S.X[k][crow(v, h) * WS + j];
         zi[v] = S.X[k][(32 + crow(v, h)) * WS + j];
       }
       wave_sync();
     }
-    // ---------------- layer 0 (real): z0 = exp(i w0 u0)
+    // ---------------- layer 0 (real): z0 = exp(i w0 u0), g_u0 = w0 Im(g_z conj(z0))
     {
       const float w = s.om[0];
       float gu[16];
-      float om_acc = 0.f;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];  // Im(g_z conj(z0))
-        om_acc += u0[v] * gwi;
-        gu[v] = w * gwi;
+        gu[v] = gzi[v] * zr[v] - gzr[v] * zi[v];  // Im(g_z conj(z0)) = g_u0 / w0
         S.G[crow(v, h) * WS + j] = gu[v];
       }
-      acc_om[0] += om_acc;
       wave_sync();
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
@@ -301,12 +301,13 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
 #pragma unroll
       for (int q = 0; q < 16; ++q) gh = mfma(s.w0[crow(q, h) * WS + fj], gu[q], gh);
       wave_sync();
-      // feature gradients, level-major [L][n] (consumed by the table kernels)
+      // feature gradients g_h = w0 W0^T Im(.), level-major [L][n] (consumed by
+      // the table kernels)
       if (valid) {
 #pragma unroll
         for (int mm = 0; mm < 8; ++mm) {
           const int l = 2 * mm + h;
-          if (l < g.n_levels) gfeat[(int64_t)l * pts.n + p] = make_float2(gh[2 * mm], gh[2 * mm + 1]);
+          if (l < g.n_levels) gfeat[(int64_t)l * pts.n + p] = make_float2(w * gh[2 * mm], w * gh[2 * mm + 1]);
         }
       }
     }
@@ -348,11 +349,6 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const
       if (h == 0) red[GL::b0 + j] += b0s;
       // acc_wo: lane l holds row l (l < 32: Re of unit l, else Im of unit l-32)
       red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
-#pragma unroll
-      for (int k = 0; k <= H; ++k) {
-        const float t = wave_sum(acc_om[k]);
-        if (lane == 0) red[GL::om + k] += t;
-      }
       const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
       if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
     }
@@ -378,29 +374,73 @@ __global__ void k_inr_grad_reduce1(const float* __restrict__ partial, int nblock
   partial2[(int64_t)c * total + t] = acc;
 }
 
+// Stage 2: totals t of the partials (P0, q0, P_k, q_k, g_wo, g_bo) into the
+// parameter gradients, plus each element's term of the frequency gradient
+// of its layer (om_terms, summed by k_inr_omega):
+//   w0 / b0 : += w0 t;        term W0 t / b0 t
+//   A_k / c_k (complex pairs (re, im)): += -i w_k t, i.e. (w Im t, -w Re t);
+//             term of the real slot -Im(W) Re(t), of the imaginary slot Re(W) Im(t)
+//   wo / bo : += t
 template <int H>
-__global__ void k_inr_grad_reduce(const float* __restrict__ partial2, PtyMlp gm) {
+__global__ void k_inr_grad_reduce(const float* __restrict__ partial2, PtyMlp m, PtyMlp gm,
+                                  float* __restrict__ om_terms) {
   using GL = GradLayout<H>;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= GL::total) return;
-  float acc = 0.f;
-  for (int c = 0; c < kReduceChunks; ++c) acc += partial2[(int64_t)c * GL::total + t];
+  auto total_of = [&](int e) {
+    float acc = 0.f;
+    for (int c = 0; c < kReduceChunks; ++c) acc += partial2[(int64_t)c * GL::total + e];
+    return acc;
+  };
+  const float acc = total_of(t);
+  float term = 0.f;
   if (t < GL::b0) {
     const int o = t >> 5, f = t & 31;
-    if (f < gm.n_in) gm.w0[o * gm.n_in + f] += acc;
+    if (f < gm.n_in) {
+      gm.w0[o * gm.n_in + f] += *m.omega[0] * acc;
+      term = m.w0[o * m.n_in + f] * acc;
+    }
   } else if (t < GL::A(0)) {
-    gm.b0[t - GL::b0] += acc;
+    gm.b0[t - GL::b0] += *m.omega[0] * acc;
+    term = m.b0[t - GL::b0] * acc;
   } else if (t < GL::om) {
     const int k = (t - GL::A(0)) / 2112, r = (t - GL::A(0)) % 2112;
-    if (r < 2048) gm.wh[k][r] += acc;
-    else gm.bh[k][r - 2048] += acc;
+    const float w = *m.omega[k + 1];
+    const float other = total_of(t ^ 1);          // A(k) and c(k) start at even offsets
+    const bool re = (r & 1) == 0;
+    const float tr = re ? acc : other, ti = re ? other : acc;
+    const float* wsrc = r < 2048 ? m.wh[k] + (r & ~1) : m.bh[k] + ((r - 2048) & ~1);
+    float* gdst = r < 2048 ? gm.wh[k] + r : gm.bh[k] + (r - 2048);
+    *gdst += re ? w * ti : -w * tr;
+    term = re ? -wsrc[1] * tr : wsrc[0] * ti;
   } else if (t < GL::wo) {
-    *gm.omega[t - GL::om] += acc;
+    // frequency slots: formed by k_inr_omega
   } else if (t < GL::bo) {
     gm.wo[t - GL::wo] += acc;
   } else {
     gm.bo[t - GL::bo] += acc;
   }
+  om_terms[t] = term;
+}
+
+// Frequency gradients: block k sums the terms of layer k (layer 0: [0, A(0));
+// hidden layer k-1: [A(k-1), A(k-1) + 2112)) in a fixed order.
+template <int H>
+__global__ __launch_bounds__(256) void k_inr_omega(const float* __restrict__ om_terms, PtyMlp gm) {
+  using GL = GradLayout<H>;
+  const int k = blockIdx.x;
+  const int lo = k == 0 ? 0 : GL::A(k - 1);
+  const int hi = k == 0 ? GL::A(0) : GL::A(k - 1) + 2112;
+  __shared__ float red[256];
+  float acc = 0.f;
+  for (int e = lo + (int)threadIdx.x; e < hi; e += 256) acc += om_terms[e];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *gm.omega[k] += red[0];
 }
 
 static int validate_mlp(const PtyMlp* m, bool need_all) {
@@ -481,8 +521,10 @@ static size_t grad_total(const PtyMlp* mlp) {
   return (mlp->n_hidden == 1) ? GradLayout<1>::total : GradLayout<2>::total;
 }
 
+// per-block partials, the chunk sums of the first reduce stage and the
+// frequency-gradient terms
 static size_t partial_bytes(const PtyMlp* mlp, const PtyPoints* pts) {
-  return align256(sizeof(float) * grad_total(mlp) * (size_t)(bwd_blocks(pts->n) + kReduceChunks));
+  return align256(sizeof(float) * grad_total(mlp) * (size_t)(bwd_blocks(pts->n) + kReduceChunks + 1));
 }
 
 extern "C" size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts) {
@@ -509,6 +551,7 @@ static int check_bwd_args(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* 
 struct BwdWork {
   float* partial;
   float* partial2;
+  float* om_terms;
   float2* gfeat;
   char* tab_work;
   int blocks, total;
@@ -521,6 +564,7 @@ static BwdWork bwd_work(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pt
   b.total = (int)grad_total(mlp);
   b.partial = (float*)w;
   b.partial2 = b.partial + (size_t)b.blocks * b.total;
+  b.om_terms = b.partial2 + (size_t)kReduceChunks * b.total;
   b.gfeat = (float2*)(w + partial_bytes(mlp, pts));
   b.tab_work = (char*)b.gfeat + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n);
   return b;
@@ -556,12 +600,15 @@ extern "C" int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const Pty
   const BwdWork b = bwd_work(g, mlp, pts, work);
   hipLaunchKernelGGL(k_inr_grad_reduce1, dim3((b.total + 255) / 256, kReduceChunks), dim3(256), 0, stream,
                      (const float*)b.partial, b.blocks, b.total, b.partial2);
-  if (mlp->n_hidden == 1)
+  if (mlp->n_hidden == 1) {
     hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((b.total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)b.partial2, *grad);
-  else
+                       (const float*)b.partial2, *mlp, *grad, b.om_terms);
+    hipLaunchKernelGGL(k_inr_omega<1>, dim3(2), dim3(256), 0, stream, (const float*)b.om_terms, *grad);
+  } else {
     hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((b.total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)b.partial2, *grad);
+                       (const float*)b.partial2, *mlp, *grad, b.om_terms);
+    hipLaunchKernelGGL(k_inr_omega<2>, dim3(3), dim3(256), 0, stream, (const float*)b.om_terms, *grad);
+  }
   rc = check_launch("inr_bwd_finish");
   if (rc) return rc;
   return launch_table_bwd(*g, *pts, b.gfeat, dtable, b.tab_work, stream);

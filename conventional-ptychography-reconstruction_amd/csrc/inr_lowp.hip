@@ -125,8 +125,10 @@ __device__ __forceinline__ float* amax_slot(float* state, int slot) {
 }
 constexpr int kSlotH = 0;                                   // layer-0 input (features)
 __device__ __forceinline__ int slot_z(int k) { return 1 + k; }      // input of hidden layer k
-constexpr int kSlotGu0 = 3;                                 // layer-0 pre-activation gradient
-__device__ __forceinline__ int slot_gv(int k) { return 4 + k; }     // hidden-layer pre-activation gradient
+// layer-0 gradient Im(g_z conj(z0)) (the pre-activation gradient / w0)
+constexpr int kSlotGu0 = 3;
+// hidden layer k: g_w = g_z conj(z) (the pre-activation gradient / (-i w))
+__device__ __forceinline__ int slot_gv(int k) { return 4 + k; }
 
 __device__ __forceinline__ float amax8(const float* x) {
   float m = 0.f;
@@ -466,12 +468,11 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
     }
   }
   float acc_c[H][2];
-  float acc_om[H + 1];
   float mg[H + 1];
 #pragma unroll
   for (int k = 0; k < H; ++k) acc_c[k][0] = acc_c[k][1] = 0.f;
 #pragma unroll
-  for (int k = 0; k <= H; ++k) acc_om[k] = mg[k] = 0.f;
+  for (int k = 0; k <= H; ++k) mg[k] = 0.f;
   float acc_b0 = 0.f, acc_wo = 0.f, acc_bor = 0.f, acc_boi = 0.f;
 
   const int64_t ntiles = (pts.n + 31) / 32;
@@ -482,7 +483,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
     float x0 = 0.5f, x1 = 0.5f;
     if (valid) point_coords(pts, p, x0, x1);
     // ---------------- recompute forward; keep only the packed inputs of
-    // every layer (the pre-activations are recomputed where needed)
+    // every layer (the pre-activations are not needed by the backward)
     op_t pf[2];
     {
       float feat[16];
@@ -534,29 +535,23 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
       acc_wo += t;
     }
     wave_sync();
-    // ---------------- hidden layers, last to first
+    // ---------------- hidden layers, last to first (the staged gradient is
+    // g_w = g_z conj(z); the factor -i w of g_v is applied to g_x here and
+    // to the weight gradients by the reduce stage, which also forms dw)
 #pragma unroll
     for (int k = H - 1; k >= 0; --k) {
       const float w = s.om[k + 1];
       const float sg = s.xs[slot_gv(k)];
-      // pre-activation of layer k, recomputed from its packed input
-      v16f vr, vi;
-      hidden_lp<F, H>(o, s, k, px[k], lane, h, vr, vi);
-      float gvr[16], gvi[16];
-      float om_acc = 0.f;
+      float gwr[16], gwi[16];
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const float gwr = gzr[v] * zr[v] + gzi[v] * zi[v];
-        const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];
-        om_acc += gwi * vr[v] - gwr * vi[v];
-        gvr[v] = w * gwi;
-        gvi[v] = -w * gwr;
-        if (F::kScaled) mg[1 + k] = fmaxf(mg[1 + k], fmaxf(fabsf(gvr[v]), fabsf(gvi[v])));
+        gwr[v] = gzr[v] * zr[v] + gzi[v] * zi[v];
+        gwi[v] = gzi[v] * zr[v] - gzr[v] * zi[v];
+        if (F::kScaled) mg[1 + k] = fmaxf(mg[1 + k], fmaxf(fabsf(gwr[v]), fabsf(gwi[v])));
         const int u = crow(v, h);
-        SG[u * BS + j] = F::el(gvr[v], sg);
-        SG[(32 + u) * BS + j] = F::el(gvi[v], sg);
+        SG[u * BS + j] = F::el(gwr[v], sg);
+        SG[(32 + u) * BS + j] = F::el(gwi[v], sg);
       }
-      acc_om[k + 1] += om_acc;
       wave_sync();
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
@@ -570,41 +565,37 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
         acc_c[k][0] += F::sum8(a0);
         acc_c[k][1] += F::sum8(a1);
       }
-      v16f gxr, gxi;
+      // t = A^H g_w (MFMA), g_x = -i w t
+      v16f txr, txi;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) { gxr[v] = 0.f; gxi[v] = 0.f; }
-      const op_t bq[4] = {F::pack8(gvr, sg), F::pack8(gvr + 8, sg), F::pack8(gvi, sg), F::pack8(gvi + 8, sg)};
+      for (int v = 0; v < 16; ++v) { txr[v] = 0.f; txi[v] = 0.f; }
+      const op_t bq[4] = {F::pack8(gwr, sg), F::pack8(gwr + 8, sg), F::pack8(gwi, sg), F::pack8(gwi + 8, sg)};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        gxr = F::mfma(ot.at[k][0][q][lane], bq[q], gxr);
-        gxi = F::mfma(ot.at[k][1][q][lane], bq[q], gxi);
+        txr = F::mfma(ot.at[k][0][q][lane], bq[q], txr);
+        txi = F::mfma(ot.at[k][1][q][lane], bq[q], txi);
       }
-      const float ginv = F::kScaled ? 1.f / (s.ws[1 + k] * sg) : 1.f;
+      const float ginv = (F::kScaled ? 1.f / (s.ws[1 + k] * sg) : 1.f) * w;
       const float zinv = F::kScaled ? 1.f / s.xs[slot_z(k)] : 1.f;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        gzr[v] = gxr[v] * ginv;
-        gzi[v] = gxi[v] * ginv;
+        gzr[v] = txi[v] * ginv;
+        gzi[v] = -txr[v] * ginv;
         zr[v] = F::val(px[k][v >> 3], v & 7) * zinv;
         zi[v] = F::val(px[k][2 + (v >> 3)], v & 7) * zinv;
       }
       wave_sync();
     }
-    // ---------------- layer 0 (real)
+    // ---------------- layer 0 (real): staged Im(g_z conj(z0)) = g_u0 / w0
     {
       const float w = s.om[0];
-      const v16f u0 = layer0_lp<F, H>(o, s, pf, lane, h);
       float gu[16];
-      float om_acc = 0.f;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const float gwi = gzi[v] * zr[v] - gzr[v] * zi[v];
-        om_acc += u0[v] * gwi;
-        gu[v] = w * gwi;
+        gu[v] = gzi[v] * zr[v] - gzr[v] * zi[v];
         if (F::kScaled) mg[0] = fmaxf(mg[0], fabsf(gu[v]));
         SG[crow(v, h) * BS + j] = F::el(gu[v], sgu);
       }
-      acc_om[0] += om_acc;
       wave_sync();
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
@@ -618,7 +609,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
       for (int v = 0; v < 16; ++v) gh[v] = 0.f;
       gh = F::mfma(ot.w0t[0][lane], F::pack8(gu, sgu), gh);
       gh = F::mfma(ot.w0t[1][lane], F::pack8(gu + 8, sgu), gh);
-      const float hinv = F::kScaled ? 1.f / (s.ws[0] * sgu) : 1.f;
+      const float hinv = (F::kScaled ? 1.f / (s.ws[0] * sgu) : 1.f) * w;
       wave_sync();
       if (valid) {
 #pragma unroll
@@ -678,11 +669,6 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
       const float b0s = acc_b0 + __shfl_xor(acc_b0, 32);
       if (h == 0) red[GL::b0 + j] += b0s * b0i;
       red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
-#pragma unroll
-      for (int k = 0; k <= H; ++k) {
-        const float t = wave_sum(acc_om[k]);
-        if (lane == 0) red[GL::om + k] += t;
-      }
       const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
       if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
     }

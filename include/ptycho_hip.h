@@ -98,8 +98,10 @@ typedef struct PtyMlp {
    * floats of delayed per-tensor scaling state, [0, 8) the scales in use
    * (224 / amax of the previous step, one binade below the e4m3 maximum 448;
    * 1 before any), then the amax recorded
-   * by this step's kernels (features, hidden inputs, pre-activation
-   * gradients; 64 sub-slots per tensor, zero-initialised).  The forward and
+   * by this step's kernels (features, hidden inputs, and the backward's
+   * activation-side gradients g_z conj(z), i.e. the pre-activation gradients
+   * without their -i*omega factor; 64 sub-slots per tensor,
+   * zero-initialised).  The forward and
    * backward of one step read the same scales; pty_fp8_roll moves the
    * recorded amax into the scales between steps. */
   float* fp8_state;
