@@ -469,9 +469,10 @@ static int validate_points(const PtyPoints* p) {
   return PTY_OK;
 }
 
-// One workgroup per CU (the backward kernels hold one wave per SIMD): every
-// wave walks its tiles in one round, and the per-workgroup set-up and
-// gradient flush are paid once per CU.
+// One workgroup per CU (a backward workgroup fills the CU's LDS): every wave
+// walks its tiles in one round, and the per-workgroup set-up and gradient
+// flush are paid once per CU.  A workgroup covers 128 points per round in
+// every variant (4 waves x 32 points, or 8 waves x 16 points in FP32).
 static int bwd_blocks(int64_t n) {
   const int64_t tiles = (n + 31) / 32;
   int64_t b = (tiles + kWaves - 1) / kWaves;
@@ -503,15 +504,10 @@ extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* tab
               g->n_levels * g->n_features);
   PTY_REQUIRE(table && out, PTY_EINVAL, "inr_fwd: null pointer");
   if (pts->n == 0) return PTY_OK;
-  const int blocks = fwd_blocks(pts->n);
   if (mlp->precision != PTY_PREC_FP32)
-    launch_inr_fwd_lowp(*g, *mlp, (const float2*)table, *pts, (float2*)out, blocks, stream);
-  else if (mlp->n_hidden == 1)
-    hipLaunchKernelGGL(k_inr_fwd<1>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, *pts,
-                       (float2*)out);
+    launch_inr_fwd_lowp(*g, *mlp, (const float2*)table, *pts, (float2*)out, fwd_blocks(pts->n), stream);
   else
-    hipLaunchKernelGGL(k_inr_fwd<2>, dim3(blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table, *pts,
-                       (float2*)out);
+    launch_inr_fwd_f32(*g, *mlp, (const float2*)table, *pts, (float2*)out, stream);
   return check_launch("inr_fwd");
 }
 
@@ -580,12 +576,9 @@ extern "C" int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float*
   if (mlp->precision != PTY_PREC_FP32)
     launch_inr_bwd_lowp(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
                         stream);
-  else if (mlp->n_hidden == 1)
-    hipLaunchKernelGGL(k_inr_bwd<1>, dim3(b.blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table,
-                       b.gfeat, *pts, (const float2*)dout, b.partial);
   else
-    hipLaunchKernelGGL(k_inr_bwd<2>, dim3(b.blocks), dim3(kThreads), 0, stream, *g, *mlp, (const float2*)table,
-                       b.gfeat, *pts, (const float2*)dout, b.partial);
+    launch_inr_bwd_f32(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
+                       stream);
   return check_launch("inr_bwd_mlp");
 }
 

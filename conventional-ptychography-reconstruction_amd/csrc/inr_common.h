@@ -246,6 +246,12 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// FP32 launchers (inr_f32.hip, 16-point wave tiles)
+void launch_inr_fwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
+                        hipStream_t stream);
+void launch_inr_bwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
+                        const float2* dout, float* partial, int blocks, hipStream_t stream);
+
 // BF16 / FP8 launchers (inr_lowp.hip)
 void launch_inr_fwd_lowp(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
                          int blocks, hipStream_t stream);

@@ -1,0 +1,555 @@
+// FP32 fused hash-grid encode + complex-exponential MLP, forward and backward,
+// on v_mfma_f32_16x16x4f32 with 16-point wave tiles (gfx950).
+//
+// Reference: ComplexINRModel2D.forward -> HashGrid (models/complex_inr.py:279,
+// 294), ComplexMLP.net (models/complex_inr.py:57-78, 313):
+//   u0 = W0 h + b0            z0 = exp(i w0 u0)                 (real first layer)
+//   v_k = A_k z_{k-1} + c_k   z_k = exp(i w_k v_k) = e^{-w Im v}(cos w Re v, sin w Re v)
+//   o = wo . z_H + bo
+//
+// Tile layout.  A wavefront owns 16 points; lane l = (j, g) = (l & 15, l >> 4)
+// holds point j.  Every layer is computed transposed (Y^T = W X^T) with the
+// points as the MFMA N dimension.  The 16x16 C layout gives lane (j, g) rows
+// 4g + v (v < 4) of column j, so with two 16-row M blocks a lane holds the 8
+// units unit16(s, g) = 16 (s >> 2) + 4 g + (s & 3), s < 8.  Taking K chunk s
+// of the next layer to be {unit16(s, g) : g < 4} makes that register set the
+// next layer's B operand: activations stay in registers through the forward.
+// Complex layers use the real form [[Ar, -Ai], [Ai, Ar]].  The first layer's
+// K chunk c holds features feat16(c, g) = 2 (4 (c >> 1) + g) + (c & 1): each
+// lane interpolates 4 levels (4 m + g, m < 4) of its point.
+//
+// Against 32-point tiles on the 32x32x2 instruction the MFMA work per point
+// is the same, but a wave holds half the activations and half the LDS
+// staging, so the backward fits two waves per SIMD (eight per workgroup) and
+// one wave's MFMA chains overlap the other's transcendental and memory work.
+//
+// Backward (PyTorch's complex-gradient convention g = dL/dRe + i dL/dIm),
+// per tile after recomputing the forward:
+//   g_z_H = conj(wo) g_o,  g_wo += g_o conj(z_H),  g_bo += g_o
+//   g_w = g_z conj(z);  P += g_w x^H (points as K), q += sum g_w;
+//   g_x = -i w A^H g_w
+//   layer 0: Im(g_z conj(z0)) = g_u0 / w0;  P0 += Im(.) h^T, q0 += Im(.);
+//   g_h = w0 W0^T Im(.), written level-major for the table-gradient kernels.
+// The reduce stage of inr.hip applies -i w (w0) to P, q (P0, q0) and forms the
+// frequency gradients from the totals.
+#include "inr_common.h"
+
+namespace pty {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kW16 = 8;            // waves per backward workgroup (two per SIMD)
+constexpr int kFwd16Waves = 4;     // waves per forward workgroup
+constexpr int WS36 = 36;           // LDS row stride of the weight matrices (floats)
+constexpr int PS = 16;             // LDS row stride of the per-point staging (16 points)
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int unit16(int s, int g) { return 16 * (s >> 2) + 4 * g + (s & 3); }
+__device__ __forceinline__ int feat16(int c, int g) { return 2 * (4 * (c >> 1) + g) + (c & 1); }
+
+template <int H>
+struct Lds16 {
+  float w0[32 * WS36];     // W0[o][f], f zero-padded to 32
+  float ar[H][32 * WS36];  // Re A_k[o][i]
+  float ai[H][32 * WS36];  // Im A_k[o][i]
+  float b0[32];
+  float cr[H][32], ci[H][32];
+  float om[H + 1];
+  float wor[32], woi[32];
+  float bo[2];
+};
+
+template <int H>
+__device__ void load16(Lds16<H>& s, const PtyMlp& m, int tid, int nt) {
+  for (int t = tid; t < 32 * 32; t += nt) {
+    const int o = t >> 5, f = t & 31;
+    s.w0[o * WS36 + f] = f < m.n_in ? m.w0[o * m.n_in + f] : 0.f;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const float2 a = ((const float2*)m.wh[k])[t];
+      s.ar[k][o * WS36 + f] = a.x;
+      s.ai[k][o * WS36 + f] = a.y;
+    }
+  }
+  for (int t = tid; t < 32; t += nt) {
+    s.b0[t] = m.b0[t];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      s.cr[k][t] = m.bh[k][2 * t];
+      s.ci[k][t] = m.bh[k][2 * t + 1];
+    }
+    s.wor[t] = m.wo[2 * t];
+    s.woi[t] = m.wo[2 * t + 1];
+  }
+  if (tid <= H) s.om[tid] = *m.omega[tid];
+  if (tid == 0) { s.bo[0] = m.bo[0]; s.bo[1] = m.bo[1]; }
+}
+
+// Level constants for lane group g at level 4 m + g: the four candidates are
+// uniform kernel-argument values, the lane selects one.
+__device__ __forceinline__ LevelC level_c4(const PtyGrid& gr, int m, int g) {
+  const int l0 = 4 * m;
+  float sc[4];
+  uint32_t rs[4], sz[4], of[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    sc[q] = __uint_as_float(uniform_u32(__float_as_uint(gr.scale[l0 + q])));
+    rs[q] = uniform_u32(gr.res[l0 + q]);
+    sz[q] = uniform_u32(gr.size[l0 + q]);
+    of[q] = uniform_u32(gr.offset[l0 + q]);
+  }
+  const uint32_t dm = uniform_u32(gr.dense_mask);
+  LevelC c;
+  c.scale = g == 0 ? sc[0] : g == 1 ? sc[1] : g == 2 ? sc[2] : sc[3];
+  c.res = g == 0 ? rs[0] : g == 1 ? rs[1] : g == 2 ? rs[2] : rs[3];
+  c.size = g == 0 ? sz[0] : g == 1 ? sz[1] : g == 2 ? sz[2] : sz[3];
+  c.offset = g == 0 ? of[0] : g == 1 ? of[1] : g == 2 ? of[2] : of[3];
+  c.dense = (dm >> (l0 + g)) & 1u;
+  return c;
+}
+
+// The lane's 8 features feat[c] = feature feat16(c, g): the 16 corner indices
+// are formed first and the 16 gathers issued back to back.
+__device__ __forceinline__ void encode16(const PtyGrid& gr, float x0, float x1, int g,
+                                         const float2* __restrict__ table, float feat[8]) {
+  uint32_t idx[4][4];
+  float fx[4], fy[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int l = 4 * m + g;
+    if (l < gr.n_levels) {
+      const LevelC c = level_c4(gr, m, g);
+      const LevelPos p = level_pos_c(c, x0, x1);
+      corner_indices_c(c, p, idx[m]);
+      fx[m] = p.fx;
+      fy[m] = p.fy;
+    } else {
+      idx[m][0] = idx[m][1] = idx[m][2] = idx[m][3] = 0u;   // in bounds; the result is discarded
+      fx[m] = fy[m] = 0.f;
+    }
+  }
+  float2 t[4][4];
+  if (gr.table_precision == PTY_TABLE_F16) {
+    const __half2* t16 = reinterpret_cast<const __half2*>(table);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[m][c] = __half22float2(t16[idx[m][c]]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[m][c] = table[idx[m][c]];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    float2 acc = make_float2(0.f, 0.f);
+    if (4 * m + g < gr.n_levels) {
+      LevelPos p;
+      p.fx = fx[m];
+      p.fy = fy[m];
+      float w[4];
+      corner_weights(p, w);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        acc.x += w[c] * t[m][c].x;
+        acc.y += w[c] * t[m][c].y;
+      }
+    }
+    feat[2 * m] = acc.x;
+    feat[2 * m + 1] = acc.y;
+  }
+}
+
+// u[mb] (units 16 mb + 4 g + v) = b0 + W0 h
+template <int H>
+__device__ __forceinline__ void layer0_16(const Lds16<H>& s, const float feat[8], int i, int g, v4f u[2]) {
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) u[mb][v] = s.b0[16 * mb + 4 * g + v];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) u[mb] = mfma4(s.w0[(16 * mb + i) * WS36 + feat16(c, g)], feat[c], u[mb]);
+}
+
+// (vr, vi)[mb] = A_k z + c_k
+template <int H>
+__device__ __forceinline__ void hidden16(const Lds16<H>& s, int k, const float zr[8], const float zi[8], int i, int g,
+                                         v4f vr[2], v4f vi[2]) {
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      vr[mb][v] = s.cr[k][16 * mb + 4 * g + v];
+      vi[mb][v] = s.ci[k][16 * mb + 4 * g + v];
+    }
+#pragma unroll
+  for (int st = 0; st < 8; ++st) {
+    const int u = unit16(st, g);
+    const float nzi = -zi[st];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      const float a_r = s.ar[k][(16 * mb + i) * WS36 + u];
+      const float a_i = s.ai[k][(16 * mb + i) * WS36 + u];
+      vr[mb] = mfma4(a_r, zr[st], vr[mb]);
+      vi[mb] = mfma4(a_i, zr[st], vi[mb]);
+      vr[mb] = mfma4(a_i, nzi, vr[mb]);
+      vi[mb] = mfma4(a_r, zi[st], vi[mb]);
+    }
+  }
+}
+
+__device__ __forceinline__ void act16(float w, const v4f vr[2], const v4f vi[2], float zr[8], float zi[8]) {
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    float c, sn;
+    cis(w * vr[s >> 2][s & 3], c, sn);
+    const float mag = exp_f(-w * vi[s >> 2][s & 3]);
+    zr[s] = mag * c;
+    zi[s] = mag * sn;
+  }
+}
+
+// ============================================================== forward
+template <int H>
+__global__ __launch_bounds__(64 * kFwd16Waves) void k_inr_fwd16(PtyGrid gr, PtyMlp m,
+                                                                const float2* __restrict__ table, PtyPoints pts,
+                                                                float2* __restrict__ out) {
+  __shared__ Lds16<H> s;
+  load16<H>(s, m, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int64_t ntiles = (pts.n + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * kFwd16Waves + wave; tile < ntiles;
+       tile += (int64_t)gridDim.x * kFwd16Waves) {
+    asm volatile("" ::: "memory");
+    const int64_t p = tile * 16 + j;
+    const bool valid = p < pts.n;
+    float x0 = 0.5f, x1 = 0.5f;
+    if (valid) point_coords(pts, p, x0, x1);
+    float feat[8];
+    encode16(gr, x0, x1, g, table, feat);
+    float zr[8], zi[8];
+    {
+      v4f u[2];
+      layer0_16<H>(s, feat, j, g, u);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) cis(s.om[0] * u[q >> 2][q & 3], zr[q], zi[q]);
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      v4f vr[2], vi[2];
+      hidden16<H>(s, k, zr, zi, j, g, vr, vi);
+      act16(s.om[k + 1], vr, vi, zr, zi);
+    }
+    float orr = 0.f, oi = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int u = unit16(q, g);
+      orr += s.wor[u] * zr[q] - s.woi[u] * zi[q];
+      oi += s.wor[u] * zi[q] + s.woi[u] * zr[q];
+    }
+    orr += __shfl_xor(orr, 16);
+    oi += __shfl_xor(oi, 16);
+    orr += __shfl_xor(orr, 32);
+    oi += __shfl_xor(oi, 32);
+    if (valid && g == 0) out[p] = make_float2(orr + s.bo[0], oi + s.bo[1]);
+  }
+}
+
+// ============================================================== backward
+template <int H>
+struct Stage16 {
+  float X[H][64 * PS];   // input of hidden layer k: rows 0..31 Re, 32..63 Im; column = point
+  float F[32 * PS];      // features, feature-major
+  float G[64 * PS];      // gradient staging
+};
+
+__device__ __forceinline__ float4 lds4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float sum4(const float4& a) { return (a.x + a.y) + (a.z + a.w); }
+
+// acc += A B over the 16 points: A rows (ra), B rows (rb) staged point-major;
+// MFMA kc contracts points {4 g' + kc : g' < 4}.
+__device__ __forceinline__ v4f mfma_pts(const float4& a, const float4& b, v4f acc) {
+  acc = mfma4(a.x, b.x, acc);
+  acc = mfma4(a.y, b.y, acc);
+  acc = mfma4(a.z, b.z, acc);
+  acc = mfma4(a.w, b.w, acc);
+  return acc;
+}
+__device__ __forceinline__ float4 neg4(const float4& a) { return make_float4(-a.x, -a.y, -a.z, -a.w); }
+
+template <int H>
+__global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, const float2* __restrict__ table,
+                                                         float2* __restrict__ gfeat, PtyPoints pts,
+                                                         const float2* __restrict__ dout,
+                                                         float* __restrict__ partial) {
+  __shared__ Lds16<H> s;
+  __shared__ Stage16<H> st[kW16];
+  load16<H>(s, m, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  Stage16<H>& S = st[wave];
+
+  // P_k[mb][nb] (re, im): rows 16 mb + 4 g + v (out units), column 16 nb + j (in units)
+  v4f Pr[H][2][2], Pi[H][2][2];
+  v4f P0[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      P0[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < H; ++k) Pr[k][a][b] = Pi[k][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  // bias-like sums: lane (i, g) holds unit 16 mb + i over points 4 g .. 4 g + 3
+  float qr[H][2], qi[H][2], q0[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    q0[a] = 0.f;
+#pragma unroll
+    for (int k = 0; k < H; ++k) qr[k][a] = qi[k][a] = 0.f;
+  }
+  float acc_wo = 0.f, acc_bor = 0.f, acc_boi = 0.f;
+
+  const int64_t ntiles = (pts.n + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * kW16 + wave; tile < ntiles; tile += (int64_t)gridDim.x * kW16) {
+    asm volatile("" ::: "memory");
+    const int64_t p = tile * 16 + j;
+    const bool valid = p < pts.n;
+    float x0 = 0.5f, x1 = 0.5f;
+    if (valid) point_coords(pts, p, x0, x1);
+    // ---------------- recompute forward (inputs of every layer staged)
+    float feat[8];
+    encode16(gr, x0, x1, g, table, feat);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) S.F[feat16(c, g) * PS + j] = feat[c];
+    float zr[8], zi[8];
+    {
+      v4f u[2];
+      layer0_16<H>(s, feat, j, g, u);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) cis(s.om[0] * u[q >> 2][q & 3], zr[q], zi[q]);
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        S.X[k][unit16(q, g) * PS + j] = zr[q];
+        S.X[k][(32 + unit16(q, g)) * PS + j] = zi[q];
+      }
+      v4f vr[2], vi[2];
+      hidden16<H>(s, k, zr, zi, j, g, vr, vi);
+      act16(s.om[k + 1], vr, vi, zr, zi);
+    }
+    // ---------------- output layer
+    float2 go = make_float2(0.f, 0.f);
+    if (valid) go = dout[p];
+    if (g == 0) { acc_bor += go.x; acc_boi += go.y; }
+    float gzr[8], gzi[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int u = unit16(q, g);
+      // g_wo[u] += g_o conj(z_u); g_z = conj(wo_u) g_o
+      S.G[u * PS + j] = go.x * zr[q] + go.y * zi[q];
+      S.G[(32 + u) * PS + j] = go.y * zr[q] - go.x * zi[q];
+      gzr[q] = s.wor[u] * go.x + s.woi[u] * go.y;
+      gzi[q] = s.wor[u] * go.y - s.woi[u] * go.x;
+    }
+    wave_sync();
+    // lane l sums staged row l (Re of unit l for l < 32, else Im of unit l - 32)
+    acc_wo += (sum4(lds4(&S.G[lane * PS])) + sum4(lds4(&S.G[lane * PS + 4]))) +
+              (sum4(lds4(&S.G[lane * PS + 8])) + sum4(lds4(&S.G[lane * PS + 12])));
+    wave_sync();
+    // ---------------- hidden layers, last to first
+#pragma unroll
+    for (int k = H - 1; k >= 0; --k) {
+      const float w = s.om[k + 1];
+      float gwr[8], gwi[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        gwr[q] = gzr[q] * zr[q] + gzi[q] * zi[q];
+        gwi[q] = gzi[q] * zr[q] - gzr[q] * zi[q];
+        const int u = unit16(q, g);
+        S.G[u * PS + j] = gwr[q];
+        S.G[(32 + u) * PS + j] = gwi[q];
+      }
+      wave_sync();
+      // P += G X^H: Pr = Gr Xr^T + Gi Xi^T, Pi = Gi Xr^T - Gr Xi^T
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        const float4 ar = lds4(&S.G[(16 * mb + j) * PS + 4 * g]);
+        const float4 ai = lds4(&S.G[(32 + 16 * mb + j) * PS + 4 * g]);
+        qr[k][mb] += sum4(ar);
+        qi[k][mb] += sum4(ai);
+        const float4 nar = neg4(ar);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+          const float4 xr = lds4(&S.X[k][(16 * nb + j) * PS + 4 * g]);
+          const float4 xi = lds4(&S.X[k][(32 + 16 * nb + j) * PS + 4 * g]);
+          Pr[k][mb][nb] = mfma_pts(ar, xr, Pr[k][mb][nb]);
+          Pr[k][mb][nb] = mfma_pts(ai, xi, Pr[k][mb][nb]);
+          Pi[k][mb][nb] = mfma_pts(ai, xr, Pi[k][mb][nb]);
+          Pi[k][mb][nb] = mfma_pts(nar, xi, Pi[k][mb][nb]);
+        }
+      }
+      // t = A^H g_w (in units 16 mb + 4 g + v), g_x = -i w t
+      v4f txr[2], txi[2];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) txr[mb] = txi[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int u = unit16(q, g);
+        const float ngwr = -gwr[q];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+          const float a_r = s.ar[k][u * WS36 + 16 * mb + j];
+          const float a_i = s.ai[k][u * WS36 + 16 * mb + j];
+          txr[mb] = mfma4(a_r, gwr[q], txr[mb]);
+          txr[mb] = mfma4(a_i, gwi[q], txr[mb]);
+          txi[mb] = mfma4(a_i, ngwr, txi[mb]);
+          txi[mb] = mfma4(a_r, gwi[q], txi[mb]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        gzr[q] = w * txi[q >> 2][q & 3];
+        gzi[q] = -w * txr[q >> 2][q & 3];
+        zr[q] = S.X[k][unit16(q, g) * PS + j];
+        zi[q] = S.X[k][(32 + unit16(q, g)) * PS + j];
+      }
+      wave_sync();
+    }
+    // ---------------- layer 0 (real): staged Im(g_z conj(z0)) = g_u0 / w0
+    {
+      const float w = s.om[0];
+      float gu[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        gu[q] = gzi[q] * zr[q] - gzr[q] * zi[q];
+        S.G[unit16(q, g) * PS + j] = gu[q];
+      }
+      wave_sync();
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        const float4 a = lds4(&S.G[(16 * mb + j) * PS + 4 * g]);
+        q0[mb] += sum4(a);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) P0[mb][nb] = mfma_pts(a, lds4(&S.F[(16 * nb + j) * PS + 4 * g]), P0[mb][nb]);
+      }
+      // g_h (features 16 fb + 4 g + v) = W0^T Im(.)
+      v4f gh[2];
+#pragma unroll
+      for (int fb = 0; fb < 2; ++fb) gh[fb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int u = unit16(q, g);
+#pragma unroll
+        for (int fb = 0; fb < 2; ++fb) gh[fb] = mfma4(s.w0[u * WS36 + 16 * fb + j], gu[q], gh[fb]);
+      }
+      wave_sync();
+      // feature 16 fb + 4 g + 2 h + e is feature e of level 8 fb + 2 g + h
+      if (valid) {
+#pragma unroll
+        for (int fb = 0; fb < 2; ++fb)
+#pragma unroll
+          for (int hh = 0; hh < 2; ++hh) {
+            const int l = 8 * fb + 2 * g + hh;
+            if (l < gr.n_levels)
+              gfeat[(int64_t)l * pts.n + p] = make_float2(w * gh[fb][2 * hh], w * gh[fb][2 * hh + 1]);
+          }
+      }
+    }
+  }
+
+  // ---------------- workgroup reduction into partial[blockIdx.x][GradLayout::total]
+  using GL = GradLayout<H>;
+  static_assert(sizeof(st) / sizeof(float) >= (size_t)GL::total, "reduction buffer does not fit");
+  __syncthreads();
+  float* red = &st[0].X[0][0];
+  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) red[t] = 0.f;
+  __syncthreads();
+  // bias sums over the four lane groups (same unit, different points)
+  float q0s[2], qrs[H][2], qis[H][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    q0s[a] = q0[a] + __shfl_xor(q0[a], 16);
+    q0s[a] += __shfl_xor(q0s[a], 32);
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      qrs[k][a] = qr[k][a] + __shfl_xor(qr[k][a], 16);
+      qrs[k][a] += __shfl_xor(qrs[k][a], 32);
+      qis[k][a] = qi[k][a] + __shfl_xor(qi[k][a], 16);
+      qis[k][a] += __shfl_xor(qis[k][a], 32);
+    }
+  }
+  const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
+  for (int w = 0; w < kW16; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int o = 16 * mb + 4 * g + v, in = 16 * nb + j;
+            red[GL::w0 + o * 32 + in] += P0[mb][nb][v];
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+              red[GL::A(k) + 2 * (o * 32 + in)] += Pr[k][mb][nb][v];
+              red[GL::A(k) + 2 * (o * 32 + in) + 1] += Pi[k][mb][nb][v];
+            }
+          }
+      if (g == 0) {
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+          red[GL::b0 + 16 * mb + j] += q0s[mb];
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            red[GL::c(k) + 2 * (16 * mb + j)] += qrs[k][mb];
+            red[GL::c(k) + 2 * (16 * mb + j) + 1] += qis[k][mb];
+          }
+        }
+      }
+      red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
+      if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
+}
+
+// ============================================================== launchers
+void launch_inr_fwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
+                        hipStream_t stream) {
+  const int64_t tiles = (pts.n + 15) / 16;
+  int64_t blocks = (tiles + kFwd16Waves - 1) / kFwd16Waves;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  if (m.n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_fwd16<1>, dim3((unsigned)blocks), dim3(64 * kFwd16Waves), 0, stream, g, m, table, pts,
+                       out);
+  else
+    hipLaunchKernelGGL(k_inr_fwd16<2>, dim3((unsigned)blocks), dim3(64 * kFwd16Waves), 0, stream, g, m, table, pts,
+                       out);
+}
+
+// `blocks` workgroups of 8 waves; each walks 16-point tiles
+void launch_inr_bwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
+                        const float2* dout, float* partial, int blocks, hipStream_t stream) {
+  if (m.n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_bwd16<1>, dim3(blocks), dim3(64 * kW16), 0, stream, g, m, table, gfeat, pts, dout,
+                       partial);
+  else
+    hipLaunchKernelGGL(k_inr_bwd16<2>, dim3(blocks), dim3(64 * kW16), 0, stream, g, m, table, gfeat, pts, dout,
+                       partial);
+}
+
+}  // namespace pty

@@ -64,12 +64,15 @@ def flat_grads(params: dict) -> dict:
     return g
 
 
-def step_reference(model, params, inst, batch, loss_type, sparsity_weight=0.0):
-    """Oracle loss and gradients of one step (Recovery.py:253-316) in FP64."""
+def step_reference(model, params, inst, batch, loss_type, sparsity_weight=0.0, dtype=torch.float64):
+    """Oracle loss and gradients of one step (Recovery.py:253-316), in FP64
+    (the parity reference) or, with dtype=float32 and FP32 parameter leaves,
+    the same autograd computation in FP32 (its rounding floor)."""
     table = oracle_table(model)
     geom = oracle_geom(model, inst.tlY, inst.tlX)
-    stack = torch.from_numpy(np.asarray(inst.stack)).to(torch.float64)
-    loss, _, _ = orc.step_loss(params, table, geom, stack, batch, loss_type, sparsity_weight=sparsity_weight)
+    stack = torch.from_numpy(np.asarray(inst.stack)).to(dtype)
+    loss, _, _ = orc.step_loss(params, table, geom, stack, batch, loss_type, dtype=dtype,
+                               sparsity_weight=sparsity_weight)
     loss.backward()
     return float(loss.detach()), flat_grads(params)
 

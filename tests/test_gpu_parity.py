@@ -256,14 +256,23 @@ def test_fused_step_matches_oracle(pkg, dev, loss_type):
     # The probe branch passes through the adjoint of the energy normalisation,
     # g_q = s g_n - (s/E) Re<q, g_n> q, which removes the component of g_n
     # along q.  The Poisson loss gradient is dominated by that (scale)
-    # component, so the FP32 result keeps fewer correct digits there.
+    # component, so an FP32 computation keeps fewer correct digits there.
+    # The bar of each probe tensor is the larger of 2e-3 (6e-3 Poisson) and
+    # three times the error of the oracle's own autograd run in FP32.
     tol_p = 6e-3 if loss_type == "Poisson_likelihood_loss" else 2e-3
-    assert rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"]) < tol_p
+    _, g32 = step_reference(model, oracle_params(model, torch.float32), inst, batch, loss_type,
+                            dtype=torch.float32)
+
+    def bar(ours32, ref64):
+        return max(tol_p, 3 * rel_err(ours32, ref64))
+
+    assert rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"]) < bar(g32["prb_table"],
+                                                                                      g["prb_table"])
     for a, b in zip(model.object_complex_mlp.tensors(), g["obj_mlp"]):
         assert rel_err(eng.flat.g(a), b) < 2e-3, ("obj", a.shape, rel_err(eng.flat.g(a), b))
-    for a, b in zip(model.probe_complex_mlp.tensors(), g["prb_mlp"]):
-        assert rel_err(eng.flat.g(a), b) < tol_p, ("prb", a.shape, rel_err(eng.flat.g(a), b))
-    assert rel_err(eng.flat.g(model.k_probe), g["k_probe"]) < tol_p
+    for a, b, c in zip(model.probe_complex_mlp.tensors(), g["prb_mlp"], g32["prb_mlp"]):
+        assert rel_err(eng.flat.g(a), b) < bar(c, b), ("prb", a.shape, rel_err(eng.flat.g(a), b), rel_err(c, b))
+    assert rel_err(eng.flat.g(model.k_probe), g["k_probe"]) < bar(g32["k_probe"], g["k_probe"])
 
 
 @pytest.mark.parametrize("shape", [(2, 2), (6, 10), (64, 96), (1024, 1024), (3, 8, 12)])
