@@ -1,87 +1,31 @@
-// Fused hash-grid encode + complex-exponential MLP on MFMA (gfx950).
+// Entry points of the fused hash-grid encode + complex-exponential MLP
+// (gfx950): argument checks, workspace layout, the two-stage reduction of the
+// per-workgroup gradient partials, and the MLP over given features.
 //
 // Reference: ComplexINRModel2D.forward -> HashGrid (models/complex_inr.py:279,
 // 294), ComplexMLP.net (models/complex_inr.py:57-78, 313):
 //   u0 = W0 h + b0            z0 = exp(i w0 u0)                 (real first layer)
 //   v_k = A_k z_{k-1} + c_k   z_k = exp(i w_k v_k) = e^{-w Im v}(cos w Re v, sin w Re v)
 //   o = wo . z_H + bo
+// The fused kernels are in inr_f32.hip (FP32, 16-point wave tiles on
+// v_mfma_f32_16x16x4f32) and inr_lowp.hip (BF16 / FP8, 32-point tiles).  The
+// MLP over given features (k_mlp_fwd) uses the 32-point FP32 layout of
+// inr_common.h.
 //
-// Layout.  A wavefront owns a tile of 32 points; lane l = (j = l&31, h = l>>5)
-// holds point j.  Every layer is computed transposed, Y^T = W X^T, with
-// v_mfma_f32_32x32x2f32: the points are the MFMA N dimension, so the output
-// of one layer (C layout: lane (j,h) holds rows crow(v,h) = 8(v>>2)+4h+(v&3),
-// v < 16, of column j) is already the B operand of the next layer once the
-// K index of the next weight matrix is permuted to match (kappa maps below).
-// Complex layers use the real 64x64 form [[Ar,-Ai],[Ai,Ar]] with two 32-row
-// M blocks (Re rows, Im rows), so each lane gets Re and Im of the same 16
-// units and applies exp(i w v) in registers.  No activation leaves registers
-// in the forward pass.
-//
-// The backward pass recomputes the forward for the tile, then runs the chain
-// rule layer by layer (PyTorch's complex-gradient convention g = dL/dRe +
-// i dL/dIm):
-//   g_z_H = conj(wo) g_o,  g_wo += g_o conj(z_H),  g_bo += g_o
-//   g_w = g_z conj(z) and the pre-activation gradient g_v = -i w g_w
-//   P += g_w x^H (MFMA with points as K; operands staged through LDS,
-//   o-major with a padded stride), q += sum g_w,
-//   g_x = A^H g_v = -i w (A^H g_w) (MFMA, then the scalar factor)
-//   layer 0: g_u0 = w0 Im(g_z conj(z0)); P0 += Im(.) h^T, q0 += Im(.),
-//   g_h = w0 W0^T Im(.)
-// The reduce stage turns the totals into the parameter gradients,
-// dA = -i w P, dc = -i w q, dW0 = w0 P0, db0 = w0 q0, and forms the
-// frequency gradients from the homogeneity of each layer in its weights
-// (v = A x + c, so sum g_w conj(v) = <conj(A), P> + <conj(c), q>):
+// The backward kernels accumulate, over the points, P = sum g_w x^H and
+// q = sum g_w per hidden layer (g_w = g_z conj(z), the pre-activation
+// gradient g_v = -i w g_w without its factor) and P0 = sum Im(g_z conj(z0)) h^T,
+// q0 = sum Im(g_z conj(z0)) for the first layer.  The reduce stage turns the
+// totals into the parameter gradients, dA = -i w P, dc = -i w q,
+// dW0 = w0 P0, db0 = w0 q0, and forms the frequency gradients from the
+// homogeneity of each layer in its weights (v = A x + c, so
+// sum g_w conj(v) = <conj(A), P> + <conj(c), q>):
 //   dw = Im(sum conj(A) P + sum conj(c) q),  dw0 = sum W0 P0 + sum b0 q0,
 // which equals sum Im(g_w conj(v)) over the points without keeping the
 // pre-activations and holds for w = 0 as well.
-//   feature gradients g_h are written level-major for the table kernels
-//   (hashbwd.hip)
 #include "inr_common.h"
 
 namespace pty {
-// ============================================================== forward
-template <int H>
-__global__ __launch_bounds__(kThreads) void k_inr_fwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
-                                                      PtyPoints pts, float2* __restrict__ out) {
-  __shared__ MlpLds<H> s;
-  load_weights<H>(s, m);
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = lane & 31, h = lane >> 5;
-  const int64_t ntiles = (pts.n + 31) / 32;
-  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
-    // weights are re-read from LDS at each use instead of being hoisted into
-    // ~270 registers for the whole loop (which forces one wave per SIMD)
-    asm volatile("" ::: "memory");
-    const int64_t p = tile * 32 + j;
-    const bool valid = p < pts.n;
-    float x0 = 0.5f, x1 = 0.5f;
-    if (valid) point_coords(pts, p, x0, x1);
-    float feat[16];
-    encode_lane(g, x0, x1, h, table, feat);
-    v16f u0 = layer0<H>(s, feat, j, h);
-    float zr[16], zi[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-      v16f vr, vi;
-      hidden_fwd<H>(s, k, zr, zi, j, h, vr, vi);
-      gabor_act(s.om[k + 1], vr, vi, zr, zi);
-    }
-    float orr = 0.f, oi = 0.f;
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int u = crow(v, h);
-      orr += s.wor[u] * zr[v] - s.woi[u] * zi[v];
-      oi += s.wor[u] * zi[v] + s.woi[u] * zr[v];
-    }
-    orr += __shfl_xor(orr, 32);
-    oi += __shfl_xor(oi, 32);
-    if (valid && h == 0) out[p] = make_float2(orr + s.bo[0], oi + s.bo[1]);
-  }
-}
-
 // ComplexMLP.net over precomputed features [n, n_in] (models/complex_inr.py:313).
 template <int H>
 __global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __restrict__ feats, int64_t n,
@@ -125,236 +69,6 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __r
     oi += __shfl_xor(oi, 32);
     if (valid && h == 0) out[p] = make_float2(orr + s.bo[0], oi + s.bo[1]);
   }
-}
-
-// ============================================================== backward
-template <int H>
-struct BwdStage {
-  float X[H][64 * WS];   // z_k (k < H) o-major: rows 0..31 Re, 32..63 Im, column = point
-  float F[32 * WS];      // features, feature-major
-  float G[64 * WS];      // gradient staging
-};
-
-template <int H>
-__global__ __launch_bounds__(kThreads) void k_inr_bwd(PtyGrid g, PtyMlp m, const float2* __restrict__ table,
-                                                      float2* __restrict__ gfeat, PtyPoints pts,
-                                                      const float2* __restrict__ dout, float* __restrict__ partial) {
-  __shared__ MlpLds<H> s;
-  __shared__ BwdStage<H> st[kWaves];
-  load_weights<H>(s, m);
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = lane & 31, h = lane >> 5;
-  BwdStage<H>& S = st[wave];
-
-  // dW of hidden layer k in complex form: dAr = G_re X_re^T + G_im X_im^T,
-  // dAi = G_im X_re^T - G_re X_im^T (the four real blocks share two
-  // accumulators).
-  v16f dWr[H], dWi[H];
-  v16f dW0;
-#pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    dW0[v] = 0.f;
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-      dWr[k][v] = 0.f;
-      dWi[k][v] = 0.f;
-    }
-  }
-  float acc_c[H][2];
-#pragma unroll
-  for (int k = 0; k < H; ++k) acc_c[k][0] = acc_c[k][1] = 0.f;
-  float acc_b0 = 0.f, acc_wo = 0.f, acc_bor = 0.f, acc_boi = 0.f;
-
-  const int64_t ntiles = (pts.n + 31) / 32;
-  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
-    // weights are re-read from LDS at each use instead of being hoisted into
-    // ~270 registers for the whole loop (which forces one wave per SIMD)
-    asm volatile("" ::: "memory");
-    const int64_t p = tile * 32 + j;
-    const bool valid = p < pts.n;
-    float x0 = 0.5f, x1 = 0.5f;
-    if (valid) point_coords(pts, p, x0, x1);
-    // ---------------- recompute forward (pre-activations are not kept)
-    float feat[16];
-    encode_lane_batched(g, x0, x1, h, table, feat);
-#pragma unroll
-    for (int v = 0; v < 16; ++v) S.F[kappa0(v, h) * WS + j] = feat[v];
-    float zr[16], zi[16];
-    {
-      const v16f u0 = layer0<H>(s, feat, j, h);
-#pragma unroll
-      for (int v = 0; v < 16; ++v) cis(s.om[0] * u0[v], zr[v], zi[v]);
-    }
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        S.X[k][crow(v, h) * WS + j] = zr[v];
-        S.X[k][(32 + crow(v, h)) * WS + j] = zi[v];
-      }
-      v16f vr, vi;
-      hidden_fwd<H>(s, k, zr, zi, j, h, vr, vi);
-      gabor_act(s.om[k + 1], vr, vi, zr, zi);
-    }
-    // ---------------- output layer
-    float2 go = make_float2(0.f, 0.f);
-    if (valid) go = dout[p];
-    if (h == 0) { acc_bor += go.x; acc_boi += go.y; }
-    float gzr[16], gzi[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int u = crow(v, h);
-      // g_wo[u] += g_o * conj(z_u); g_z = conj(wo_u) * g_o
-      S.G[u * WS + j] = go.x * zr[v] + go.y * zi[v];
-      S.G[(32 + u) * WS + j] = go.y * zr[v] - go.x * zi[v];
-      gzr[v] = s.wor[u] * go.x + s.woi[u] * go.y;
-      gzi[v] = s.wor[u] * go.y - s.woi[u] * go.x;
-    }
-    wave_sync();
-    {
-      float t = 0.f;
-#pragma unroll 8
-      for (int q = 0; q < 32; ++q) t += S.G[lane * WS + q];
-      acc_wo += t;
-    }
-    wave_sync();
-    // ---------------- hidden layers, last to first
-#pragma unroll
-    for (int k = H - 1; k >= 0; --k) {
-      const float w = s.om[k + 1];
-      float gwr[16], gwi[16];
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        // g_w = g_z * conj(z)
-        gwr[v] = gzr[v] * zr[v] + gzi[v] * zi[v];
-        gwi[v] = gzi[v] * zr[v] - gzr[v] * zi[v];
-        const int u = crow(v, h);
-        S.G[u * WS + j] = gwr[v];
-        S.G[(32 + u) * WS + j] = gwi[v];
-      }
-      wave_sync();
-      // P += G X^H over the 32 points (K = points); X[k] holds the input of
-      // hidden layer k.
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int col = 2 * q + h;
-        const float a0 = S.G[j * WS + col];
-        const float a1 = S.G[(32 + j) * WS + col];
-        const float* Xp = S.X[k];
-        const float b0v = Xp[j * WS + col];
-        const float b1v = Xp[(32 + j) * WS + col];
-        dWr[k] = mfma(a0, b0v, dWr[k]);
-        dWr[k] = mfma(a1, b1v, dWr[k]);
-        dWi[k] = mfma(a1, b0v, dWi[k]);
-        dWi[k] = mfma(-a0, b1v, dWi[k]);
-        acc_c[k][0] += a0;
-        acc_c[k][1] += a1;
-      }
-      // t = A^H g_w, g_x = -i w t; also fetch the layer input z (C layout)
-      // from LDS
-      v16f txr, txi;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) { txr[v] = 0.f; txi[v] = 0.f; }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int u = crow(q, h);
-        txr = mfma(s.ar[k][u * WS + j], gwr[q], txr);
-        txi = mfma(s.nai[k][u * WS + j], gwr[q], txi);
-      }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int u = crow(q, h);
-        txr = mfma(s.ai[k][u * WS + j], gwi[q], txr);
-        txi = mfma(s.ar[k][u * WS + j], gwi[q], txi);
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        gzr[v] = w * txi[v];
-        gzi[v] = -w * txr[v];
-        zr[v] = S.X[k][crow(v, h) * WS + j];
-        zi[v] = S.X[k][(32 + crow(v, h)) * WS + j];
-      }
-      wave_sync();
-    }
-    // ---------------- layer 0 (real): z0 = exp(i w0 u0), g_u0 = w0 Im(g_z conj(z0))
-    {
-      const float w = s.om[0];
-      float gu[16];
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        gu[v] = gzi[v] * zr[v] - gzr[v] * zi[v];  // Im(g_z conj(z0)) = g_u0 / w0
-        S.G[crow(v, h) * WS + j] = gu[v];
-      }
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int col = 2 * q + h;
-        const float a = S.G[j * WS + col];
-        dW0 = mfma(a, S.F[j * WS + col], dW0);
-        acc_b0 += a;
-      }
-      v16f gh;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) gh[v] = 0.f;
-      const int fj = phi0(j);
-#pragma unroll
-      for (int q = 0; q < 16; ++q) gh = mfma(s.w0[crow(q, h) * WS + fj], gu[q], gh);
-      wave_sync();
-      // feature gradients g_h = w0 W0^T Im(.), level-major [L][n] (consumed by
-      // the table kernels)
-      if (valid) {
-#pragma unroll
-        for (int mm = 0; mm < 8; ++mm) {
-          const int l = 2 * mm + h;
-          if (l < g.n_levels) gfeat[(int64_t)l * pts.n + p] = make_float2(w * gh[2 * mm], w * gh[2 * mm + 1]);
-        }
-      }
-    }
-  }
-
-  // ---------------- workgroup reduction into partial[blockIdx.x][GradLayout::total]
-  using GL = GradLayout<H>;
-  static_assert(sizeof(st) / (sizeof(float)) >= (size_t)GL::total, "reduction buffer does not fit");
-  __syncthreads();
-  float* red = &st[0].X[0][0];  // reuse the staging memory as one flat buffer
-  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) red[t] = 0.f;
-  __syncthreads();
-  for (int w = 0; w < kWaves; ++w) {
-    if (wave == w) {
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int o = crow(v, h);
-        red[GL::w0 + o * 32 + j] += dW0[v];
-#pragma unroll
-        for (int k = 0; k < H; ++k) {
-          // block (mb, nb): rows 32mb + o, cols 32nb + j
-          const float dar = dWr[k][v];
-          const float dai = dWi[k][v];
-          red[GL::A(k) + 2 * (o * 32 + j)] += dar;
-          red[GL::A(k) + 2 * (o * 32 + j) + 1] += dai;
-        }
-      }
-      // bias-like sums: lane (j, h) holds partial sums over its half of the points
-#pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const float cr = acc_c[k][0] + __shfl_xor(acc_c[k][0], 32);
-        const float ci = acc_c[k][1] + __shfl_xor(acc_c[k][1], 32);
-        if (h == 0) {
-          red[GL::c(k) + 2 * j] += cr;
-          red[GL::c(k) + 2 * j + 1] += ci;
-        }
-      }
-      const float b0s = acc_b0 + __shfl_xor(acc_b0, 32);
-      if (h == 0) red[GL::b0 + j] += b0s;
-      // acc_wo: lane l holds row l (l < 32: Re of unit l, else Im of unit l-32)
-      red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
-      const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
-      if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
-    }
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
 }
 
 // Two-stage fixed-order reduction of the workgroup partials:
