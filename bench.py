@@ -16,7 +16,7 @@ are epochs = ceil(positions/batch) steps.  value = iters/s = epochs per
 second.
 
 Also reported: the roofline of the dominant kernel (the MLP backward with
-recompute, k_inr_bwd, launched once for the object grid and once for the
+recompute, k_inr_bwd16 in FP32, launched once for the object grid and once for the
 probe grid per step; HIP events on its launch stream bracket each launch
 inside the timed region), its HBM traffic from the committed rocprofv3 PMC
 summary, and the CPU baseline (the FP32 oracle restatement on the host
@@ -49,7 +49,7 @@ WORKLOADS = {
 MLP_FLOP_BWD_PER_POINT = 37376      # dX + dW of the complex MLP per point (recompute not counted)
 MLP_FLOP_FWD_PER_POINT = 18688
 MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}     # MI355X dense
-MLP_BWD_KERNEL = {"fp32": "k_inr_bwd<2>", "bf16": "k_inr_bwd_lp<pty::FmtBf16, 2>",
+MLP_BWD_KERNEL = {"fp32": "k_inr_bwd16<2>", "bf16": "k_inr_bwd_lp<pty::FmtBf16, 2>",
                   "fp8": "k_inr_bwd_lp<pty::FmtFp8, 2>"}
 PROFILES = ROOT / "profiles"
 

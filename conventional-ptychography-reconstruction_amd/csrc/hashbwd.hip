@@ -265,6 +265,9 @@ __global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, PlanLayout L, Pt
     const int cell = cx - dx;
     if (cell < 0) continue;
     const int j1 = st[cell + 1];
+    // unrolled so that the loads of coarse cells (tens of points) overlap;
+    // the summation order is unchanged
+#pragma unroll 8
     for (int j = st[cell]; j < j1; ++j) {
       const float w = dx ? fxl[j] : 1.f - fxl[j];
       const float2 v = gl[j];
@@ -286,6 +289,7 @@ __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanP
     const int cell = cy - dy;
     if (cell < 0 || cell > res) continue;
     const int i1 = st[cell + 1];
+#pragma unroll 8
     for (int i = st[cell]; i < i1; ++i) {
       const float w = dy ? fyl[i] : 1.f - fyl[i];
       const float2 v = Al[(int64_t)i * (res + 1) + cx];
