@@ -612,8 +612,12 @@ def nrmse_mode(args):
         return pkg.metrics.aligned_nrmse(o.detach().cpu().numpy(), inst.obj, mask, max_shift=2)
 
     curve_plain = []
-    wall_plain = plain_reconstruct(make(), inst, args.epochs, check_every=args.check_every,
-                                   on_check=lambda e, t, o: curve_plain.append((e, t, err(o))))
+
+    def plain_check(e, t, o):
+        curve_plain.append((e, t, err(o)))
+        print(f"plain epoch {e}: {t:.1f} s, NRMSE {curve_plain[-1][2]:.4f}", file=sys.stderr, flush=True)
+
+    wall_plain = plain_reconstruct(make(), inst, args.epochs, check_every=args.check_every, on_check=plain_check)
     final_plain = curve_plain[-1][2]
     target = final_plain * 1.01
     model = make()
@@ -631,6 +635,7 @@ def nrmse_mode(args):
             e = err(o)
             t = time.perf_counter() - t0
             curve_fused.append((ep + 1, t, e))
+            print(f"fused epoch {ep + 1}: {t:.2f} s, NRMSE {e:.4f}", file=sys.stderr, flush=True)
             if hit is None and e <= target:
                 hit = (ep + 1, t)
     wall_fused_total = time.perf_counter() - t0
@@ -649,6 +654,113 @@ def nrmse_mode(args):
     print(json.dumps(out))
 
 
+
+# ----------------------------------------------------------------------------
+# kernel mode: the gather and FFT kernels on their own, algorithmic GB/s
+# ----------------------------------------------------------------------------
+HBM_PEAK_GBS = 8000.0
+
+
+def _time_launches(fn, reps=20, warmup=3):
+    """Mean milliseconds of fn() between HIP events on the current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / reps
+
+
+def kernels_mode(args):
+    """Standalone kernels at the canvas-3648 shapes: the hash-grid encoder
+    (pty_hashgrid_fwd, gathers of 4 corners x 16 levels per point), the
+    Fourier upsample 1824 -> 3648 and its adjoint (pty_upsample_fwd/adj), and
+    the far-field fused forward + adjoint for a batch of 64 probe-512
+    patterns.  Algorithmic bytes are the minimum the operation must move
+    (inputs read once, outputs written once); achieved = bytes / mean launch
+    time, against the 8 TB/s HBM peak (table gathers mostly hit L2 / MALL, so
+    the gather kernel's figure can exceed it)."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    _lib = pkg._lib
+    dev = torch.device("cuda:0")
+    rows = []
+
+    def row(name, shape, nbytes, ms, extra=None):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        r = {"kernel": name, "shape": shape, "algorithmic_bytes": nbytes, "ms": ms, "GB_per_s": gbs,
+             "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+        if extra:
+            r.update(extra)
+        rows.append(r)
+        print(f"{name:34s} {shape:40s} {ms * 1e3:9.1f} us {gbs:9.0f} GB/s", file=sys.stderr, flush=True)
+
+    # --- hash-grid encoder: n = 1824^2 points, log2T 20, fp32 and fp16 tables
+    n = 1824 * 1824
+    spec = pkg.level_table(16, 2, 20, 16, 1.5)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    coords = torch.rand(n, 2, device=dev, generator=g)
+    params = (torch.rand(spec.n_entries * 2, device=dev, generator=g) - 0.5) * 2e-4
+    out = torch.empty(n, 32, device=dev)
+    s = _lib.stream_ptr(dev)
+    for tp in ("fp32", "fp16"):
+        desc = spec.descriptor(tp)
+        tab = params if tp == "fp32" else params.to(torch.float16)
+        ms = _time_launches(lambda: _lib.call("pty_hashgrid_fwd", desc, coords.data_ptr(), n, tab.data_ptr(),
+                                              out.data_ptr(), s))
+        esz = 8 if tp == "fp32" else 4
+        nbytes = n * (8 + 16 * 4 * esz + 32 * 4)
+        row(f"k_hashgrid_fwd ({tp} table)", f"{n} points, 16 levels, log2T 20", nbytes, ms,
+            {"bytes_per_point": nbytes / n})
+    del coords, params, out
+
+    # --- Fourier upsample and adjoint: 1824^2 -> 3648^2 complex64
+    nsm, ds = 1824, 2
+    m = nsm * ds
+    small = torch.randn(nsm, nsm, dtype=torch.complex64, device=dev)
+    big = torch.empty(m, m, dtype=torch.complex64, device=dev)
+    ws = _lib.workspace(_lib.query("pty_upsample_workspace", nsm, ds), dev)
+    nbytes = 8 * (nsm * nsm + m * m)
+    ms = _time_launches(lambda: _lib.call("pty_upsample_fwd", small.data_ptr(), nsm, ds, big.data_ptr(),
+                                          ws.data_ptr(), ws.numel(), s))
+    row("upsample fwd (rows, T, rows, T)", f"{nsm}^2 -> {m}^2 c64", nbytes, ms)
+    ms = _time_launches(lambda: _lib.call("pty_upsample_adj", big.data_ptr(), nsm, ds, small.data_ptr(),
+                                          ws.data_ptr(), ws.numel(), s))
+    row("upsample adjoint", f"{m}^2 -> {nsm}^2 c64", nbytes, ms)
+    del small, big, ws
+
+    # --- far field fused forward + adjoint: 64 patterns of 512^2, canvas 3648
+    M, H, B, F = 512, 3648, 64, 128
+    obj = torch.randn(H, H, dtype=torch.complex64, device=dev)
+    prb = torch.randn(M, M, dtype=torch.complex64, device=dev)
+    stack = torch.rand(F, M, M, device=dev) * 10
+    tl = torch.randint(0, H - M, (F, 2), device=dev, dtype=torch.int32)
+    bidx = torch.arange(B, device=dev, dtype=torch.int32)
+    loss = torch.empty(B, device=dev)
+    dobj = torch.empty_like(obj)
+    dprb = torch.empty_like(prb)
+    for lt in ("smooth_L1_loss", "FD_loss"):
+        d = pkg.farfield.far_descriptor(M, M, H, H, B, F, lt)
+        ws = _lib.workspace(_lib.query("pty_farfield_workspace", d), dev)
+        ms = _time_launches(lambda: _lib.call("pty_farfield_fwd_bwd", d, obj.data_ptr(), prb.data_ptr(),
+                                              stack.data_ptr(), bidx.data_ptr(), tl.data_ptr(), loss.data_ptr(),
+                                              dobj.data_ptr(), dprb.data_ptr(), ws.data_ptr(), ws.numel(), s))
+        # per pattern pixel: window read 8 B + measurement 4 B; canvas gradient
+        # written once (8 B / canvas pixel); probe read + gradient 16 B / probe pixel
+        nbytes = B * M * M * 12 + H * H * 8 + M * M * 16
+        row(f"far field fwd+adjoint ({lt})", f"{B} x {M}^2 patterns, canvas {H}", nbytes, ms)
+    out = {"mode": "kernels", "device": torch.cuda.get_device_name(0), "hbm_peak_GBs": HBM_PEAK_GBS,
+           "rows": rows}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -660,7 +772,7 @@ def main():
     ap.add_argument("--table-precision", choices=["fp32", "fp16"], default="fp32",
                     help="hash-table storage the kernels gather from (f32 master kept for Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart"], default="throughput")
+    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart", "kernels"], default="throughput")
     ap.add_argument("--ws-train", type=int, default=200, help="warmstart: training instances")
     ap.add_argument("--ws-epochs", type=int, default=8, help="warmstart: net training epochs")
     ap.add_argument("--ws-heldout", type=int, default=3, help="warmstart: held-out seeds")
@@ -693,6 +805,9 @@ def main():
     if args.mode == "warmstart":
         args.workload = args.workload or "cfg1"
         warmstart_mode(args)
+        return
+    if args.mode == "kernels":
+        kernels_mode(args)
         return
     args.workload = args.workload or "cfg1"
     shapes, _, batch, loss_type, wl_name = WORKLOADS[args.workload]

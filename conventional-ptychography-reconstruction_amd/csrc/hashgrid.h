@@ -133,36 +133,6 @@ __device__ __forceinline__ void corner_indices(const PtyGrid& g, int l, const Le
   idx[3] = level_index(g, l, p.gx + 1u, p.gy + 1u);
 }
 
-// Interpolated (f0, f1) of one level.  `table` holds float2 entries, or
-// __half2 entries when the descriptor says PTY_TABLE_F16.
-__device__ __forceinline__ float2 level_feature(const PtyGrid& g, int l, float x0, float x1,
-                                                const float2* __restrict__ table) {
-  LevelPos p = level_pos(g, l, x0, x1);
-  uint32_t idx[4];
-  float w[4];
-  corner_indices(g, l, p, idx);
-  corner_weights(p, w);
-  float2 acc = make_float2(0.f, 0.f);
-  if (g.table_precision == PTY_TABLE_F16) {
-    // FP16 copy of the table (entry = one __half2); interpolation in f32
-    const __half2* t16 = reinterpret_cast<const __half2*>(table);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const float2 t = __half22float2(t16[idx[c]]);
-      acc.x += w[c] * t.x;
-      acc.y += w[c] * t.y;
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const float2 t = table[idx[c]];
-      acc.x += w[c] * t.x;
-      acc.y += w[c] * t.y;
-    }
-  }
-  return acc;
-}
-
 __device__ __forceinline__ void level_scatter(const PtyGrid& g, int l, float x0, float x1, float2 dy,
                                               float* __restrict__ dtable) {
   LevelPos p = level_pos(g, l, x0, x1);

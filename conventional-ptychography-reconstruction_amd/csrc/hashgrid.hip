@@ -4,24 +4,102 @@
 
 namespace pty {
 
+// Level constants of the descriptor staged in LDS, so that lanes on
+// different levels read their own without indexing the kernel arguments.
+struct LevelTab {
+  float scale[PTY_MAX_LEVELS];
+  uint32_t res[PTY_MAX_LEVELS], size[PTY_MAX_LEVELS], offset[PTY_MAX_LEVELS], dense[PTY_MAX_LEVELS];
+};
+
+__device__ __forceinline__ void stage_levels(const PtyGrid& g, LevelTab& t) {
+  const int l = threadIdx.x;
+  if (l < g.n_levels) {
+    t.scale[l] = g.scale[l];
+    t.res[l] = g.res[l];
+    t.size[l] = g.size[l];
+    t.offset[l] = g.offset[l];
+    t.dense[l] = (g.dense_mask >> l) & 1u;
+  }
+}
+
+__device__ __forceinline__ LevelC level_from(const LevelTab& t, int l) {
+  LevelC c;
+  c.scale = t.scale[l];
+  c.res = t.res[l];
+  c.size = t.size[l];
+  c.offset = t.offset[l];
+  c.dense = t.dense[l];
+  return c;
+}
+
+// One lane per (point, level): lane t of a workgroup handles level t % 16 of
+// point t / 16, so a point's output row [L x 2] is written by consecutive
+// lanes (coalesced) and each lane issues the 4 gathers of its level.
+constexpr int kPtsPerBlock = 256 / PTY_MAX_LEVELS;
+
 __global__ __launch_bounds__(256) void k_hashgrid_fwd(PtyGrid g, const float* __restrict__ coords,
                                                        int64_t n, const float2* __restrict__ table,
                                                        float2* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  __shared__ LevelTab lt;
+  stage_levels(g, lt);
+  __syncthreads();
+  const int l = threadIdx.x % PTY_MAX_LEVELS;
+  const int64_t i = (int64_t)blockIdx.x * kPtsPerBlock + threadIdx.x / PTY_MAX_LEVELS;
+  if (i >= n || l >= g.n_levels) return;
   const float x0 = coords[2 * i], x1 = coords[2 * i + 1];
-  float2* o = out + i * g.n_levels;
-  for (int l = 0; l < g.n_levels; ++l) o[l] = level_feature(g, l, x0, x1, table);
+  const LevelC c = level_from(lt, l);
+  const LevelPos p = level_pos_c(c, x0, x1);
+  uint32_t idx[4];
+  float w[4];
+  corner_indices_c(c, p, idx);
+  corner_weights(p, w);
+  float2 acc = make_float2(0.f, 0.f);
+  if (g.table_precision == PTY_TABLE_F16) {
+    const __half2* t16 = reinterpret_cast<const __half2*>(table);
+    float2 t[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] = __half22float2(t16[idx[k]]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc.x += w[k] * t[k].x;
+      acc.y += w[k] * t[k].y;
+    }
+  } else {
+    float2 t[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] = table[idx[k]];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc.x += w[k] * t[k].x;
+      acc.y += w[k] * t[k].y;
+    }
+  }
+  out[i * g.n_levels + l] = acc;
 }
 
 __global__ __launch_bounds__(256) void k_hashgrid_bwd(PtyGrid g, const float* __restrict__ coords,
                                                        int64_t n, const float2* __restrict__ dout,
                                                        float* __restrict__ dtable) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  __shared__ LevelTab lt;
+  stage_levels(g, lt);
+  __syncthreads();
+  const int l = threadIdx.x % PTY_MAX_LEVELS;
+  const int64_t i = (int64_t)blockIdx.x * kPtsPerBlock + threadIdx.x / PTY_MAX_LEVELS;
+  if (i >= n || l >= g.n_levels) return;
+  const float2 dy = dout[i * g.n_levels + l];
+  if (dy.x == 0.f && dy.y == 0.f) return;
   const float x0 = coords[2 * i], x1 = coords[2 * i + 1];
-  const float2* d = dout + i * g.n_levels;
-  for (int l = 0; l < g.n_levels; ++l) level_scatter(g, l, x0, x1, d[l], dtable);
+  const LevelC c = level_from(lt, l);
+  const LevelPos p = level_pos_c(c, x0, x1);
+  uint32_t idx[4];
+  float w[4];
+  corner_indices_c(c, p, idx);
+  corner_weights(p, w);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    atomic_add_f32(dtable + 2 * idx[k], w[k] * dy.x);
+    atomic_add_f32(dtable + 2 * idx[k] + 1, w[k] * dy.y);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_hashgrid_indices(PtyGrid g, const float* __restrict__ coords,
@@ -70,7 +148,8 @@ extern "C" int pty_hashgrid_fwd(const PtyGrid* g, const float* coords, int64_t n
   if (rc) return rc;
   PTY_REQUIRE(n >= 0 && (n == 0 || (coords && table && out)), PTY_EINVAL, "null pointer");
   if (n == 0) return PTY_OK;
-  const int64_t blocks = (n + 255) / 256;
+  const int64_t blocks = (n + kPtsPerBlock - 1) / kPtsPerBlock;
+  PTY_REQUIRE(blocks <= 0x7fffffff, PTY_EUNSUP, "hashgrid_fwd: %lld points exceed one launch", (long long)n);
   hipLaunchKernelGGL(k_hashgrid_fwd, dim3((unsigned)blocks), dim3(256), 0, stream, *g, coords, n,
                      (const float2*)table, (float2*)out);
   return check_launch("hashgrid_fwd");
@@ -82,7 +161,8 @@ extern "C" int pty_hashgrid_bwd(const PtyGrid* g, const float* coords, int64_t n
   if (rc) return rc;
   PTY_REQUIRE(n >= 0 && (n == 0 || (coords && dout && dtable)), PTY_EINVAL, "null pointer");
   if (n == 0) return PTY_OK;
-  const int64_t blocks = (n + 255) / 256;
+  const int64_t blocks = (n + kPtsPerBlock - 1) / kPtsPerBlock;
+  PTY_REQUIRE(blocks <= 0x7fffffff, PTY_EUNSUP, "hashgrid_bwd: %lld points exceed one launch", (long long)n);
   hipLaunchKernelGGL(k_hashgrid_bwd, dim3((unsigned)blocks), dim3(256), 0, stream, *g, coords, n,
                      (const float2*)dout, dtable);
   return check_launch("hashgrid_bwd");

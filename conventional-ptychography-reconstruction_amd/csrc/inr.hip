@@ -196,10 +196,14 @@ static int bwd_blocks(int64_t n) {
   return (int)b;
 }
 
+// Forward grids: up to four 4-wave workgroups per CU (the forward kernels
+// run three to four waves per SIMD), each wave walking several tiles, so the
+// weights are staged in LDS once per workgroup rather than once per tile.
 static int fwd_blocks(int64_t n) {
   const int64_t tiles = (n + 31) / 32;
   int64_t b = (tiles + kWaves - 1) / kWaves;
-  if (b > 4096) b = 4096;
+  const int64_t cap = 4 * (int64_t)device_cu_count();
+  if (b > cap) b = cap;
   if (b < 1) b = 1;
   return (int)b;
 }
