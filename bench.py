@@ -706,20 +706,25 @@ def kernels_mode(args):
     spec = pkg.level_table(16, 2, 20, 16, 1.5)
     g = torch.Generator(device=dev)
     g.manual_seed(0)
-    coords = torch.rand(n, 2, device=dev, generator=g)
+    # the INR's own points: the cell-centred 1824 x 1824 grid in raster order
+    # (init_scale_grids), and the same number of uniform random points
+    ax = (torch.arange(1824, device=dev, dtype=torch.float32) + 0.5) / 1824
+    grid = torch.stack([ax.repeat(1824), ax.repeat_interleave(1824)], 1).contiguous()
+    rnd = torch.rand(n, 2, device=dev, generator=g)
     params = (torch.rand(spec.n_entries * 2, device=dev, generator=g) - 0.5) * 2e-4
     out = torch.empty(n, 32, device=dev)
     s = _lib.stream_ptr(dev)
-    for tp in ("fp32", "fp16"):
-        desc = spec.descriptor(tp)
-        tab = params if tp == "fp32" else params.to(torch.float16)
-        ms = _time_launches(lambda: _lib.call("pty_hashgrid_fwd", desc, coords.data_ptr(), n, tab.data_ptr(),
-                                              out.data_ptr(), s))
-        esz = 8 if tp == "fp32" else 4
-        nbytes = n * (8 + 16 * 4 * esz + 32 * 4)
-        row(f"k_hashgrid_fwd ({tp} table)", f"{n} points, 16 levels, log2T 20", nbytes, ms,
-            {"bytes_per_point": nbytes / n})
-    del coords, params, out
+    for cname, coords in (("grid", grid), ("random", rnd)):
+        for tp in ("fp32", "fp16"):
+            desc = spec.descriptor(tp)
+            tab = params if tp == "fp32" else params.to(torch.float16)
+            ms = _time_launches(lambda: _lib.call("pty_hashgrid_fwd", desc, coords.data_ptr(), n, tab.data_ptr(),
+                                                  out.data_ptr(), s))
+            esz = 8 if tp == "fp32" else 4
+            nbytes = n * (8 + 16 * 4 * esz + 32 * 4)
+            row(f"k_hashgrid_fwd ({tp} table, {cname} points)", f"{n} points, 16 levels, log2T 20", nbytes, ms,
+                {"bytes_per_point": nbytes / n})
+    del grid, rnd, params, out
 
     # --- Fourier upsample and adjoint: 1824^2 -> 3648^2 complex64
     nsm, ds = 1824, 2
