@@ -60,12 +60,80 @@ __device__ __forceinline__ float2 twiddle(const float2* tw, int idx) {
   return w;
 }
 
+// 8-point DFT in registers: y_q = sum_r v_r W^{rq}, W = exp(DIR 2 pi i / 8).
+// Split r = r' + 4s: y_{2q'} is the 4-point DFT of a_r' = v_r' + v_{r'+4};
+// y_{2q'+1} the 4-point DFT of (v_r' - v_{r'+4}) W^{r'}.
+template <int DIR>
+__device__ __forceinline__ void dft4(const float2 x[4], float2 y[4]) {
+  const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]), t2 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
+  // DIR * i * (x1 - x3)
+  const float2 t3 = (DIR < 0) ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
+  y[0] = cadd(t0, t2);
+  y[1] = cadd(t1, t3);
+  y[2] = csub(t0, t2);
+  y[3] = csub(t1, t3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(const float2 v[8], float2 y[8]) {
+  constexpr float h = 0.70710678118654752f;
+  float2 a[4], b[4], ya[4], yb[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    a[r] = cadd(v[r], v[r + 4]);
+    b[r] = csub(v[r], v[r + 4]);
+  }
+  // b_r *= W^r: W = (1 + DIR i)/sqrt2, W^2 = DIR i, W^3 = (-1 + DIR i)/sqrt2
+  const float s = (float)DIR;
+  b[1] = make_float2(h * (b[1].x - s * b[1].y), h * (b[1].y + s * b[1].x));
+  b[2] = make_float2(-s * b[2].y, s * b[2].x);
+  b[3] = make_float2(h * (-b[3].x - s * b[3].y), h * (-b[3].y + s * b[3].x));
+  dft4<DIR>(a, ya);
+  dft4<DIR>(b, yb);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    y[2 * q] = ya[q];
+    y[2 * q + 1] = yb[q];
+  }
+}
+
 template <int DIR>
 __device__ float2* fft_lines(float2* a, float2* b, int n, int log2n, int lines, int stride,
                              const float2* tw, int tws, int tid, int nthreads) {
   int Ns = 1;
   int p = log2n;
   while (p > 0) {
+    if (p >= 3) {
+      // radix-8 pass (register butterfly): fewer LDS round trips per line
+      const int lgnb = log2n - 3;
+      const int nb = 1 << lgnb;
+      const int total = nb * lines;
+      const int twscale = (n / (Ns * 8)) * tws;
+      for (int t = tid; t < total; t += nthreads) {
+        const int line = t >> lgnb;
+        const int j = t & (nb - 1);
+        const float2* src = a + line * stride;
+        float2* dst = b + line * stride;
+        const int k = j & (Ns - 1);
+        const int d = (j - k) * 8 + k;
+        float2 v[8], y[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = src[j + r * nb];
+        if (Ns > 1) {
+          const int ti = k * twscale;
+#pragma unroll
+          for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, r * ti));
+        }
+        dft8<DIR>(v, y);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) dst[d + q * Ns] = y[q];
+      }
+      __syncthreads();
+      float2* tmp = a; a = b; b = tmp;
+      Ns *= 8;
+      p -= 3;
+      continue;
+    }
     const int R = (p >= 2) ? 4 : 2;
     const int lgR = (R == 4) ? 2 : 1;
     const int lgnb = log2n - lgR;
@@ -183,7 +251,7 @@ __device__ void fft_pass_prime(const float2* __restrict__ a, float2* __restrict_
 
 // Mixed-radix Stockham autosort FFT over `lines` lines in LDS (same contract
 // as fft_lines; tw may point to LDS or global memory and holds
-// exp(-2 pi i k / (n*tws)), k < n*tws).  Radices 2, 3, 4 have dedicated
+// exp(-2 pi i k / (n*tws)), k < n*tws).  Radices 2, 3, 4, 8 have dedicated
 // butterflies, primes 5..31 the register pass above; larger primes use a
 // direct DFT of the R twiddled inputs.
 template <int DIR>
@@ -217,7 +285,19 @@ __device__ float2* fft_lines_mixed(float2* a, float2* b, const FftPlan& P, int l
         float2* dst = b + line * stride;
         const int k = j % Ns;
         const int d = (j - k) * R + k;
-        if (R == 4) {
+        if (R == 8) {
+          float2 v[8], y[8];
+#pragma unroll
+          for (int r = 0; r < 8; ++r) v[r] = src[j + r * nb];
+          if (k > 0) {
+            const int ti = k * twscale;
+#pragma unroll
+            for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, r * ti));
+          }
+          dft8<DIR>(v, y);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) dst[d + q * Ns] = y[q];
+        } else if (R == 4) {
           float2 v0 = src[j], v1 = src[j + nb], v2 = src[j + 2 * nb], v3 = src[j + 3 * nb];
           if (k > 0) {
             const int ti = k * twscale;

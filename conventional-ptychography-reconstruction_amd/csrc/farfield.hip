@@ -282,36 +282,64 @@ __global__ __launch_bounds__(256) void k_far_dprobe(FarArgs a) {
 
 // F5: dobject (overwritten, whole canvas) = sum over the windows covering each
 // pixel of dpsi_b * conj(probe_n), in batch order: a gather, no atomics.
+// Workgroup = one kTile x kTile canvas tile: wave 0 first lists, in batch
+// order, the batch windows that intersect the tile (a ballot scan), so each
+// pixel only tests the few windows near it instead of all B.
 constexpr int kMaxWinLds = 1024;
+constexpr int kTile = 32;
 
 __global__ __launch_bounds__(256) void k_far_dobject(FarArgs a) {
-  __shared__ int2 win[kMaxWinLds];
-  for (int b = threadIdx.x; b < a.B && b < kMaxWinLds; b += blockDim.x) {
-    int ty, tx, p;
-    win[b] = window_of(a, b, ty, tx, p) ? make_int2(ty, tx) : make_int2(-(1 << 30), -(1 << 30));
-  }
-  __syncthreads();
-  const int64_t HW = (int64_t)a.H * a.W;
+  __shared__ int3 list[kMaxWinLds];
+  __shared__ int n_list;
+  const int tiles_x = (a.W + kTile - 1) / kTile;
+  const int Y0 = (blockIdx.x / tiles_x) * kTile, X0 = (blockIdx.x % tiles_x) * kTile;
   const int MN = a.M * a.N;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < HW; q += (int64_t)gridDim.x * blockDim.x) {
-    const int yy = (int)(q / a.W), xx = (int)(q - (int64_t)yy * a.W);
-    float2 acc = make_float2(0.f, 0.f);
-    for (int b = 0; b < a.B; ++b) {
-      int2 w;
-      if (b < kMaxWinLds) {
-        w = win[b];
-      } else {
-        int ty, tx, p;
-        w = window_of(a, b, ty, tx, p) ? make_int2(ty, tx) : make_int2(-(1 << 30), -(1 << 30));
+  constexpr int kPer = kTile * kTile / 256;   // pixels per thread
+  float2 acc[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) acc[i] = make_float2(0.f, 0.f);
+  // batch entries in chunks of kMaxWinLds (one chunk for every practical batch)
+  for (int c0 = 0; c0 < a.B; c0 += kMaxWinLds) {
+    const int c1 = min(a.B, c0 + kMaxWinLds);
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      int count = 0;
+      for (int b0 = c0; b0 < c1; b0 += 64) {
+        const int b = b0 + lane;
+        int ty = 0, tx = 0, p;
+        bool hit = false;
+        if (b < c1 && window_of(a, b, ty, tx, p))
+          hit = ty < Y0 + kTile && ty + a.M > Y0 && tx < X0 + kTile && tx + a.N > X0;
+        const uint64_t mask = __ballot(hit);
+        const int pos = count + __popcll(mask & ((1ull << lane) - 1ull));
+        if (hit) list[pos] = make_int3(b, ty, tx);
+        count += __popcll(mask);
       }
-      const int r = yy - w.x, c = xx - w.y;
-      if ((unsigned)r < (unsigned)a.M && (unsigned)c < (unsigned)a.N) {
-        const float2 v = cmulc(a.psi[(int64_t)b * MN + r * a.N + c], a.probe[r * a.N + c]);
-        acc.x += v.x;
-        acc.y += v.y;
+      if (lane == 0) n_list = count;
+    }
+    __syncthreads();
+    const int nl = n_list;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int t = threadIdx.x + 256 * i;
+      const int yy = Y0 + t / kTile, xx = X0 + t % kTile;
+      for (int k = 0; k < nl; ++k) {
+        const int3 w = list[k];
+        const int r = yy - w.y, c = xx - w.z;
+        if ((unsigned)r < (unsigned)a.M && (unsigned)c < (unsigned)a.N) {
+          const float2 v = cmulc(a.psi[(int64_t)w.x * MN + r * a.N + c], a.probe[r * a.N + c]);
+          acc[i].x += v.x;
+          acc[i].y += v.y;
+        }
       }
     }
-    a.dobj[q] = acc;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int t = threadIdx.x + 256 * i;
+    const int yy = Y0 + t / kTile, xx = X0 + t % kTile;
+    if (yy < a.H && xx < a.W) a.dobj[(int64_t)yy * a.W + xx] = acc[i];
   }
 }
 
@@ -420,9 +448,8 @@ extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const 
   hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), shm, stream, a, lines);
   const int mn = a.M * a.N;
   hipLaunchKernelGGL(k_far_dprobe, dim3((mn + 255) / 256 < 1024 ? (mn + 255) / 256 : 1024), dim3(256), 0, stream, a);
-  const int64_t hw = (int64_t)a.H * a.W;
-  hipLaunchKernelGGL(k_far_dobject, dim3((unsigned)((hw + 255) / 256 < 4096 ? (hw + 255) / 256 : 4096)), dim3(256), 0,
-                     stream, a);
+  const int64_t tiles = (int64_t)((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
+  hipLaunchKernelGGL(k_far_dobject, dim3((unsigned)tiles), dim3(256), 0, stream, a);
   return check_launch("farfield_fwd_bwd");
 }
 

@@ -128,6 +128,7 @@ int make_fft_plan(int n, FftPlan& P) {
   P.log2n = P.pow2 ? ilog2(n) : 0;
   int r = n, k = 0;
   auto push = [&](int R) { P.radix[k++] = R; r /= R; };
+  while (r % 8 == 0 && k < kMaxPasses) push(8);
   while (r % 4 == 0 && k < kMaxPasses) push(4);
   while (r % 2 == 0 && k < kMaxPasses) push(2);
   for (int p = 3; p <= kMaxRadix && r > 1; p += 2)
