@@ -12,7 +12,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libptycho_hip.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_LEVELS = 16
 MAX_HIDDEN = 4
 
@@ -45,6 +45,7 @@ class PtyPoints(ctypes.Structure):
     _fields_ = [
         ("coords", c_void_p), ("axis0", c_void_p), ("axis1", c_void_p),
         ("n_axis0", c_int32), ("n_axis1", c_int32), ("n", c_int64), ("plan", c_void_p),
+        ("features", c_void_p),
     ]
 
 
@@ -71,6 +72,7 @@ _SIGNATURES = {
     "pty_inr_bwd_finish": (c_int32, [P(PtyGrid), P(PtyMlp), P(PtyMlp), c_void_p, P(PtyPoints), c_void_p, c_size_t,
                                      c_void_p]),
     "pty_points_plan_bytes": (c_size_t, [P(PtyGrid), P(PtyPoints)]),
+    "pty_points_feature_floats": (c_size_t, [c_int64]),
     "pty_points_plan_build": (c_int32, [P(PtyGrid), P(PtyPoints), c_void_p, c_size_t, c_void_p]),
     "pty_mlp_fwd": (c_int32, [P(PtyMlp), c_void_p, c_int64, c_void_p, c_void_p]),
     "pty_upsample_workspace": (c_size_t, [c_int32, c_int32]),

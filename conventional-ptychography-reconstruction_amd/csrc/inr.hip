@@ -180,6 +180,7 @@ static int validate_points(const PtyPoints* p) {
     PTY_REQUIRE(p->axis0 && p->axis1 && p->n_axis0 > 0 && p->n_axis1 > 0, PTY_EINVAL, "points: no coordinates");
     PTY_REQUIRE((int64_t)p->n_axis0 * p->n_axis1 == p->n, PTY_EINVAL, "points: n != n_axis0*n_axis1");
   }
+  PTY_REQUIRE(((uintptr_t)p->features & 15u) == 0, PTY_EINVAL, "points: feature cache must be 16-byte aligned");
   return PTY_OK;
 }
 
@@ -211,6 +212,11 @@ static int fwd_blocks(int64_t n) {
 }  // namespace pty
 
 using namespace pty;
+
+extern "C" size_t pty_points_feature_floats(int64_t n) {
+  if (n <= 0) return 0;
+  return 32 * (size_t)((n + 31) / 32) * 32;
+}
 
 extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* table, const PtyPoints* pts, float* out,
                            hipStream_t stream) {

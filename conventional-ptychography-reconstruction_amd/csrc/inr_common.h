@@ -174,6 +174,22 @@ __device__ __forceinline__ void encode_lane_batched(const PtyGrid& g, float x0, 
   }
 }
 
+// Feature cache of the 32-point tile layout: the 16 features of lane `lane`
+// of tile `tile` at features[(tile * 64 + lane) * 16] (64 coalesced bytes).
+__device__ __forceinline__ void store_features32(float* f, int64_t tile, int lane, const float feat[16]) {
+  float4* q = reinterpret_cast<float4*>(f + (tile * 64 + lane) * 16);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] = make_float4(feat[4 * i], feat[4 * i + 1], feat[4 * i + 2], feat[4 * i + 3]);
+}
+__device__ __forceinline__ void load_features32(const float* f, int64_t tile, int lane, float feat[16]) {
+  const float4* q = reinterpret_cast<const float4*>(f + (tile * 64 + lane) * 16);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float4 a = q[i];
+    feat[4 * i] = a.x; feat[4 * i + 1] = a.y; feat[4 * i + 2] = a.z; feat[4 * i + 3] = a.w;
+  }
+}
+
 // Layer 0: u0[v] (unit crow(v,h)) = b0 + W0 feat.
 template <int H>
 __device__ __forceinline__ v16f layer0(const MlpLds<H>& s, const float feat[16], int j, int h) {

@@ -164,6 +164,20 @@ __device__ __forceinline__ void encode16(const PtyGrid& gr, float x0, float x1, 
   }
 }
 
+// Feature cache: the 8 features of lane `lane` of tile `tile` at
+// features[(tile * 64 + lane) * 8] (32 coalesced bytes per lane).
+__device__ __forceinline__ void store_features16(float* f, int64_t tile, int lane, const float feat[8]) {
+  float4* p = reinterpret_cast<float4*>(f + (tile * 64 + lane) * 8);
+  p[0] = make_float4(feat[0], feat[1], feat[2], feat[3]);
+  p[1] = make_float4(feat[4], feat[5], feat[6], feat[7]);
+}
+__device__ __forceinline__ void load_features16(const float* f, int64_t tile, int lane, float feat[8]) {
+  const float4* p = reinterpret_cast<const float4*>(f + (tile * 64 + lane) * 8);
+  const float4 a = p[0], b = p[1];
+  feat[0] = a.x; feat[1] = a.y; feat[2] = a.z; feat[3] = a.w;
+  feat[4] = b.x; feat[5] = b.y; feat[6] = b.z; feat[7] = b.w;
+}
+
 // u[mb] (units 16 mb + 4 g + v) = b0 + W0 h
 template <int H>
 __device__ __forceinline__ void layer0_16(const Lds16<H>& s, const float feat[8], int i, int g, v4f u[2]) {
@@ -235,6 +249,7 @@ __global__ __launch_bounds__(64 * kFwd16Waves) void k_inr_fwd16(PtyGrid gr, PtyM
     if (valid) point_coords(pts, p, x0, x1);
     float feat[8];
     encode16(gr, x0, x1, g, table, feat);
+    if (pts.features) store_features16(pts.features, tile, lane, feat);
     float zr[8], zi[8];
     {
       v4f u[2];
@@ -324,11 +339,16 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     asm volatile("" ::: "memory");
     const int64_t p = tile * 16 + j;
     const bool valid = p < pts.n;
-    float x0 = 0.5f, x1 = 0.5f;
-    if (valid) point_coords(pts, p, x0, x1);
-    // ---------------- recompute forward (inputs of every layer staged)
+    // ---------------- recompute forward (inputs of every layer staged); the
+    // features come from the forward's cache when there is one
     float feat[8];
-    encode16(gr, x0, x1, g, table, feat);
+    if (pts.features) {
+      load_features16(pts.features, tile, lane, feat);
+    } else {
+      float x0 = 0.5f, x1 = 0.5f;
+      if (valid) point_coords(pts, p, x0, x1);
+      encode16(gr, x0, x1, g, table, feat);
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c) S.F[feat16(c, g) * PS + j] = feat[c];
     float zr[8], zi[8];

@@ -391,6 +391,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_fwd_lp(PtyGrid g, PtyMlp m, co
     if (valid) point_coords(pts, p, x0, x1);
     float feat[16];
     encode_lane(g, x0, x1, h, table, feat);
+    if (pts.features) store_features32(pts.features, tile, lane, feat);
     typename F::op_t pf[2] = {F::pack8(feat, s.xs[kSlotH]), F::pack8(feat + 8, s.xs[kSlotH])};
     if (F::kScaled) mx[0] = fmaxf(mx[0], fmaxf(amax8(feat), amax8(feat + 8)));
     const v16f u0 = layer0_lp<F, H>(o, s, pf, lane, h);
@@ -480,14 +481,19 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
     asm volatile("" ::: "memory");
     const int64_t p = tile * 32 + j;
     const bool valid = p < pts.n;
-    float x0 = 0.5f, x1 = 0.5f;
-    if (valid) point_coords(pts, p, x0, x1);
     // ---------------- recompute forward; keep only the packed inputs of
-    // every layer (the pre-activations are not needed by the backward)
+    // every layer (the pre-activations are not needed by the backward); the
+    // features come from the forward's cache when there is one
     op_t pf[2];
     {
       float feat[16];
-      encode_lane_batched(g, x0, x1, h, table, feat);
+      if (pts.features) {
+        load_features32(pts.features, tile, lane, feat);
+      } else {
+        float x0 = 0.5f, x1 = 0.5f;
+        if (valid) point_coords(pts, p, x0, x1);
+        encode_lane_batched(g, x0, x1, h, table, feat);
+      }
 #pragma unroll
       for (int v = 0; v < 16; ++v) S.F_[kappa0(v, h) * BS + j] = F::el(feat[v], sh);
       pf[0] = F::pack8(feat, sh);

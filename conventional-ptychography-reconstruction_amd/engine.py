@@ -142,6 +142,15 @@ class FusedReconstruction:
         self.prb_pts = model.grid_points("probe")
         self.obj_pdesc = self.obj_pts.descriptor(self.obj_spec)
         self.prb_pdesc = self.prb_pts.descriptor(self.prb_spec)
+        # feature caches: the forward of a step stores the encoded features,
+        # its backward reads them instead of gathering the tables again (the
+        # tables only change in Adam, after the backward)
+        self.obj_feat = torch.empty(_lib.query("pty_points_feature_floats", self.obj_pts.n), dtype=torch.float32,
+                                    device=dev)
+        self.prb_feat = torch.empty(_lib.query("pty_points_feature_floats", self.prb_pts.n), dtype=torch.float32,
+                                    device=dev)
+        self.obj_pdesc.features = self.obj_feat.data_ptr()
+        self.prb_pdesc.features = self.prb_feat.data_ptr()
         # FP8: per-MLP delayed-scaling state, rolled after every backward
         self.fp8_states = []
         st_o = st_p = None

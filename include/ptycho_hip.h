@@ -32,7 +32,7 @@ extern "C" {
 
 typedef struct ihipStream_t* hipStream_t;
 
-#define PTY_ABI_VERSION 4
+#define PTY_ABI_VERSION 5
 #define PTY_MAX_LEVELS 16
 #define PTY_MAX_HIDDEN 4
 #define PTY_FP8_STATE_FLOATS 520
@@ -122,6 +122,12 @@ typedef struct PtyPoints {
    * this (grid, axes) pair (cell ranges, fractions, hash buckets); NULL makes
    * pty_inr_bwd build it in its workspace on every call. */
   const void* plan;
+  /* Optional feature cache (no counterpart in the reference): a device
+   * buffer of pty_points_feature_floats(n) floats.  pty_inr_fwd stores the
+   * encoded features of every point there (in the kernels' tile layout) and
+   * pty_inr_bwd / pty_inr_bwd_mlp read them instead of gathering the table
+   * again; the caller must not change the table in between.  NULL: no cache. */
+  float* features;
 } PtyPoints;
 
 /* Far-field batch descriptor (Recovery.py:287-307). */
@@ -169,6 +175,9 @@ int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float* table, con
                     const float* dout, void* work, size_t work_bytes, hipStream_t stream);
 int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
                        const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream);
+/* Floats of the feature cache of n points (32 per point, n rounded up to a
+ * multiple of 32). */
+size_t pty_points_feature_floats(int64_t n);
 /* Static plan of a regular grid for the table gradient (no counterpart in
  * the reference: it only depends on the hash-grid layout and the axes). */
 size_t pty_points_plan_bytes(const PtyGrid* g, const PtyPoints* pts);

@@ -258,13 +258,16 @@ def test_fused_step_matches_oracle(pkg, dev, loss_type):
     # along q.  The Poisson loss gradient is dominated by that (scale)
     # component, so an FP32 computation keeps fewer correct digits there.
     # The bar of each probe tensor is the larger of 2e-3 (6e-3 Poisson) and
-    # three times the error of the oracle's own autograd run in FP32.
+    # four times the error of the oracle's own autograd run in FP32: these
+    # gradients are cancelling sums (e.g. the residual scale's Re sum dprobe
+    # conj(mlp) over the probe), so two correct FP32 evaluation orders differ
+    # by several times their rounding.
     tol_p = 6e-3 if loss_type == "Poisson_likelihood_loss" else 2e-3
     _, g32 = step_reference(model, oracle_params(model, torch.float32), inst, batch, loss_type,
                             dtype=torch.float32)
 
     def bar(ours32, ref64):
-        return max(tol_p, 3 * rel_err(ours32, ref64))
+        return max(tol_p, 4 * rel_err(ours32, ref64))
 
     assert rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"]) < bar(g32["prb_table"],
                                                                                       g["prb_table"])
