@@ -246,36 +246,62 @@ __global__ void k_plan_buckets(PtyGrid g, PlanLayout L, const int* __restrict__ 
 }
 
 // ------------------------------------------------------------------ gradient
-// Dense pass 1: A[l][i][cx] = sum_dx sum_{j in cell cx-dx} w(fx_j, dx) g[l][i n0 + j]
+// Dense pass 1: A[l][i][cx] = S0(cx) + S1(cx - 1), with the cell sums
+// S0(c) = sum_{j in cell c} (1 - fx_j) g[l][i n0 + j] and S1(c) = sum fx_j g:
+// each thread reads its own cell's points once (the cells of a row are
+// consecutive index ranges, so a workgroup's reads are contiguous) and takes
+// the left neighbour's S1 from LDS; the first thread of a workgroup forms it.
+__device__ __forceinline__ void cell_sums(const int* st, const float* fxl, const float2* gl, int cell, float2& s0,
+                                          float2& s1) {
+  s0 = s1 = make_float2(0.f, 0.f);
+  if (cell < 0) return;
+  const int j1 = st[cell + 1];
+  // unrolled so that the loads of coarse cells (tens of points) overlap
+#pragma unroll 8
+  for (int j = st[cell]; j < j1; ++j) {
+    const float f = fxl[j];
+    const float2 v = gl[j];
+    s0.x += (1.f - f) * v.x;
+    s0.y += (1.f - f) * v.y;
+    s1.x += f * v.x;
+    s1.y += f * v.y;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                      const float2* __restrict__ gfeat, float2* __restrict__ A) {
+  __shared__ float2 s1_lds[256];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.off_a[g.n_levels]) return;
-  int l = 0;
-  while (l + 1 < g.n_levels && t >= L.off_a[l + 1]) ++l;
-  const int res1 = (int)g.res[l] + 1;
-  const int64_t r = t - L.off_a[l];
-  const int i = (int)(r / res1), cx = (int)(r - (int64_t)i * res1);
-  const int* st = P.s0 + L.off_s[l];
-  const float* fxl = P.fx + (int64_t)l * L.n0;
-  const float2* gl = gfeat + (int64_t)l * pts.n + (int64_t)i * L.n0;
-  float ax = 0.f, ay = 0.f;
-#pragma unroll
-  for (int dx = 0; dx < 2; ++dx) {
-    const int cell = cx - dx;
-    if (cell < 0) continue;
-    const int j1 = st[cell + 1];
-    // unrolled so that the loads of coarse cells (tens of points) overlap;
-    // the summation order is unchanged
-#pragma unroll 8
-    for (int j = st[cell]; j < j1; ++j) {
-      const float w = dx ? fxl[j] : 1.f - fxl[j];
-      const float2 v = gl[j];
-      ax += w * v.x;
-      ay += w * v.y;
+  const bool active = t < L.off_a[g.n_levels];
+  int l = 0, i = 0, cx = 0;
+  float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
+  const int* st = nullptr;
+  const float* fxl = nullptr;
+  const float2* gl = nullptr;
+  if (active) {
+    while (l + 1 < g.n_levels && t >= L.off_a[l + 1]) ++l;
+    const int res1 = (int)g.res[l] + 1;
+    const int64_t r = t - L.off_a[l];
+    i = (int)(r / res1);
+    cx = (int)(r - (int64_t)i * res1);
+    st = P.s0 + L.off_s[l];
+    fxl = P.fx + (int64_t)l * L.n0;
+    gl = gfeat + (int64_t)l * pts.n + (int64_t)i * L.n0;
+    cell_sums(st, fxl, gl, cx, s0, s1);
+  }
+  s1_lds[threadIdx.x] = s1;
+  __syncthreads();
+  if (!active) return;
+  float2 left = make_float2(0.f, 0.f);
+  if (cx > 0) {
+    if (threadIdx.x > 0) {
+      left = s1_lds[threadIdx.x - 1];   // thread t-1 holds cell cx-1 of the same row
+    } else {
+      float2 d0;
+      cell_sums(st, fxl, gl, cx - 1, d0, left);
     }
   }
-  A[t] = make_float2(ax, ay);
+  A[t] = make_float2(s0.x + left.x, s0.y + left.y);
 }
 
 // G(cx, cy) from A (dense levels)
