@@ -15,12 +15,20 @@ forward, one batch of positions, backward, Adam); the reference's "iters"
 are epochs = ceil(positions/batch) steps.  value = iters/s = epochs per
 second.
 
+Two timed regions of K steps each, both bracketed by a synchronisation:
+(1) eager steps (the host enqueues every launch), with HIP events on the
+launch streams around each launch of the dominant kernel; (2) the same
+steps replayed from the captured HIP graph, which is how
+FusedReconstruction.run_epoch (the reconstruction loop) runs full batches;
+value and ms_per_step come from (2) (from (1) with --no-graph), and
+eager_ms_per_step reports (1).
+
 Also reported: the roofline of the dominant kernel (the MLP backward with
 recompute, k_inr_bwd16 in FP32, launched once for the object grid and once for the
-probe grid per step; HIP events on its launch stream bracket each launch
-inside the timed region), its HBM traffic from the committed rocprofv3 PMC
-summary, and the CPU baseline (the FP32 oracle restatement on the host
-cores, a bounded sample of steps).
+probe grid per step; its average launch duration from the events of region
+(1)), its HBM traffic from the committed rocprofv3 PMC summary, and the CPU
+baseline (the FP32 oracle restatement on the host cores, a bounded sample of
+steps).
 """
 from __future__ import annotations
 
@@ -850,7 +858,9 @@ def main():
     for _ in range(args.warmup):
         eng.step(next_batch())
     torch.cuda.synchronize()
-    eng.enable_timing(["obj_mlp_bwd", "prb_mlp_bwd", "obj_inr_fwd", "farfield"])
+    # timed region 1: K eager steps (one launch sequence per step from the
+    # host), HIP events on the launch streams around each MLP-backward launch
+    eng.enable_timing(["obj_mlp_bwd", "prb_mlp_bwd"])
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
@@ -860,10 +870,9 @@ def main():
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    total_ms = start.elapsed_time(stop)
-    ms_per_step = total_ms / args.steps
-    steps_per_s = 1000.0 / ms_per_step
+    eager_ms = start.elapsed_time(stop) / args.steps
     tim = eng.timings_ms()
+    eng.enable_timing([])
     # the MLP-backward kernel runs twice a step (object grid, probe grid);
     # average launch duration and average points per launch over both
     launches = tim["obj_mlp_bwd"] + tim["prb_mlp_bwd"]
@@ -874,7 +883,9 @@ def main():
     achieved = MLP_FLOP_BWD_PER_POINT * n_avg / (bwd_ms * 1e-3) / 1e12
     peak = MFMA_PEAK_TFLOPS[args.precision]
     loss_now = float(eng.loss.sum().item())
-    # the same steps replayed from one captured HIP graph
+    # timed region 2: the same steps replayed from the captured HIP graph, as
+    # FusedReconstruction.run_epoch (the reconstruction loop) runs them; the
+    # copy of each batch's indices into the graph's buffer is inside the region
     graph_ms = None
     if not args.no_graph:
         eng.capture()
@@ -883,12 +894,16 @@ def main():
         torch.cuda.synchronize()
         g0 = torch.cuda.Event(enable_timing=True)
         g1 = torch.cuda.Event(enable_timing=True)
+        t1 = time.perf_counter()
         g0.record()
         for _ in range(args.steps):
             eng.graph_step(next_batch())
         g1.record()
         torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
         graph_ms = g0.elapsed_time(g1) / args.steps
+    ms_per_step = graph_ms if graph_ms is not None else eager_ms
+    steps_per_s = 1000.0 / ms_per_step
 
     line = {
         "metric": "wall-s to plain-final object NRMSE + iters/s: HIP fused vs plain, 1 MI355X",
@@ -907,7 +922,9 @@ def main():
         "config": {"workload": wl_name, "steps_per_iter": steps_per_iter, "global_batch": batch,
                    "positions": frames, "parallelism": "single-gpu"},
         "steps_per_s": steps_per_s,
+        "timed_path": "graph replay (run_epoch)" if graph_ms is not None else "eager",
         "graph_ms_per_step": graph_ms,
+        "eager_ms_per_step": eager_ms,
         "wall_s_timed": wall,
         "kernel_ms": {k: (sum(v) / len(v) if v else None) for k, v in tim.items()},
         "loss_last_step": loss_now,

@@ -1,7 +1,14 @@
 """Fused reconstruction step: one optimizer step of Recovery.py:247-321 as a
-fixed sequence of HIP launches on one stream, with no autograd graph and no
-host synchronisation (every pointer is fixed at construction, so the step can
-be replayed or captured).
+fixed sequence of HIP launches, with no autograd graph and no host
+synchronisation (every pointer is fixed at construction, so the step can be
+replayed or captured).
+
+The object and probe chains are independent between the start of the step
+and the far field, and again between the far field and Adam: the probe chain
+runs on a second stream (default priority) forked from and joined back into
+the caller's stream, so its launches fill the gaps the object chain leaves
+(small kernels, launch boundaries).  Each chain has its own buffers and
+workspaces, so the result does not depend on how the two interleave.
 
 Per step (batch of positions ``batch_idx`` on the device):
   object  = Upsample(INR_obj(grid))            [+ k_o * . + initial_o]
@@ -189,8 +196,8 @@ class FusedReconstruction:
         self.ws_up = _lib.workspace(q("pty_upsample_workspace", n, self.ds), dev)
         self.ws_probe = _lib.workspace(q("pty_probe_workspace", M, N), dev)
         self.ws_haar = _lib.workspace(q("pty_haar_workspace", H, W) if self.sparsity_weight else 1, dev)
-        self.ws_res = _lib.workspace(max(q("pty_residual_workspace", H * W), q("pty_residual_workspace", M * N)),
-                                     dev)
+        self.ws_res_o = _lib.workspace(q("pty_residual_workspace", H * W), dev)
+        self.ws_res_p = _lib.workspace(q("pty_residual_workspace", M * N), dev)
         self.ws_obj = _lib.workspace(q("pty_inr_bwd_workspace", self.obj_grid, self.obj_mlp, self.obj_pdesc), dev)
         self.ws_prb = _lib.workspace(q("pty_inr_bwd_workspace", self.prb_grid, self.prb_mlp, self.prb_pdesc), dev)
         self.far = {}
@@ -200,6 +207,9 @@ class FusedReconstruction:
                 self.far[b] = (d, _lib.workspace(q("pty_farfield_workspace", d), dev))
         self.epoch = 0
         self._timers = None
+        # the probe chain's stream (default priority, like every stream here)
+        self.side = torch.cuda.Stream(device=dev)
+        self.graph = None
         self._grad_clear = True   # FlatParams starts with zero gradients
         if self.fp8_states:
             # calibration: one step without the update records the amax of
@@ -214,11 +224,11 @@ class FusedReconstruction:
         'prb_mlp_bwd', which time the MLP-backward kernel alone)."""
         self._timers = {n: [] for n in names}
 
-    def _mark(self, name, begin):
+    def _mark(self, name, begin, stream=None):
         if self._timers is None or name not in self._timers:
             return None
         ev = torch.cuda.Event(enable_timing=True)
-        ev.record()
+        ev.record(stream)
         if begin:
             self._timers[name].append([ev, None])
         else:
@@ -240,46 +250,55 @@ class FusedReconstruction:
                 _lib.call("pty_cast_f16", enc.params.data_ptr(), enc.params.numel(), tab.data_ptr(), s)
 
     def forward_fields(self):
-        s = _lib.stream_ptr(self.device)
+        main, side = torch.cuda.current_stream(self.device), self.side
         C = _lib.call
         m = self.model
-        self._refresh_tables(s)
-        self._mark("obj_inr_fwd", True)
+        self._refresh_tables(main.cuda_stream)
+        side.wait_stream(main)
+        # object chain on the caller's stream
+        s = main.cuda_stream
+        self._mark("obj_inr_fwd", True, main)
         C("pty_inr_fwd", self.obj_grid, self.obj_mlp, self.obj_tab.data_ptr(), self.obj_pdesc,
           self.obj_small.data_ptr(), s)
-        self._mark("obj_inr_fwd", False)
+        self._mark("obj_inr_fwd", False, main)
+        # probe chain on the side stream (enqueued while the object encoder runs)
+        p = side.cuda_stream
+        C("pty_inr_fwd", self.prb_grid, self.prb_mlp, self.prb_tab.data_ptr(), self.prb_pdesc,
+          self.prb_mlp_out.data_ptr(), p)
+        if m.has_probe_residual:
+            C("pty_residual_fwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), m.probe_initial.data_ptr(),
+              self.prb_mlp_out.numel(), self.probe.data_ptr(), p)
+        C("pty_probe_normalize_fwd", self.probe.data_ptr(), self.M, self.N, self.probe_n.data_ptr(),
+          self.stats.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), p)
         C("pty_upsample_fwd", self.obj_small.data_ptr(), self.n_small, self.ds, self.obj_up.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
         if m.has_object_residual:
             C("pty_residual_fwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), m.object_initial.data_ptr(),
               self.obj_up.numel(), self.obj.data_ptr(), s)
-        C("pty_inr_fwd", self.prb_grid, self.prb_mlp, self.prb_tab.data_ptr(), self.prb_pdesc,
-          self.prb_mlp_out.data_ptr(), s)
-        if m.has_probe_residual:
-            C("pty_residual_fwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), m.probe_initial.data_ptr(),
-              self.prb_mlp_out.numel(), self.probe.data_ptr(), s)
-        C("pty_probe_normalize_fwd", self.probe.data_ptr(), self.M, self.N, self.probe_n.data_ptr(),
-          self.stats.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
+        main.wait_stream(side)
 
-    def _inr_bwd(self, which, grid, mlp, gmlp, table, dtable, pdesc, dout, ws, s):
-        """INR backward; with timing on for '<which>_mlp_bwd' the MLP stage is
-        bracketed by events on its own (the reduce and table gradient follow)."""
+    def _inr_bwd(self, which, grid, mlp, gmlp, table, dtable, pdesc, dout, ws, stream):
+        """INR backward on ``stream``; with timing on for '<which>_mlp_bwd' the
+        MLP stage is bracketed by events on its own (the reduce and table
+        gradient follow)."""
         C = _lib.call
+        s = stream.cuda_stream
         name = which + "_mlp_bwd"
         if self._timers is None or name not in self._timers:
-            self._mark(which + "_inr_bwd", True)
+            self._mark(which + "_inr_bwd", True, stream)
             C("pty_inr_bwd", grid, mlp, gmlp, table.data_ptr(), dtable.data_ptr(), pdesc, dout.data_ptr(),
               ws.data_ptr(), ws.numel(), s)
-            self._mark(which + "_inr_bwd", False)
+            self._mark(which + "_inr_bwd", False, stream)
             return
-        self._mark(name, True)
+        self._mark(name, True, stream)
         C("pty_inr_bwd_mlp", grid, mlp, table.data_ptr(), pdesc, dout.data_ptr(), ws.data_ptr(), ws.numel(), s)
-        self._mark(name, False)
+        self._mark(name, False, stream)
         C("pty_inr_bwd_finish", grid, mlp, gmlp, dtable.data_ptr(), pdesc, ws.data_ptr(), ws.numel(), s)
 
     def step(self, batch_idx: torch.Tensor, update: bool = True):
         """One optimizer step on the positions ``batch_idx`` (int32, device)."""
-        s = _lib.stream_ptr(self.device)
+        main, side = torch.cuda.current_stream(self.device), self.side
+        s = main.cuda_stream
         C = _lib.call
         m = self.model
         b = batch_idx.numel()
@@ -287,32 +306,39 @@ class FusedReconstruction:
         if not self._grad_clear:
             C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
         self.forward_fields()
-        self._mark("farfield", True)
+        self._mark("farfield", True, main)
         C("pty_farfield_fwd_bwd", d, self.obj.data_ptr(), self.probe_n.data_ptr(), self.stack.data_ptr(),
           batch_idx.data_ptr(), self.tl.data_ptr(), self.loss.data_ptr(), self.dobj.data_ptr(),
           self.dprobe_n.data_ptr(), ws_far.data_ptr(), ws_far.numel(), s)
-        self._mark("farfield", False)
+        self._mark("farfield", False, main)
         if self.sparsity_weight:
             C("pty_haar_sparsity", self.obj.data_ptr(), self.H, self.W, 1.0, self.sparsity_weight,
               self.sparsity.data_ptr(), self.dobj.data_ptr(), self.ws_haar.data_ptr(), self.ws_haar.numel(), s)
-        C("pty_probe_normalize_bwd", self.probe.data_ptr(), self.dprobe_n.data_ptr(), self.M, self.N,
-          self.stats.data_ptr(), self.dprobe.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), s)
-        if m.has_probe_residual:
-            C("pty_residual_bwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), self.dprobe.data_ptr(),
-              self.dprobe.numel(), self.dprb_mlp.data_ptr(), self.flat.g(m.k_probe).data_ptr(),
-              self.ws_res.data_ptr(), self.ws_res.numel(), s)
+        side.wait_stream(main)
+        # object adjoint of the upsample first (the object chain is the longer
+        # one), then the probe chain on the side stream, then the object INR
         if m.has_object_residual:
             C("pty_residual_bwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), self.dobj.data_ptr(),
               self.dobj.numel(), self.dobj_up.data_ptr(), self.flat.g(m.k_object).data_ptr(),
-              self.ws_res.data_ptr(), self.ws_res.numel(), s)
+              self.ws_res_o.data_ptr(), self.ws_res_o.numel(), s)
         C("pty_upsample_adj", self.dobj_up.data_ptr(), self.n_small, self.ds, self.dsmall.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
-        self._inr_bwd("obj", self.obj_grid, self.obj_mlp, self.obj_gmlp, self.obj_tab,
-                      self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, s)
+        p = side.cuda_stream
+        C("pty_probe_normalize_bwd", self.probe.data_ptr(), self.dprobe_n.data_ptr(), self.M, self.N,
+          self.stats.data_ptr(), self.dprobe.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), p)
+        if m.has_probe_residual:
+            C("pty_residual_bwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), self.dprobe.data_ptr(),
+              self.dprobe.numel(), self.dprb_mlp.data_ptr(), self.flat.g(m.k_probe).data_ptr(),
+              self.ws_res_p.data_ptr(), self.ws_res_p.numel(), p)
         self._inr_bwd("prb", self.prb_grid, self.prb_mlp, self.prb_gmlp, self.prb_tab,
-                      self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, s)
-        for st in self.fp8_states:
-            C("pty_fp8_roll", st.data_ptr(), s)
+                      self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, side)
+        self._inr_bwd("obj", self.obj_grid, self.obj_mlp, self.obj_gmlp, self.obj_tab,
+                      self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, main)
+        if self.fp8_states:
+            st_o, st_p = self.fp8_states
+            C("pty_fp8_roll", st_o.data_ptr(), s)
+            C("pty_fp8_roll", st_p.data_ptr(), p)
+        main.wait_stream(side)
         if update:
             # Adam clears the trainable gradients it consumes; frozen ones are
             # cleared with them when they exist
@@ -354,8 +380,12 @@ class FusedReconstruction:
         self.graph.replay()
         return self.loss
 
-    def run_epoch(self, generator: torch.Generator | None = None, use_graph: bool = False):
-        """One epoch: randperm over frames, ceil(frames/batch) steps (Recovery.py:243-350)."""
+    def run_epoch(self, generator: torch.Generator | None = None, use_graph: bool = True):
+        """One epoch: randperm over frames, ceil(frames/batch) steps (Recovery.py:243-350).
+
+        Full batches replay the captured step (captured on first use); the
+        trailing partial batch, if any, runs eagerly.  Both give the same
+        bits (the graph holds the same launches)."""
         lr = self.lr0 * self.gamma ** (self.epoch // self.decay_step)
         if self.epoch == 0 or self.epoch % self.decay_step == 0:
             self.set_lr(lr)
@@ -364,6 +394,8 @@ class FusedReconstruction:
         for start in range(0, self.frames, self.batch_size):
             bidx = perm[start:start + self.batch_size]
             if use_graph and bidx.numel() == self.batch_size:
+                if self.graph is None:
+                    self.capture()
                 total += self.graph_step(bidx).sum()
             else:
                 total += self.step(bidx).sum()
