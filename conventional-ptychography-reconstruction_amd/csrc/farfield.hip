@@ -14,10 +14,12 @@
 //   F1 rows  : Psi1[b, r, :] = FFT_N(probe_n[r,:] * patch[r,:])
 //   F2 cols  : FFT_M; smooth-L1 / Poisson: loss, dI, dPsi, IFFT_M in place
 //              FD: store Psi and I, then FD1 (loss, gx, gy), FD2 (dPsi), F2i (IFFT_M)
-//   F3 rows  : per (row block, b): dpsi = IFFT_N(dPsi rows), in place
+//   F3 rows  : per (row block, b): dpsi = IFFT_N(dPsi rows), in place; the
+//              first row block of each pattern also sums its loss partials
 //   F4       : dprobe = sum_b dpsi_b * conj(patch_b)             (fixed order)
 //   F5       : dobject[px] = sum over windows covering px of dpsi_b * conj(probe_n)
 //              (a per-pixel gather in batch order: no atomics, deterministic)
+//              F4 and F5 share one launch.
 #include "common.h"
 
 namespace pty {
@@ -233,17 +235,9 @@ __global__ __launch_bounds__(256) void k_far_fd2(FarArgs a) {
   }
 }
 
-// per-pattern loss = sum of partials / (M N)
-__global__ void k_far_loss_final(FarArgs a) {
-  const int b = blockIdx.x;
-  if (threadIdx.x != 0) return;
-  KSum acc;
-  for (int i = 0; i < a.P; ++i) acc.add(a.partial[b * a.P + i]);
-  a.loss_out[b] = acc.value() / ((float)a.M * (float)a.N);
-}
-
 // F3: rows, inverse.  Workgroup (row block, pattern b): dpsi = IFFT_N of the
-// dPsi rows (unnormalised), written back in place.
+// dPsi rows (unnormalised), written back in place.  Workgroup (0, b) also
+// forms loss_b = sum of the pattern's P loss partials (in order) / (M N).
 __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   extern __shared__ float2 smem[];
   const int N = a.N;
@@ -252,6 +246,11 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   float2* y = x + lines * N;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int b = blockIdx.y;
+  if (blockIdx.x == 0 && tid == 0 && a.loss_out) {
+    KSum acc;
+    for (int i = 0; i < a.P; ++i) acc.add(a.partial[b * a.P + i]);
+    a.loss_out[b] = acc.value() / ((float)a.M * (float)N);
+  }
   int ty, tx, p;
   if (!window_of(a, b, ty, tx, p)) return;  // uniform across the block
   fill_twiddles(tw, N, tid, nt);
@@ -263,10 +262,11 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   for (int t = tid; t < lines * N; t += nt) base[(int64_t)(r0 + t / N) * N + (t % N)] = res[t];
 }
 
-// F4: dprobe = sum_b dpsi_b * conj(patch_b) over valid batch entries (fixed order)
-__global__ __launch_bounds__(256) void k_far_dprobe(FarArgs a) {
+// F4: dprobe = sum_b dpsi_b * conj(patch_b) over valid batch entries (fixed
+// order); workgroup blk of nblk strides over the probe pixels.
+__device__ __forceinline__ void far_dprobe(const FarArgs& a, int blk, int nblk) {
   const int MN = a.M * a.N;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < MN; q += gridDim.x * blockDim.x) {
+  for (int q = blk * 256 + (int)threadIdx.x; q < MN; q += nblk * 256) {
     const int r = q / a.N, c = q - r * a.N;
     float2 acc = make_float2(0.f, 0.f);
     for (int b = 0; b < a.B; ++b) {
@@ -288,11 +288,11 @@ __global__ __launch_bounds__(256) void k_far_dprobe(FarArgs a) {
 constexpr int kMaxWinLds = 1024;
 constexpr int kTile = 32;
 
-__global__ __launch_bounds__(256) void k_far_dobject(FarArgs a) {
+__device__ __forceinline__ void far_dobject(const FarArgs& a, int tile) {
   __shared__ int3 list[kMaxWinLds];
   __shared__ int n_list;
   const int tiles_x = (a.W + kTile - 1) / kTile;
-  const int Y0 = (blockIdx.x / tiles_x) * kTile, X0 = (blockIdx.x % tiles_x) * kTile;
+  const int Y0 = (tile / tiles_x) * kTile, X0 = (tile % tiles_x) * kTile;
   const int MN = a.M * a.N;
   constexpr int kPer = kTile * kTile / 256;   // pixels per thread
   float2 acc[kPer];
@@ -341,6 +341,14 @@ __global__ __launch_bounds__(256) void k_far_dobject(FarArgs a) {
     const int yy = Y0 + t / kTile, xx = X0 + t % kTile;
     if (yy < a.H && xx < a.W) a.dobj[(int64_t)yy * a.W + xx] = acc[i];
   }
+}
+
+// F4 + F5 in one launch: the first n_probe_blocks workgroups form dprobe, the
+// rest one canvas tile of dobject each (the two read the same dpsi and write
+// disjoint outputs).
+__global__ __launch_bounds__(256) void k_far_dgrad(FarArgs a, int n_probe_blocks) {
+  if ((int)blockIdx.x < n_probe_blocks) far_dprobe(a, blockIdx.x, n_probe_blocks);
+  else far_dobject(a, blockIdx.x - n_probe_blocks);
 }
 
 static int far_col_tile(int M, int N) {
@@ -442,14 +450,13 @@ extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const 
   } else {
     launch_cols<0>(a, L.C, stream);
   }
-  hipLaunchKernelGGL(k_far_loss_final, dim3(a.B), dim3(64), 0, stream, a);
   const int lines = a.N >= 1024 ? 1 : 1024 / a.N < a.M ? 1024 / a.N : a.M;
   const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
   hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), shm, stream, a, lines);
   const int mn = a.M * a.N;
-  hipLaunchKernelGGL(k_far_dprobe, dim3((mn + 255) / 256 < 1024 ? (mn + 255) / 256 : 1024), dim3(256), 0, stream, a);
+  const int probe_blocks = (mn + 255) / 256 < 1024 ? (mn + 255) / 256 : 1024;
   const int64_t tiles = (int64_t)((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
-  hipLaunchKernelGGL(k_far_dobject, dim3((unsigned)tiles), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_far_dgrad, dim3((unsigned)(probe_blocks + tiles)), dim3(256), 0, stream, a, probe_blocks);
   return check_launch("farfield_fwd_bwd");
 }
 

@@ -1,6 +1,6 @@
 // Entry points of the fused hash-grid encode + complex-exponential MLP
-// (gfx950): argument checks, workspace layout, the two-stage reduction of the
-// per-workgroup gradient partials, and the MLP over given features.
+// (gfx950): argument checks, workspace layout, the fixed-order reduction of
+// the per-workgroup gradient partials, and the MLP over given features.
 //
 // Reference: ComplexINRModel2D.forward -> HashGrid (models/complex_inr.py:279,
 // 294), ComplexMLP.net (models/complex_inr.py:57-78, 313):
@@ -15,14 +15,16 @@
 // The backward kernels accumulate, over the points, P = sum g_w x^H and
 // q = sum g_w per hidden layer (g_w = g_z conj(z), the pre-activation
 // gradient g_v = -i w g_w without its factor) and P0 = sum Im(g_z conj(z0)) h^T,
-// q0 = sum Im(g_z conj(z0)) for the first layer.  The reduce stage turns the
+// q0 = sum Im(g_z conj(z0)) for the first layer.  The finish kernel turns the
 // totals into the parameter gradients, dA = -i w P, dc = -i w q,
-// dW0 = w0 P0, db0 = w0 q0, and forms the frequency gradients from the
+// dW0 = w0 P0, db0 = w0 q0.  The frequency gradients follow from the
 // homogeneity of each layer in its weights (v = A x + c, so
 // sum g_w conj(v) = <conj(A), P> + <conj(c), q>):
 //   dw = Im(sum conj(A) P + sum conj(c) q),  dw0 = sum W0 P0 + sum b0 q0,
 // which equals sum Im(g_w conj(v)) over the points without keeping the
-// pre-activations and holds for w = 0 as well.
+// pre-activations and holds for w = 0 as well; being linear in P and q, each
+// backward workgroup forms it for its own share (omega_terms) and the finish
+// totals the shares.
 #include "inr_common.h"
 
 namespace pty {
@@ -71,90 +73,62 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __r
   }
 }
 
-// Two-stage fixed-order reduction of the workgroup partials:
-// stage 1: partial2[c][t] = sum of rows [c*R, (c+1)*R) of partial[.][t];
-// stage 2: grad[t] += sum over c of partial2[c][t].
-constexpr int kReduceChunks = 32;
-
-__global__ void k_inr_grad_reduce1(const float* __restrict__ partial, int nblocks, int total,
-                                   float* __restrict__ partial2) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = blockIdx.y;
-  if (t >= total) return;
-  const int R = (nblocks + kReduceChunks - 1) / kReduceChunks;
-  const int b0 = c * R, b1 = min(nblocks, b0 + R);
-  float acc = 0.f;
-  for (int b = b0; b < b1; ++b) acc += partial[(int64_t)b * total + t];
-  partial2[(int64_t)c * total + t] = acc;
-}
-
-// Stage 2: totals t of the partials (P0, q0, P_k, q_k, g_wo, g_bo) into the
-// parameter gradients, plus each element's term of the frequency gradient
-// of its layer (om_terms, summed by k_inr_omega):
-//   w0 / b0 : += w0 t;        term W0 t / b0 t
-//   A_k / c_k (complex pairs (re, im)): += -i w_k t, i.e. (w Im t, -w Re t);
-//             term of the real slot -Im(W) Re(t), of the imaginary slot Re(W) Im(t)
+// Fixed-order reduction of the workgroup partials into the parameter
+// gradients, one launch: workgroup c covers the 32 gradient slots
+// [32c, 32c + 32); its 8 row groups of 32 threads each sum the partials of
+// workgroups rg, rg + 8, ... (in order), then the 8 group sums are added in
+// group order.  With total t of a slot:
+//   w0 / b0 : += w0 t
+//   A_k / c_k (complex pairs (re, im)): += -i w_k t, i.e. (w Im t, -w Re t)
+//   omega_k : += t   (each workgroup stored its frequency terms in these slots)
 //   wo / bo : += t
+constexpr int kFinishCols = 32;
+constexpr int kFinishGroups = 8;
+
 template <int H>
-__global__ void k_inr_grad_reduce(const float* __restrict__ partial2, PtyMlp m, PtyMlp gm,
-                                  float* __restrict__ om_terms) {
+__global__ __launch_bounds__(256) void k_inr_grad_finish(const float* __restrict__ partial, int nblocks, PtyMlp m,
+                                                         PtyMlp gm) {
   using GL = GradLayout<H>;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= GL::total) return;
-  auto total_of = [&](int e) {
-    float acc = 0.f;
-    for (int c = 0; c < kReduceChunks; ++c) acc += partial2[(int64_t)c * GL::total + e];
-    return acc;
-  };
-  const float acc = total_of(t);
-  float term = 0.f;
+  __shared__ float sums[kFinishGroups][kFinishCols];
+  __shared__ float tot[kFinishCols];
+  const int c = threadIdx.x & (kFinishCols - 1), rg = threadIdx.x / kFinishCols;
+  const int t = blockIdx.x * kFinishCols + c;
+  float acc = 0.f;
+  if (t < GL::total) {
+#pragma unroll 8
+    for (int b = rg; b < nblocks; b += kFinishGroups) acc += partial[(int64_t)b * GL::total + t];
+  }
+  sums[rg][c] = acc;
+  __syncthreads();
+  if (rg == 0) {
+    float s = sums[0][c];
+#pragma unroll
+    for (int r = 1; r < kFinishGroups; ++r) s += sums[r][c];
+    tot[c] = s;
+  }
+  __syncthreads();
+  if (rg != 0 || t >= GL::total) return;
+  const float a = tot[c];
   if (t < GL::b0) {
     const int o = t >> 5, f = t & 31;
-    if (f < gm.n_in) {
-      gm.w0[o * gm.n_in + f] += *m.omega[0] * acc;
-      term = m.w0[o * m.n_in + f] * acc;
-    }
+    if (f < gm.n_in) gm.w0[o * gm.n_in + f] += *m.omega[0] * a;
   } else if (t < GL::A(0)) {
-    gm.b0[t - GL::b0] += *m.omega[0] * acc;
-    term = m.b0[t - GL::b0] * acc;
+    gm.b0[t - GL::b0] += *m.omega[0] * a;
   } else if (t < GL::om) {
     const int k = (t - GL::A(0)) / 2112, r = (t - GL::A(0)) % 2112;
     const float w = *m.omega[k + 1];
-    const float other = total_of(t ^ 1);          // A(k) and c(k) start at even offsets
+    const float other = tot[c ^ 1];          // A(k) and c(k) start at even offsets
     const bool re = (r & 1) == 0;
-    const float tr = re ? acc : other, ti = re ? other : acc;
-    const float* wsrc = r < 2048 ? m.wh[k] + (r & ~1) : m.bh[k] + ((r - 2048) & ~1);
+    const float tr = re ? a : other, ti = re ? other : a;
     float* gdst = r < 2048 ? gm.wh[k] + r : gm.bh[k] + (r - 2048);
     *gdst += re ? w * ti : -w * tr;
-    term = re ? -wsrc[1] * tr : wsrc[0] * ti;
   } else if (t < GL::wo) {
-    // frequency slots: formed by k_inr_omega
+    *gm.omega[t - GL::om] += a;
   } else if (t < GL::bo) {
-    gm.wo[t - GL::wo] += acc;
+    gm.wo[t - GL::wo] += a;
   } else {
-    gm.bo[t - GL::bo] += acc;
+    gm.bo[t - GL::bo] += a;
   }
-  om_terms[t] = term;
-}
-
-// Frequency gradients: block k sums the terms of layer k (layer 0: [0, A(0));
-// hidden layer k-1: [A(k-1), A(k-1) + 2112)) in a fixed order.
-template <int H>
-__global__ __launch_bounds__(256) void k_inr_omega(const float* __restrict__ om_terms, PtyMlp gm) {
-  using GL = GradLayout<H>;
-  const int k = blockIdx.x;
-  const int lo = k == 0 ? 0 : GL::A(k - 1);
-  const int hi = k == 0 ? GL::A(0) : GL::A(k - 1) + 2112;
-  __shared__ float red[256];
-  float acc = 0.f;
-  for (int e = lo + (int)threadIdx.x; e < hi; e += 256) acc += om_terms[e];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *gm.omega[k] += red[0];
 }
 
 static int validate_mlp(const PtyMlp* m, bool need_all) {
@@ -241,10 +215,9 @@ static size_t grad_total(const PtyMlp* mlp) {
   return (mlp->n_hidden == 1) ? GradLayout<1>::total : GradLayout<2>::total;
 }
 
-// per-block partials, the chunk sums of the first reduce stage and the
-// frequency-gradient terms
+// per-workgroup gradient partials
 static size_t partial_bytes(const PtyMlp* mlp, const PtyPoints* pts) {
-  return align256(sizeof(float) * grad_total(mlp) * (size_t)(bwd_blocks(pts->n) + kReduceChunks + 1));
+  return align256(sizeof(float) * grad_total(mlp) * (size_t)bwd_blocks(pts->n));
 }
 
 extern "C" size_t pty_inr_bwd_workspace(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pts) {
@@ -270,8 +243,6 @@ static int check_bwd_args(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* 
 // [n_levels][n] c64, then the table-gradient scratch.
 struct BwdWork {
   float* partial;
-  float* partial2;
-  float* om_terms;
   float2* gfeat;
   char* tab_work;
   int blocks, total;
@@ -283,8 +254,6 @@ static BwdWork bwd_work(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* pt
   b.blocks = bwd_blocks(pts->n);
   b.total = (int)grad_total(mlp);
   b.partial = (float*)w;
-  b.partial2 = b.partial + (size_t)b.blocks * b.total;
-  b.om_terms = b.partial2 + (size_t)kReduceChunks * b.total;
   b.gfeat = (float2*)(w + partial_bytes(mlp, pts));
   b.tab_work = (char*)b.gfeat + align256(sizeof(float2) * (size_t)g->n_levels * (size_t)pts->n);
   return b;
@@ -315,17 +284,13 @@ extern "C" int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const Pty
   PTY_REQUIRE(dtable, PTY_EINVAL, "inr_bwd: null pointer");
   if (pts->n == 0) return PTY_OK;
   const BwdWork b = bwd_work(g, mlp, pts, work);
-  hipLaunchKernelGGL(k_inr_grad_reduce1, dim3((b.total + 255) / 256, kReduceChunks), dim3(256), 0, stream,
-                     (const float*)b.partial, b.blocks, b.total, b.partial2);
-  if (mlp->n_hidden == 1) {
-    hipLaunchKernelGGL(k_inr_grad_reduce<1>, dim3((b.total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)b.partial2, *mlp, *grad, b.om_terms);
-    hipLaunchKernelGGL(k_inr_omega<1>, dim3(2), dim3(256), 0, stream, (const float*)b.om_terms, *grad);
-  } else {
-    hipLaunchKernelGGL(k_inr_grad_reduce<2>, dim3((b.total + 255) / 256), dim3(256), 0, stream,
-                       (const float*)b.partial2, *mlp, *grad, b.om_terms);
-    hipLaunchKernelGGL(k_inr_omega<2>, dim3(3), dim3(256), 0, stream, (const float*)b.om_terms, *grad);
-  }
+  const dim3 grid((b.total + kFinishCols - 1) / kFinishCols);
+  if (mlp->n_hidden == 1)
+    hipLaunchKernelGGL(k_inr_grad_finish<1>, grid, dim3(kFinishCols * kFinishGroups), 0, stream,
+                       (const float*)b.partial, b.blocks, *mlp, *grad);
+  else
+    hipLaunchKernelGGL(k_inr_grad_finish<2>, grid, dim3(kFinishCols * kFinishGroups), 0, stream,
+                       (const float*)b.partial, b.blocks, *mlp, *grad);
   rc = check_launch("inr_bwd_finish");
   if (rc) return rc;
   return launch_table_bwd(*g, *pts, b.gfeat, dtable, b.tab_work, stream);

@@ -250,6 +250,54 @@ struct GradLayout {
   static constexpr int total = bo + 2;
 };
 
+__device__ __forceinline__ float wave_sum(float v);
+
+// Frequency-gradient terms of one workgroup's complete partial `red`
+// (GradLayout<H>, in LDS): layer 0 sum W0 P0 + sum b0 q0; hidden layer k the
+// sum over its (re, im) slots of -Im(W) Re(.) and Re(W) Im(.), W = A_k or c_k
+// (the homogeneity identity applied to this workgroup's share of the sums).
+// Stored into the partial's frequency slots red[om + k], which the gradient
+// finish then totals over the workgroups like every other slot.  All threads
+// of the workgroup call it (at most 16 waves); it ends with a barrier.
+template <int H>
+__device__ void omega_terms(float* red, const PtyMlp& m) {
+  using GL = GradLayout<H>;
+  __shared__ float part[H + 1][16];
+  float acc[H + 1];
+#pragma unroll
+  for (int k = 0; k <= H; ++k) acc[k] = 0.f;
+  for (int e = threadIdx.x; e < GL::om; e += blockDim.x) {
+    const float v = red[e];
+    if (e < GL::b0) {
+      const int o = e >> 5, f = e & 31;
+      if (f < m.n_in) acc[0] += m.w0[o * m.n_in + f] * v;
+    } else if (e < GL::A(0)) {
+      acc[0] += m.b0[e - GL::b0] * v;
+    } else {
+      const int k = (e - GL::A(0)) / 2112, r = (e - GL::A(0)) % 2112;
+#pragma unroll
+      for (int kk = 0; kk < H; ++kk) {
+        if (kk != k) continue;
+        const float* W = r < 2048 ? m.wh[kk] + (r & ~1) : m.bh[kk] + ((r - 2048) & ~1);
+        acc[kk + 1] += (r & 1) ? W[0] * v : -W[1] * v;
+      }
+    }
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k <= H; ++k) {
+    const float sw = wave_sum(acc[k]);
+    if (lane == 0) part[k][wave] = sw;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x <= H) {
+    float sum = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += part[threadIdx.x][w];
+    red[GL::om + threadIdx.x] = sum;
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

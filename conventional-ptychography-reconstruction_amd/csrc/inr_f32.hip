@@ -543,6 +543,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     }
     __syncthreads();
   }
+  omega_terms<H>(red, m);
   for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
 }
 

@@ -680,6 +680,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
     }
     __syncthreads();
   }
+  omega_terms<H>(red, m);
   for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
 }
 
