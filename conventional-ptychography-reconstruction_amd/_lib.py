@@ -12,7 +12,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libptycho_hip.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_LEVELS = 16
 MAX_HIDDEN = 4
 
@@ -71,6 +71,8 @@ _SIGNATURES = {
                                   c_void_p]),
     "pty_inr_bwd_finish": (c_int32, [P(PtyGrid), P(PtyMlp), P(PtyMlp), c_void_p, P(PtyPoints), c_void_p, c_size_t,
                                      c_void_p]),
+    "pty_inr_bwd_finish_adam": (c_int32, [P(PtyGrid), P(PtyMlp), P(PtyMlp), c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_float, c_float, c_float, P(PtyPoints), c_void_p, c_size_t, c_void_p]),
     "pty_points_plan_bytes": (c_size_t, [P(PtyGrid), P(PtyPoints)]),
     "pty_points_feature_floats": (c_size_t, [c_int64]),
     "pty_points_plan_build": (c_int32, [P(PtyGrid), P(PtyPoints), c_void_p, c_size_t, c_void_p]),

@@ -98,33 +98,21 @@ __global__ void k_sum_into(const float* __restrict__ partial, int m, float* __re
   if (threadIdx.x == 0) out[0] += red[0];
 }
 
-// torch.optim.Adam (amsgrad=False, maximize=False, weight_decay=0) on real
-// views: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
-// p -= lr/bc1 * m / (sqrt(v)/sqrt(bc2) + eps), bc_i = 1 - b_i^step.
-// hyper = [lr, step] on the device; pty_adam_tick advances step.  float4
+// torch.optim.Adam over real views (adam_elem in common.h); hyper = [lr, step]
+// on the device; pty_adam_tick advances step.  float4
 // lanes over n4 = n/4 groups (the flat buffer is padded to 4 floats); the
 // consumed gradient is cleared for the next step when clear != 0.
 __global__ void k_adam(float4* __restrict__ p, float4* __restrict__ g, float4* __restrict__ m,
                        float4* __restrict__ v, int64_t n4, const float* __restrict__ hyper, float b1, float b2,
                        float eps, int clear) {
-  const float lr = hyper[0], t = hyper[1];
-  const float bc1 = 1.f - powf(b1, t);
-  const float sbc2 = sqrtf(1.f - powf(b2, t));
-  const float step = lr / bc1;
+  const AdamCoef c = adam_coef(hyper, b1, b2, eps);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
     const float4 gi = g[i];
     float4 mi = m[i], vi = v[i], pi = p[i];
-    float* gp = (float*)&gi;
-    float* mp = (float*)&mi;
-    float* vp = (float*)&vi;
-    float* pp = (float*)&pi;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      mp[k] = b1 * mp[k] + (1.f - b1) * gp[k];
-      vp[k] = b2 * vp[k] + (1.f - b2) * gp[k] * gp[k];
-      const float denom = sqrtf(vp[k]) / sbc2 + eps;
-      pp[k] = pp[k] - step * (mp[k] / denom);
-    }
+    adam_elem(pi.x, mi.x, vi.x, gi.x, c);
+    adam_elem(pi.y, mi.y, vi.y, gi.y, c);
+    adam_elem(pi.z, mi.z, vi.z, gi.z, c);
+    adam_elem(pi.w, mi.w, vi.w, gi.w, c);
     m[i] = mi;
     v[i] = vi;
     p[i] = pi;
@@ -136,17 +124,14 @@ __global__ void k_adam(float4* __restrict__ p, float4* __restrict__ g, float4* _
 __global__ void k_adam_tail(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
                             float* __restrict__ v, int64_t i0, int64_t n, const float* __restrict__ hyper, float b1,
                             float b2, float eps, int clear) {
-  const float lr = hyper[0], t = hyper[1];
-  const float bc1 = 1.f - powf(b1, t);
-  const float sbc2 = sqrtf(1.f - powf(b2, t));
+  const AdamCoef c = adam_coef(hyper, b1, b2, eps);
   const int64_t i = i0 + threadIdx.x;
   if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  float pi = p[i], mi = m[i], vi = v[i];
+  adam_elem(pi, mi, vi, g[i], c);
   m[i] = mi;
   v[i] = vi;
-  p[i] = p[i] - (lr / bc1) * (mi / (sqrtf(vi) / sbc2 + eps));
+  p[i] = pi;
   if (clear) g[i] = 0.f;
 }
 

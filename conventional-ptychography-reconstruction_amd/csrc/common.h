@@ -46,6 +46,52 @@ struct KSum {
   __device__ __forceinline__ float value() const { return s + c; }
 };
 
+// ---------------------------------------------------------------- Adam
+// torch.optim.Adam (amsgrad=False, maximize=False, weight_decay=0) on one
+// real element: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+// p -= lr/bc1 * m / (sqrt(v)/sqrt(bc2) + eps), bc_i = 1 - b_i^step, with
+// hyper = [lr, step] on the device.  Shared by the flat Adam kernel and the
+// table-gradient kernels that apply the update in place.
+struct AdamCoef {
+  float b1, b2, eps, sbc2, step;
+};
+__device__ __forceinline__ AdamCoef adam_coef(const float* hyper, float b1, float b2, float eps) {
+  const float lr = hyper[0], t = hyper[1];
+  const float bc1 = 1.f - powf(b1, t);
+  AdamCoef c;
+  c.b1 = b1;
+  c.b2 = b2;
+  c.eps = eps;
+  c.sbc2 = sqrtf(1.f - powf(b2, t));
+  c.step = lr / bc1;
+  return c;
+}
+__device__ __forceinline__ void adam_elem(float& p, float& m, float& v, float g, const AdamCoef& c) {
+  m = c.b1 * m + (1.f - c.b1) * g;
+  v = c.b2 * v + (1.f - c.b2) * g * g;
+  const float denom = sqrtf(v) / c.sbc2 + c.eps;
+  p = p - c.step * (m / denom);
+}
+
+// A table's Adam state for the table-gradient kernels (p == nullptr: store
+// the gradient into dtable instead): p / m / v are the table's parameter and
+// moment buffers, [n_entries, 2] like the table.
+struct TableAdam {
+  float2* p;
+  float2* m;
+  float2* v;
+  const float* hyper;
+  float b1, b2, eps;
+};
+__device__ __forceinline__ void table_adam_update(const TableAdam& a, const AdamCoef& c, uint32_t t, float2 g) {
+  float2 p = a.p[t], m = a.m[t], v = a.v[t];
+  adam_elem(p.x, m.x, v.x, g.x, c);
+  adam_elem(p.y, m.y, v.y, g.y, c);
+  a.p[t] = p;
+  a.m[t] = m;
+  a.v[t] = v;
+}
+
 // ---------------------------------------------------------------- FFT in LDS
 // Stockham autosort FFT (radix-4 passes, one radix-2 pass when log2 n is odd)
 // over `lines` independent lines held in LDS at a + line*stride.  Ping-pongs

@@ -330,7 +330,7 @@ __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanP
 // skipped here and handled by k_table_hashed).
 __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, PlanPtrs P,
                                                      const float2* __restrict__ A, uint32_t n_dense_entries,
-                                                     float2* __restrict__ dtable) {
+                                                     float2* __restrict__ dtable, TableAdam ad) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_dense_entries) return;
   int l = 0;
@@ -353,7 +353,9 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
       acc.y += v.y;
     }
   }
-  if (acc.x != 0.f || acc.y != 0.f) {
+  if (ad.p) {
+    table_adam_update(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), t, acc);
+  } else if (acc.x != 0.f || acc.y != 0.f) {
     float2 d = dtable[t];
     d.x += acc.x;
     d.y += acc.y;
@@ -492,7 +494,7 @@ constexpr int kHashWin = 2048;
 
 __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                       const float2* __restrict__ gfeat,
-                                                      float2* __restrict__ dtable) {
+                                                      float2* __restrict__ dtable, TableAdam ad) {
   __shared__ int s_off[257];
   __shared__ float2 s_val[kHashWin];
   int l;
@@ -532,7 +534,9 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
     }
     __syncthreads();
   }
-  if (tid < ne && (ax != 0.f || ay != 0.f)) {
+  if (tid < ne && ad.p) {
+    table_adam_update(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), g.offset[l] + e0 + tid, make_float2(ax, ay));
+  } else if (tid < ne && (ax != 0.f || ay != 0.f)) {
     const uint32_t t = g.offset[l] + e0 + tid;
     float2 d = dtable[t];
     d.x += ax;
@@ -596,8 +600,9 @@ size_t table_bwd_workspace(const PtyGrid& g, const PtyPoints& pts) {
 }
 
 int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat, float* dtable, char* work,
-                     hipStream_t stream) {
+                     hipStream_t stream, const TableAdam& ad) {
   if (pts.coords) {
+    PTY_REQUIRE(!ad.p, PTY_EUNSUP, "table gradient with Adam in place needs grid points (a plan), not coordinates");
     hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)((pts.n + 255) / 256), g.n_levels), dim3(256), 0, stream, g,
                        pts, gfeat, dtable);
     return check_launch("table_bwd_scatter");
@@ -623,10 +628,10 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
   }
   if (n_dense_entries > 0)
     hipLaunchKernelGGL(k_table_dense, dim3((n_dense_entries + 255) / 256), dim3(256), 0, stream, g, L, P,
-                       (const float2*)A, n_dense_entries, (float2*)dtable);
+                       (const float2*)A, n_dense_entries, (float2*)dtable, ad);
   if (hashed_chunks > 0)
     hipLaunchKernelGGL(k_table_hashed, dim3(hashed_chunks), dim3(256), 0, stream, g, L, pts, P, gfeat,
-                       (float2*)dtable);
+                       (float2*)dtable, ad);
   return check_launch("table_bwd");
 }
 

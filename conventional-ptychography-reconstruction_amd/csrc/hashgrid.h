@@ -17,8 +17,10 @@ int validate_grid(const PtyGrid* g);
 // table gradient from level-major feature gradients gfeat[l][p] (hashbwd.hip)
 size_t points_plan_bytes(const PtyGrid& g, const PtyPoints& pts);
 size_t table_bwd_workspace(const PtyGrid& g, const PtyPoints& pts);
+// With ad.p set, each entry's gradient goes straight into the table's Adam
+// update (ad) instead of being added to dtable.
 int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat, float* dtable, char* work,
-                     hipStream_t stream);
+                     hipStream_t stream, const TableAdam& ad);
 
 struct LevelPos {
   uint32_t gx, gy;

@@ -275,13 +275,14 @@ extern "C" int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float*
   return check_launch("inr_bwd_mlp");
 }
 
-extern "C" int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
-                                  const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream) {
+static int inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
+                          const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream,
+                          const TableAdam& ad) {
   int rc = check_bwd_args(g, mlp, pts, work, work_bytes);
   if (!rc) rc = validate_mlp(grad, true);
   if (rc) return rc;
   PTY_REQUIRE(grad->n_in == mlp->n_in && grad->n_hidden == mlp->n_hidden, PTY_EINVAL, "grad descriptor mismatch");
-  PTY_REQUIRE(dtable, PTY_EINVAL, "inr_bwd: null pointer");
+  PTY_REQUIRE(dtable || ad.p, PTY_EINVAL, "inr_bwd: null pointer");
   if (pts->n == 0) return PTY_OK;
   const BwdWork b = bwd_work(g, mlp, pts, work);
   const dim3 grid((b.total + kFinishCols - 1) / kFinishCols);
@@ -293,7 +294,27 @@ extern "C" int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const Pty
                        (const float*)b.partial, b.blocks, *mlp, *grad);
   rc = check_launch("inr_bwd_finish");
   if (rc) return rc;
-  return launch_table_bwd(*g, *pts, b.gfeat, dtable, b.tab_work, stream);
+  return launch_table_bwd(*g, *pts, b.gfeat, dtable, b.tab_work, stream, ad);
+}
+
+extern "C" int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
+                                  const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream) {
+  TableAdam none{};
+  PTY_REQUIRE(dtable, PTY_EINVAL, "inr_bwd: null pointer");
+  return inr_bwd_finish(g, mlp, grad, dtable, pts, work, work_bytes, stream, none);
+}
+
+extern "C" int pty_inr_bwd_finish_adam(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* table,
+                                       float* exp_avg, float* exp_avg_sq, const float* hyper, float beta1,
+                                       float beta2, float eps, const PtyPoints* pts, void* work, size_t work_bytes,
+                                       hipStream_t stream) {
+  PTY_REQUIRE(table && exp_avg && exp_avg_sq && hyper, PTY_EINVAL, "inr_bwd_finish_adam: null pointer");
+  PTY_REQUIRE(((((uintptr_t)table) | ((uintptr_t)exp_avg) | ((uintptr_t)exp_avg_sq)) & 7) == 0, PTY_EINVAL,
+              "inr_bwd_finish_adam: table and moments must be 8-byte aligned");
+  PTY_REQUIRE(pts && pts->n > 0 && !pts->coords, PTY_EINVAL,
+              "inr_bwd_finish_adam: needs a non-empty grid of points (axis vectors)");
+  const TableAdam ad{(float2*)table, (float2*)exp_avg, (float2*)exp_avg_sq, hyper, beta1, beta2, eps};
+  return inr_bwd_finish(g, mlp, grad, nullptr, pts, work, work_bytes, stream, ad);
 }
 
 extern "C" int pty_inr_bwd(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, const float* table, float* dtable,

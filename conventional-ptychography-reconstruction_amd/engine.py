@@ -18,7 +18,9 @@ Per step (batch of positions ``batch_idx`` on the device):
   dobject += sparsity_weight * d Haar(object)   (only when the weight is set)
   dprobe  = normalise^T(dprobe_n);  residual^T;  dsmall = Upsample^T(dobject)
   grads   = INR^T (recomputed activations, hash-table atomics)
-  Adam over all trainable parameters (one flat launch)
+  Adam over all trainable parameters: the hash tables inside their
+  table-gradient kernels (the table gradient is never stored), the rest in
+  one flat launch
 Parameters of the model are rebound as views into one flat fp32 buffer so
 that Adam is a single elementwise kernel with torch.optim.Adam semantics.
 """
@@ -62,6 +64,7 @@ class FlatParams:
         self.data = torch.zeros(total, dtype=torch.float32, device=device)
         self.grad = torch.zeros(total, dtype=torch.float32, device=device)
         self.grad_views = {}
+        self.offsets = {}
         off = 0
         for p in order:
             n = _real_numel(p)
@@ -77,10 +80,15 @@ class FlatParams:
                 gview = gv.view(p.shape)
             p.data = pview
             self.grad_views[id(p)] = gview
+            self.offsets[id(p)] = off
             off += _padded(n)
 
     def g(self, p):
         return self.grad_views[id(p)]
+
+    def span(self, p):
+        """(offset, floats) of p in the flat buffer."""
+        return self.offsets[id(p)], _real_numel(p)
 
 
 class FusedReconstruction:
@@ -88,7 +96,7 @@ class FusedReconstruction:
 
     def __init__(self, model: ComplexINRModel2D, stack_mnf: torch.Tensor, tlY, tlX, loss_type="smooth_L1_loss",
                  lr=1e-3, batch_size=10, lr_decay_step=10000, lr_decay_gamma=0.1, betas=(0.9, 0.999),
-                 eps=1e-8, sparsity_weight=0.0):
+                 eps=1e-8, sparsity_weight=0.0, fuse_table_adam=True):
         dev = model.xy_hash_encoder.params.device
         if dev.type != "cuda":
             raise RuntimeError("FusedReconstruction needs the model on the GPU")
@@ -130,6 +138,21 @@ class FusedReconstruction:
         self.m = torch.zeros(self.flat.n_train, dtype=torch.float32, device=dev)
         self.v = torch.zeros(self.flat.n_train, dtype=torch.float32, device=dev)
         self.hyper = torch.tensor([self.lr0, 0.0], dtype=torch.float32, device=dev)
+        # The hash tables (almost every parameter) take their Adam step inside
+        # the table-gradient kernels, which then never store the table
+        # gradient; the flat Adam launch covers the rest.  Needs both tables
+        # trainable, first in the flat buffer.
+        tabs = [model.xy_hash_encoder.params, model.probe_hash_encoder.params]
+        self.fuse_table_adam = bool(fuse_table_adam) and all(t.requires_grad for t in tabs)
+        self._adam_from = 0
+        if self.fuse_table_adam:
+            (o0, n0), (o1, n1) = self.flat.span(tabs[0]), self.flat.span(tabs[1])
+            assert o0 == 0 and o1 == _padded(n0)
+            self._adam_from = o1 + _padded(n1)
+            self._tab_adam = {
+                "obj": (tabs[0].data_ptr(), self.m[o0:o0 + n0].data_ptr(), self.v[o0:o0 + n0].data_ptr()),
+                "prb": (tabs[1].data_ptr(), self.m[o1:o1 + n1].data_ptr(), self.v[o1:o1 + n1].data_ptr()),
+            }
         self._lr_epoch = 0
 
         prec = model.mlp_precision
@@ -277,13 +300,21 @@ class FusedReconstruction:
               self.obj_up.numel(), self.obj.data_ptr(), s)
         main.wait_stream(side)
 
-    def _inr_bwd(self, which, grid, mlp, gmlp, table, dtable, pdesc, dout, ws, stream):
+    def _inr_bwd(self, which, grid, mlp, gmlp, table, dtable, pdesc, dout, ws, stream, adam=False):
         """INR backward on ``stream``; with timing on for '<which>_mlp_bwd' the
         MLP stage is bracketed by events on its own (the reduce and table
-        gradient follow)."""
+        gradient follow).  adam: the table takes its Adam step in place."""
         C = _lib.call
         s = stream.cuda_stream
         name = which + "_mlp_bwd"
+        if adam:
+            self._mark(name, True, stream)
+            C("pty_inr_bwd_mlp", grid, mlp, table.data_ptr(), pdesc, dout.data_ptr(), ws.data_ptr(), ws.numel(), s)
+            self._mark(name, False, stream)
+            p, m, v = self._tab_adam[which]
+            C("pty_inr_bwd_finish_adam", grid, mlp, gmlp, p, m, v, self.hyper.data_ptr(), self.beta1, self.beta2,
+              self.eps, pdesc, ws.data_ptr(), ws.numel(), s)
+            return
         if self._timers is None or name not in self._timers:
             self._mark(which + "_inr_bwd", True, stream)
             C("pty_inr_bwd", grid, mlp, gmlp, table.data_ptr(), dtable.data_ptr(), pdesc, dout.data_ptr(),
@@ -303,8 +334,12 @@ class FusedReconstruction:
         m = self.model
         b = batch_idx.numel()
         d, ws_far = self.far[b]
+        fused = update and self.fuse_table_adam
         if not self._grad_clear:
             C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
+        if fused:
+            # the table-gradient kernels apply Adam with this step's count
+            C("pty_adam_tick", self.hyper.data_ptr(), s)
         self.forward_fields()
         self._mark("farfield", True, main)
         C("pty_farfield_fwd_bwd", d, self.obj.data_ptr(), self.probe_n.data_ptr(), self.stack.data_ptr(),
@@ -331,9 +366,9 @@ class FusedReconstruction:
               self.dprobe.numel(), self.dprb_mlp.data_ptr(), self.flat.g(m.k_probe).data_ptr(),
               self.ws_res_p.data_ptr(), self.ws_res_p.numel(), p)
         self._inr_bwd("prb", self.prb_grid, self.prb_mlp, self.prb_gmlp, self.prb_tab,
-                      self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, side)
+                      self.prb_dtable, self.prb_pdesc, self.dprb_mlp, self.ws_prb, side, adam=fused)
         self._inr_bwd("obj", self.obj_grid, self.obj_mlp, self.obj_gmlp, self.obj_tab,
-                      self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, main)
+                      self.obj_dtable, self.obj_pdesc, self.dsmall, self.ws_obj, main, adam=fused)
         if self.fp8_states:
             st_o, st_p = self.fp8_states
             C("pty_fp8_roll", st_o.data_ptr(), s)
@@ -342,9 +377,12 @@ class FusedReconstruction:
         if update:
             # Adam clears the trainable gradients it consumes; frozen ones are
             # cleared with them when they exist
-            C("pty_adam_tick", self.hyper.data_ptr(), s)
-            C("pty_adam", self.flat.data.data_ptr(), self.flat.grad.data_ptr(), self.m.data_ptr(),
-              self.v.data_ptr(), self.flat.n_train, self.hyper.data_ptr(), self.beta1, self.beta2, self.eps, 1, s)
+            a0 = self._adam_from if fused else 0
+            if not fused:
+                C("pty_adam_tick", self.hyper.data_ptr(), s)
+            C("pty_adam", self.flat.data.data_ptr() + 4 * a0, self.flat.grad.data_ptr() + 4 * a0,
+              self.m.data_ptr() + 4 * a0, self.v.data_ptr() + 4 * a0, self.flat.n_train - a0,
+              self.hyper.data_ptr(), self.beta1, self.beta2, self.eps, 1, s)
             n_frozen = self.flat.grad.numel() - self.flat.n_train
             if n_frozen:
                 C("pty_fill", self.flat.grad.data_ptr() + 4 * self.flat.n_train, n_frozen, 0.0, s)

@@ -32,7 +32,7 @@ extern "C" {
 
 typedef struct ihipStream_t* hipStream_t;
 
-#define PTY_ABI_VERSION 5
+#define PTY_ABI_VERSION 6
 #define PTY_MAX_LEVELS 16
 #define PTY_MAX_HIDDEN 4
 #define PTY_FP8_STATE_FLOATS 520
@@ -175,6 +175,17 @@ int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float* table, con
                     const float* dout, void* work, size_t work_bytes, hipStream_t stream);
 int pty_inr_bwd_finish(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* dtable,
                        const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream);
+/* pty_inr_bwd_finish with the table's Adam step (torch.optim.Adam.step,
+ * Recovery.py:321, over the hash-table parameter) applied in place: each
+ * table entry's gradient goes straight into its update of table / exp_avg /
+ * exp_avg_sq ([n_entries, 2] f32 each, 8-byte aligned), with hyper =
+ * [lr, step] as for pty_adam (advance step with pty_adam_tick first), so the
+ * table gradient is never stored.  Same bits as pty_inr_bwd_finish followed by
+ * pty_adam over the table.  Needs axis-generated points (a plan) with n > 0.
+ * The MLP gradients still accumulate into *grad. */
+int pty_inr_bwd_finish_adam(const PtyGrid* g, const PtyMlp* mlp, const PtyMlp* grad, float* table,
+                            float* exp_avg, float* exp_avg_sq, const float* hyper, float beta1, float beta2,
+                            float eps, const PtyPoints* pts, void* work, size_t work_bytes, hipStream_t stream);
 /* Floats of the feature cache of n points (32 per point, n rounded up to a
  * multiple of 32). */
 size_t pty_points_feature_floats(int64_t n);

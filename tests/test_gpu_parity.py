@@ -608,6 +608,31 @@ def test_fused_step_is_deterministic(pkg, dev):
     assert torch.equal(engines[0].flat.data, engines[1].flat.data)
 
 
+def test_table_adam_in_place_matches_flat_adam(pkg, dev):
+    """The table-gradient kernels' in-place Adam step gives the bits of
+    storing the table gradient and running the flat Adam over it (same
+    element update, same gradient), over steps with and without the update."""
+    inst = pkg.synthetic.make_named("small", seed=8)
+    M = inst.probe.shape[0]
+    pinit = torch.from_numpy(inst.probe.astype(np.complex64))
+    engines = []
+    for fuse in (True, False):
+        m = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=17)
+        e = pkg.FusedReconstruction(m, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX, batch_size=5,
+                                    fuse_table_adam=fuse)
+        e.set_lr(1e-3)
+        engines.append(e)
+    assert engines[0].fuse_table_adam and not engines[1].fuse_table_adam
+    for it, b in enumerate(([0, 1, 2, 3, 4], [10, 11, 12, 13, 14], [30, 5, 17, 2, 21], [7, 8, 9, 20, 22])):
+        bt = torch.tensor(b, dtype=torch.int32, device=dev)
+        losses = [e.step(bt, update=(it != 1)).clone() for e in engines]
+        assert torch.equal(losses[0], losses[1]), it
+    f, u = engines
+    assert torch.equal(f.flat.data, u.flat.data)
+    assert torch.equal(f.m, u.m) and torch.equal(f.v, u.v)
+    assert torch.equal(f.hyper, u.hyper)
+
+
 def _mlp_grads_vs_oracle(model, params):
     """(name, error) per MLP tensor.  Weights and biases: relative error (norm).
     omega: omega * dL/domega = Re<G_W, W> + Re<G_b, b> sums terms that cancel,
