@@ -191,31 +191,36 @@ __device__ __forceinline__ void layer0_16(const Lds16<H>& s, const float feat[8]
     for (int mb = 0; mb < 2; ++mb) u[mb] = mfma4(s.w0[(16 * mb + i) * WS36 + feat16(c, g)], feat[c], u[mb]);
 }
 
-// (vr, vi)[mb] = A_k z + c_k
+// (vr, vi)[mb] = A_k z + c_k, three real products per K step (Gauss):
+// T1 = Ar zr, T2 = Ai zi, T3 = (Ar + Ai)(zr + zi); vr = c_r + T1 - T2,
+// vi = c_i + T3 - T1 - T2.  Ar + Ai is formed from the two LDS values the
+// four-product form reads anyway.
 template <int H>
 __device__ __forceinline__ void hidden16(const Lds16<H>& s, int k, const float zr[8], const float zi[8], int i, int g,
                                          v4f vr[2], v4f vi[2]) {
+  v4f t1[2], t2[2], t3[2];
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      vr[mb][v] = s.cr[k][16 * mb + 4 * g + v];
-      vi[mb][v] = s.ci[k][16 * mb + 4 * g + v];
-    }
+  for (int mb = 0; mb < 2; ++mb) t1[mb] = t2[mb] = t3[mb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int st = 0; st < 8; ++st) {
     const int u = unit16(st, g);
-    const float nzi = -zi[st];
+    const float zs = zr[st] + zi[st];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
       const float a_r = s.ar[k][(16 * mb + i) * WS36 + u];
       const float a_i = s.ai[k][(16 * mb + i) * WS36 + u];
-      vr[mb] = mfma4(a_r, zr[st], vr[mb]);
-      vi[mb] = mfma4(a_i, zr[st], vi[mb]);
-      vr[mb] = mfma4(a_i, nzi, vr[mb]);
-      vi[mb] = mfma4(a_r, zi[st], vi[mb]);
+      t1[mb] = mfma4(a_r, zr[st], t1[mb]);
+      t2[mb] = mfma4(a_i, zi[st], t2[mb]);
+      t3[mb] = mfma4(a_r + a_i, zs, t3[mb]);
     }
   }
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      vr[mb][v] = s.cr[k][16 * mb + 4 * g + v] + (t1[mb][v] - t2[mb][v]);
+      vi[mb][v] = s.ci[k][16 * mb + 4 * g + v] + (t3[mb][v] - t1[mb][v] - t2[mb][v]);
+    }
 }
 
 __device__ __forceinline__ void act16(float w, const v4f vr[2], const v4f vi[2], float zr[8], float zi[8]) {
@@ -231,7 +236,7 @@ __device__ __forceinline__ void act16(float w, const v4f vr[2], const v4f vi[2],
 
 // ============================================================== forward
 template <int H>
-__global__ __launch_bounds__(64 * kFwd16Waves) void k_inr_fwd16(PtyGrid gr, PtyMlp m,
+__global__ __launch_bounds__(64 * kFwd16Waves, 4) void k_inr_fwd16(PtyGrid gr, PtyMlp m,
                                                                 const float2* __restrict__ table, PtyPoints pts,
                                                                 float2* __restrict__ out) {
   __shared__ Lds16<H> s;
