@@ -425,28 +425,30 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
           Pi[k][mb][nb] = mfma_pts(nar, xi, Pi[k][mb][nb]);
         }
       }
-      // t = A^H g_w (in units 16 mb + 4 g + v), g_x = -i w t
-      v4f txr[2], txi[2];
+      // t = A^H g_w (in units 16 mb + 4 g + v), g_x = -i w t; three real
+      // products (Gauss): T1 = Ar^T gwr, T2 = Ai^T gwi,
+      // T3 = (Ar - Ai)^T (gwr + gwi); Re t = T1 + T2, Im t = T3 - T1 + T2
+      v4f t1[2], t2[2], t3[2];
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb) txr[mb] = txi[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int mb = 0; mb < 2; ++mb) t1[mb] = t2[mb] = t3[mb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int u = unit16(q, g);
-        const float ngwr = -gwr[q];
+        const float gs = gwr[q] + gwi[q];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb) {
           const float a_r = s.ar[k][u * WS36 + 16 * mb + j];
           const float a_i = s.ai[k][u * WS36 + 16 * mb + j];
-          txr[mb] = mfma4(a_r, gwr[q], txr[mb]);
-          txr[mb] = mfma4(a_i, gwi[q], txr[mb]);
-          txi[mb] = mfma4(a_i, ngwr, txi[mb]);
-          txi[mb] = mfma4(a_r, gwi[q], txi[mb]);
+          t1[mb] = mfma4(a_r, gwr[q], t1[mb]);
+          t2[mb] = mfma4(a_i, gwi[q], t2[mb]);
+          t3[mb] = mfma4(a_r - a_i, gs, t3[mb]);
         }
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        gzr[q] = w * txi[q >> 2][q & 3];
-        gzi[q] = -w * txr[q >> 2][q & 3];
+        const float a1 = t1[q >> 2][q & 3], a2 = t2[q >> 2][q & 3], a3 = t3[q >> 2][q & 3];
+        gzr[q] = w * ((a3 - a1) + a2);
+        gzi[q] = -w * (a1 + a2);
         zr[q] = S.X[k][unit16(q, g) * PS + j];
         zi[q] = S.X[k][(32 + unit16(q, g)) * PS + j];
       }
