@@ -557,12 +557,12 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
 // ============================================================== launchers
 void launch_inr_fwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
                         hipStream_t stream) {
-  // four workgroups per CU (the kernel runs four waves per SIMD): each wave
+  // five workgroups per CU (the kernel fits five waves per SIMD): each wave
   // walks several tiles, so the weights are staged in LDS once per
   // workgroup rather than once per 64 points
   const int64_t tiles = (pts.n + 15) / 16;
   int64_t blocks = (tiles + kFwd16Waves - 1) / kFwd16Waves;
-  const int64_t cap = 4 * (int64_t)device_cu_count();
+  const int64_t cap = 5 * (int64_t)device_cu_count();
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   if (m.n_hidden == 1)
