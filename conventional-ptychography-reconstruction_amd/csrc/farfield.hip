@@ -103,14 +103,16 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   return r;
 }
 
-// F2: columns of pattern b = blockIdx.y, C columns per workgroup.
+// F2: columns of pattern b = blockIdx.y, C columns per workgroup of
+// kColThreads threads.
+constexpr int kColThreads = 512;
 // MODE 0: pointwise loss (smooth-L1 / Poisson) fused with dPsi and IFFT_M.
 // MODE 1: forward only, store natural-order Psi (in place) and I.
 // MODE 2: inverse only (FD path): IFFT_M of the stored dPsi, in place.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_far_cols(FarArgs a, int C) {
+__global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   extern __shared__ float2 smem[];
-  __shared__ float red[256];
+  __shared__ float red[kColThreads];
   const int M = a.M, N = a.N, S = M + 1;
   float2* tw = smem;
   float2* x = tw + M;
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(256) void k_far_cols(FarArgs a, int C) {
     f[c * S + r] = gv;
   }
   __syncthreads();
-  const float s = block_sum<256>(acc.value(), red);
+  const float s = block_sum<kColThreads>(acc.value(), red);
   if (tid == 0) a.partial[b * a.P + blockIdx.x] = s;
   float2* g2 = (f == x) ? y : x;
   float2* res = fft_lines<+1>(f, g2, M, a.lgM, C, S, tw, 1, tid, nt);
@@ -351,9 +353,11 @@ __global__ __launch_bounds__(256) void k_far_dgrad(FarArgs a, int n_probe_blocks
   else far_dobject(a, blockIdx.x - n_probe_blocks);
 }
 
+// Columns per workgroup of the column pass: up to 8 (measured best for
+// probes of 256 and 512), within 80 KB of LDS (two workgroups per CU).
 static int far_col_tile(int M, int N) {
-  int C = 16;
-  while (C > 1 && (size_t)(M + 2 * C * (M + 1)) * sizeof(float2) > 65536) C >>= 1;
+  int C = 8;
+  while (C > 1 && (size_t)(M + 2 * C * (M + 1)) * sizeof(float2) > 81920) C >>= 1;
   while (C > 1 && N % C) C >>= 1;
   return C;
 }
@@ -419,7 +423,7 @@ static void launch_rows_fwd(const FarArgs& a, hipStream_t s) {
 template <int MODE>
 static void launch_cols(const FarArgs& a, int C, hipStream_t s) {
   const size_t shm = sizeof(float2) * (size_t)(a.M + 2 * C * (a.M + 1));
-  hipLaunchKernelGGL(k_far_cols<MODE>, dim3(a.N / C, a.B), dim3(256), shm, s, a, C);
+  hipLaunchKernelGGL(k_far_cols<MODE>, dim3(a.N / C, a.B), dim3(kColThreads), shm, s, a, C);
 }
 
 }  // namespace pty

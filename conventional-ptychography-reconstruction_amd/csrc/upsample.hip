@@ -18,6 +18,17 @@
 // Twiddles (exp(-2 pi i k/m), the n-point transforms read every ds-th)
 // live in LDS when the line buffers leave room, else in a global table in
 // the workspace (filled at the start of every call).
+//
+// ds = 2 with n even (both workloads) has a cheaper exact form of U: the
+// embedded spectrum is q = s(k), the signed frequency (s = k below n/2,
+// k - n from n/2 on, Nyquist negative), so with m = 2n
+//   U x [2j]     = sum_k X_k e^{2 pi i k j / n}                 = n x_j
+//   U x [2j + 1] = sum_k X_k e^{2 pi i s(k) / m} e^{2 pi i k j / n}
+// i.e. the even samples are the input and the odd ones the half-sample
+// shift IFFT_n(phi X), phi_k = e^{2 pi i s(k)/m}; and V = U^H gives
+//   V g [j] = n g_{2j} + IFFT_n(conj(phi) FFT_n(g_odd))_j.
+// Two n-point transforms per line instead of an n- and an m-point one, and
+// line buffers of n instead of m (k_up2_rows).
 #include "common.h"
 
 namespace pty {
@@ -120,6 +131,63 @@ __global__ __launch_bounds__(256) void k_up_rows(const float2* __restrict__ in, 
   }
 }
 
+// ds = 2, n even: the half-sample-shift form above over `rows` lines, `lines`
+// per workgroup, with the same scale convention as k_up_rows.
+//   MODE 0: in rows of n -> out rows of m: out[2j] = n x_j, out[2j+1] = IFFT_n(phi FFT_n(x))_j
+//   MODE 1: in rows of m -> out rows of n: out[j] = n g_{2j} + IFFT_n(conj(phi) FFT_n(g_odd))_j
+// tw: exp(-2 pi i k/m), k < m (LDS copy when gtw is null).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_up2_rows(const float2* __restrict__ in, float2* __restrict__ out, FftPlan Pn,
+                                                  int rows, int lines, float scale, const float2* __restrict__ gtw) {
+  extern __shared__ float2 smem[];
+  const int n = Pn.n, m = 2 * n, h = n / 2;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const float2* tw;
+  float2* a;
+  if (gtw) {
+    tw = gtw;
+    a = smem;
+  } else {
+    fill_twiddles(smem, m, tid, nt);
+    tw = smem;
+    a = smem + m;
+  }
+  float2* b = a + lines * n;
+  const int r0 = blockIdx.x * lines;
+  const int nl = min(lines, rows - r0);
+  for (int t = tid; t < lines * n; t += nt) {
+    const int li = t / n, j = t - li * n;
+    float2 v = make_float2(0.f, 0.f);
+    if (li < nl) v = (MODE == 0) ? in[(int64_t)(r0 + li) * n + j] : in[(int64_t)(r0 + li) * m + 2 * j + 1];
+    a[t] = v;
+  }
+  __syncthreads();
+  float2* f = fft_any<-1>(a, b, Pn, lines, n, tw, 2, tid, nt);
+  for (int t = tid; t < lines * n; t += nt) {
+    const int k = t % n;
+    const int sk = k < h ? k : k - n;          // signed frequency
+    const float2 w = tw[sk < 0 ? sk + m : sk];  // exp(-2 pi i s/m) = conj(phi)
+    f[t] = cmul(f[t], (MODE == 0) ? make_float2(w.x, -w.y) : w);
+  }
+  __syncthreads();
+  float2* g = (f == a) ? b : a;
+  float2* res = fft_any<+1>(f, g, Pn, lines, n, tw, 2, tid, nt);
+  const float ns = (float)n * scale;
+  for (int t = tid; t < lines * n; t += nt) {
+    const int li = t / n, j = t - li * n;
+    if (li >= nl) continue;
+    const float2 r = res[t];
+    if (MODE == 0) {
+      const float2 x = in[(int64_t)(r0 + li) * n + j];
+      reinterpret_cast<float4*>(out + (int64_t)(r0 + li) * m)[j] =
+          make_float4(x.x * ns, x.y * ns, r.x * scale, r.y * scale);
+    } else {
+      const float2 e = in[(int64_t)(r0 + li) * m + 2 * j];
+      out[(int64_t)(r0 + li) * n + j] = make_float2(e.x * ns + r.x * scale, e.y * ns + r.y * scale);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- host side
 int make_fft_plan(int n, FftPlan& P) {
   P = FftPlan{};
@@ -174,6 +242,44 @@ static void launch_up_rows(const float2* in, float2* out, const FftPlan& Pn, con
                      u.lds_tw ? nullptr : gtw);
 }
 
+// ds = 2 form: lines of n; the twiddle table goes to LDS only while it is
+// small (a large LDS copy costs more in occupancy than the L1-resident
+// global table costs in latency)
+static UpLaunch up2_launch(int n) {
+  const int m = 2 * n;
+  UpLaunch u;
+  int lines = 1;
+  while (lines < 16 && lines * 2 * n <= 2048) lines *= 2;
+  u.lines = lines;
+  u.lds_tw = m <= 2048 && sizeof(float2) * (size_t)(m + 2 * lines * n) <= kLdsBudget;
+  return u;
+}
+
+template <int MODE>
+static void launch_up2_rows(const float2* in, float2* out, const FftPlan& Pn, int rows, float scale,
+                            const UpLaunch& u, const float2* gtw, hipStream_t s) {
+  const int n = Pn.n;
+  const size_t shm = sizeof(float2) * (size_t)((u.lds_tw ? 2 * n : 0) + 2 * u.lines * n);
+  const int blocks = (rows + u.lines - 1) / u.lines;
+  hipLaunchKernelGGL(k_up2_rows<MODE>, dim3(blocks), dim3(256), shm, s, in, out, Pn, rows, u.lines, scale,
+                     u.lds_tw ? nullptr : gtw);
+}
+
+// one upsample row stage: the ds = 2 form when it applies
+template <int MODE>
+static void up_stage(const float2* in, float2* out, const FftPlan& Pn, const FftPlan& Pm, int rows, float scale,
+                     const float2* gtw, hipStream_t s) {
+  const int n = Pn.n, m = Pm.n;
+  if (m == 2 * n && n % 2 == 0)
+    launch_up2_rows<MODE>(in, out, Pn, rows, scale, up2_launch(n), gtw, s);
+  else
+    launch_up_rows<MODE>(in, out, Pn, Pm, rows, scale, up_launch(n, m), gtw, s);
+}
+
+static bool up_lds_twiddles(int n, int m) {
+  return (m == 2 * n && n % 2 == 0) ? up2_launch(n).lds_tw : up_launch(n, m).lds_tw;
+}
+
 static void launch_transpose(const float2* in, float2* out, int R, int C, hipStream_t s) {
   hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32), dim3(256), 0, s, in, out, R, C);
 }
@@ -211,17 +317,16 @@ extern "C" int pty_upsample_fwd(const float* small, int32_t n, int32_t ds, float
   const UpWork W = up_work(n, ds);
   PTY_REQUIRE(work && work_bytes >= W.total, PTY_EWORK, "upsample_fwd: workspace too small");
   const int m = n * ds;
-  const UpLaunch u = up_launch(n, m);
   char* w = (char*)work;
   float2* sT = (float2*)(w + W.t_nn);   // small^T        [n, n]
   float2* YT = (float2*)(w + W.yt);     // U along columns, transposed [n, m]
   float2* Y = (float2*)(w + W.y);       // [m, n]
   float2* tw = (float2*)(w + W.tw);
-  if (!u.lds_tw) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, stream, tw, m);
+  if (!up_lds_twiddles(n, m)) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, stream, tw, m);
   launch_transpose((const float2*)small, sT, n, n, stream);
-  launch_up_rows<0>(sT, YT, Pn, Pm, n, 1.f, u, tw, stream);
+  up_stage<0>(sT, YT, Pn, Pm, n, 1.f, tw, stream);
   launch_transpose(YT, Y, n, m, stream);
-  launch_up_rows<0>(Y, (float2*)big, Pn, Pm, m, 1.f / ((float)n * (float)n), u, tw, stream);
+  up_stage<0>(Y, (float2*)big, Pn, Pm, m, 1.f / ((float)n * (float)n), tw, stream);
   return check_launch("upsample_fwd");
 }
 
@@ -234,16 +339,15 @@ extern "C" int pty_upsample_adj(const float* gbig, int32_t n, int32_t ds, float*
   const UpWork W = up_work(n, ds);
   PTY_REQUIRE(work && work_bytes >= W.total, PTY_EWORK, "upsample_adj: workspace too small");
   const int m = n * ds;
-  const UpLaunch u = up_launch(n, m);
   char* w = (char*)work;
   float2* Wr = (float2*)(w + W.y);      // V along rows            [m, n]
   float2* WT = (float2*)(w + W.yt);     // its transpose           [n, m]
   float2* gT = (float2*)(w + W.t_nn);   // gsmall^T                [n, n]
   float2* tw = (float2*)(w + W.tw);
-  if (!u.lds_tw) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, stream, tw, m);
-  launch_up_rows<1>((const float2*)gbig, Wr, Pn, Pm, m, 1.f, u, tw, stream);
+  if (!up_lds_twiddles(n, m)) hipLaunchKernelGGL(k_twiddle_table, dim3((m + 255) / 256), dim3(256), 0, stream, tw, m);
+  up_stage<1>((const float2*)gbig, Wr, Pn, Pm, m, 1.f, tw, stream);
   launch_transpose(Wr, WT, m, n, stream);
-  launch_up_rows<1>(WT, gT, Pn, Pm, n, 1.f / ((float)n * (float)n), u, tw, stream);
+  up_stage<1>(WT, gT, Pn, Pm, n, 1.f / ((float)n * (float)n), tw, stream);
   launch_transpose(gT, (float2*)gsmall, n, n, stream);
   return check_launch("upsample_adj");
 }
