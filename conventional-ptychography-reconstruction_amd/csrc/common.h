@@ -66,7 +66,11 @@ __device__ __forceinline__ AdamCoef adam_coef(const float* hyper, float b1, floa
   c.step = lr / bc1;
   return c;
 }
+// Contraction off: every caller rounds the update the same way whatever the
+// surrounding code, so the in-place table update and the flat kernel agree
+// to the bit.
 __device__ __forceinline__ void adam_elem(float& p, float& m, float& v, float g, const AdamCoef& c) {
+#pragma clang fp contract(off)
   m = c.b1 * m + (1.f - c.b1) * g;
   v = c.b2 * v + (1.f - c.b2) * g * g;
   const float denom = sqrtf(v) / c.sbc2 + c.eps;
@@ -83,13 +87,21 @@ struct TableAdam {
   const float* hyper;
   float b1, b2, eps;
 };
-__device__ __forceinline__ void table_adam_update(const TableAdam& a, const AdamCoef& c, uint32_t t, float2 g) {
-  float2 p = a.p[t], m = a.m[t], v = a.v[t];
-  adam_elem(p.x, m.x, v.x, g.x, c);
-  adam_elem(p.y, m.y, v.y, g.y, c);
-  a.p[t] = p;
-  a.m[t] = m;
-  a.v[t] = v;
+// One table entry's Adam state, loaded when the entry's gradient work starts
+// so that the loads overlap it.
+struct EntryAdam {
+  float2 p, m, v;
+};
+__device__ __forceinline__ EntryAdam table_adam_load(const TableAdam& a, uint32_t t) {
+  return EntryAdam{a.p[t], a.m[t], a.v[t]};
+}
+__device__ __forceinline__ void table_adam_store(const TableAdam& a, const AdamCoef& c, uint32_t t, EntryAdam e,
+                                                 float2 g) {
+  adam_elem(e.p.x, e.m.x, e.v.x, g.x, c);
+  adam_elem(e.p.y, e.m.y, e.v.y, g.y, c);
+  a.p[t] = e.p;
+  a.m[t] = e.m;
+  a.v[t] = e.v;
 }
 
 // ---------------------------------------------------------------- FFT in LDS

@@ -333,6 +333,8 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
                                                      float2* __restrict__ dtable, TableAdam ad) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_dense_entries) return;
+  EntryAdam st{};
+  if (ad.p) st = table_adam_load(ad, t);
   int l = 0;
   while (l + 1 < g.n_levels && t >= g.offset[l + 1]) ++l;
   const uint32_t e = t - g.offset[l];
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
     }
   }
   if (ad.p) {
-    table_adam_update(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), t, acc);
+    table_adam_store(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), t, st, acc);
   } else if (acc.x != 0.f || acc.y != 0.f) {
     float2 d = dtable[t];
     d.x += acc.x;
@@ -509,6 +511,8 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
   const float* fyl = P.fy + (int64_t)l * L.n1;
   const float2* gl = gfeat + (int64_t)l * pts.n;
   const int cap = 4 * L.n0 * L.n1;
+  EntryAdam st{};
+  if (ad.p && tid < ne) st = table_adam_load(ad, g.offset[l] + e0 + tid);
   for (int t = tid; t <= ne; t += 256) s_off[t] = min(off[t], cap);
   __syncthreads();
   const int k0 = s_off[0], k1 = s_off[ne];
@@ -535,7 +539,7 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
     __syncthreads();
   }
   if (tid < ne && ad.p) {
-    table_adam_update(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), g.offset[l] + e0 + tid, make_float2(ax, ay));
+    table_adam_store(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), g.offset[l] + e0 + tid, st, make_float2(ax, ay));
   } else if (tid < ne && (ax != 0.f || ay != 0.f)) {
     const uint32_t t = g.offset[l] + e0 + tid;
     float2 d = dtable[t];
