@@ -242,14 +242,16 @@ static void launch_up_rows(const float2* in, float2* out, const FftPlan& Pn, con
                      u.lds_tw ? nullptr : gtw);
 }
 
-// ds = 2 form: lines of n; the twiddle table goes to LDS only while it is
-// small (a large LDS copy costs more in occupancy than the L1-resident
-// global table costs in latency)
+// ds = 2 form: lines of n, one line per workgroup from n = 512 on (more
+// independent workgroups beat fuller ones for these latency-bound passes:
+// n = 512 with 1 line 33 us against 40 us with 2); the twiddle table goes to
+// LDS only while it is small (a large LDS copy costs more in occupancy than
+// the L1-resident global table costs in latency)
 static UpLaunch up2_launch(int n) {
   const int m = 2 * n;
   UpLaunch u;
   int lines = 1;
-  while (lines < 16 && lines * 2 * n <= 2048) lines *= 2;
+  while (lines < 16 && lines * 2 * n <= 1024) lines *= 2;
   u.lines = lines;
   u.lds_tw = m <= 2048 && sizeof(float2) * (size_t)(m + 2 * lines * n) <= kLdsBudget;
   return u;
@@ -261,7 +263,9 @@ static void launch_up2_rows(const float2* in, float2* out, const FftPlan& Pn, in
   const int n = Pn.n;
   const size_t shm = sizeof(float2) * (size_t)((u.lds_tw ? 2 * n : 0) + 2 * u.lines * n);
   const int blocks = (rows + u.lines - 1) / u.lines;
-  hipLaunchKernelGGL(k_up2_rows<MODE>, dim3(blocks), dim3(256), shm, s, in, out, Pn, rows, u.lines, scale,
+  // 128 threads for long lines (n = 1824: 243 us against 255 us with 256)
+  const int threads = n > 1024 ? 128 : 256;
+  hipLaunchKernelGGL(k_up2_rows<MODE>, dim3(blocks), dim3(threads), shm, s, in, out, Pn, rows, u.lines, scale,
                      u.lds_tw ? nullptr : gtw);
 }
 
