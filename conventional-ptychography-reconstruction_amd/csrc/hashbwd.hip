@@ -304,6 +304,60 @@ __global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, PlanLayout L, Pt
   A[t] = make_float2(s0.x + left.x, s0.y + left.y);
 }
 
+// Row form of pass 1: one workgroup per (dense level l, row i).  The row's
+// feature gradients and x fractions are staged in LDS with coalesced loads,
+// then each cell's S0 / S1 is summed from LDS in point order (the same sums
+// as cell_sums) and A[l][i][cx] = S0(cx) + S1(cx - 1) is written.  LDS:
+// n0 * 12 B + 2 (res + 1) * 8 B.
+__global__ __launch_bounds__(256) void k_dense_pass1_rows(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
+                                                          const float2* __restrict__ gfeat, float2* __restrict__ A) {
+  extern __shared__ float2 srow[];
+  const int n0 = L.n0;
+  int b = blockIdx.x, l = -1;
+  for (int q = 0; q < g.n_levels; ++q) {
+    if (!((g.dense_mask >> q) & 1u)) continue;
+    if (b < L.n1) { l = q; break; }
+    b -= L.n1;
+  }
+  if (l < 0) return;   // uniform across the workgroup
+  const int i = b;
+  const int res1 = (int)g.res[l] + 1;
+  float2* sg = srow;                                  // [n0] feature gradients of the row
+  float* sf = reinterpret_cast<float*>(sg + n0);      // [n0] x fractions
+  float2* s0c = sg + n0 + (n0 + 1) / 2;               // [res1] S0 per cell
+  float2* s1c = s0c + res1;                           // [res1] S1 per cell
+  const float2* gl = gfeat + (int64_t)l * pts.n + (int64_t)i * n0;
+  const float* fxl = P.fx + (int64_t)l * n0;
+  for (int t = threadIdx.x; t < n0; t += blockDim.x) {
+    sg[t] = gl[t];
+    sf[t] = fxl[t];
+  }
+  __syncthreads();
+  const int* st = P.s0 + L.off_s[l];
+  for (int c = threadIdx.x; c < res1; c += blockDim.x) {
+    float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
+    const int j1 = st[c + 1];
+#pragma unroll 4
+    for (int j = st[c]; j < j1; ++j) {
+      const float f = sf[j];
+      const float2 v = sg[j];
+      s0.x += (1.f - f) * v.x;
+      s0.y += (1.f - f) * v.y;
+      s1.x += f * v.x;
+      s1.y += f * v.y;
+    }
+    s0c[c] = s0;
+    s1c[c] = s1;
+  }
+  __syncthreads();
+  float2* Ar = A + L.off_a[l] + (int64_t)i * res1;
+  for (int c = threadIdx.x; c < res1; c += blockDim.x) {
+    const float2 s0 = s0c[c];
+    const float2 left = c > 0 ? s1c[c - 1] : make_float2(0.f, 0.f);
+    Ar[c] = make_float2(s0.x + left.x, s0.y + left.y);
+  }
+}
+
 // G(cx, cy) from A (dense levels)
 __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanPtrs& P, int l, int res,
                                                 const float2* __restrict__ Al, int cx, int cy) {
@@ -622,9 +676,21 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
   }
   const PlanPtrs P = plan_ptrs(L, plan);
   const int64_t na = L.off_a[g.n_levels];
-  if (na > 0)
-    hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, P, gfeat,
-                       A);
+  if (na > 0) {
+    int n_dense = 0, res1_max = 0;
+    for (int l = 0; l < g.n_levels; ++l)
+      if ((g.dense_mask >> l) & 1u) {
+        ++n_dense;
+        if ((int)g.res[l] + 1 > res1_max) res1_max = (int)g.res[l] + 1;
+      }
+    const size_t shm = sizeof(float2) * ((size_t)L.n0 + (L.n0 + 1) / 2 + 2 * (size_t)res1_max);
+    if (shm <= 65536)
+      hipLaunchKernelGGL(k_dense_pass1_rows, dim3((unsigned)(n_dense * L.n1)), dim3(256), shm, stream, g, L, pts, P,
+                         gfeat, A);
+    else
+      hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, P,
+                         gfeat, A);
+  }
   uint32_t n_dense_entries = 0, hashed_chunks = 0;
   for (int l = 0; l < g.n_levels; ++l) {
     if ((g.dense_mask >> l) & 1u) n_dense_entries = g.offset[l] + g.size[l];

@@ -633,6 +633,39 @@ def test_table_adam_in_place_matches_flat_adam(pkg, dev):
     assert torch.equal(f.hyper, u.hyper)
 
 
+def test_table_adam_in_place_contract(pkg, dev):
+    """pty_inr_bwd_finish_adam: after an updating fused step the tables' slots
+    of the flat gradient stay zero (the MLP slots are consumed by Adam too);
+    the entry point rejects explicit coordinates and misaligned buffers with
+    an error and no launch."""
+    inst = pkg.synthetic.make_named("small", seed=9)
+    M = inst.probe.shape[0]
+    pinit = torch.from_numpy(inst.probe.astype(np.complex64))
+    m = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=19)
+    e = pkg.FusedReconstruction(m, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX, batch_size=5)
+    e.set_lr(1e-3)
+    before = m.xy_hash_encoder.params.detach().clone()
+    e.step(torch.arange(5, dtype=torch.int32, device=dev))
+    assert not torch.equal(before, m.xy_hash_encoder.params.detach())      # the table moved
+    assert not e.flat.grad.any()                                            # nothing left behind
+    L = pkg._lib
+    n_tab = m.xy_hash_encoder.params.numel()
+    buf = torch.zeros(3 * n_tab + 8, device=dev)
+    p, mm, v = (buf[i * n_tab:(i + 1) * n_tab] for i in range(3))
+    # explicit coordinates have no plan: rejected
+    pts = pkg.complex_inr.CoordPoints(torch.rand(64, 2, device=dev)).descriptor(e.obj_spec)
+    with pytest.raises(RuntimeError, match="grid"):
+        L.call("pty_inr_bwd_finish_adam", e.obj_grid, e.obj_mlp, e.obj_gmlp, p.data_ptr(), mm.data_ptr(),
+               v.data_ptr(), e.hyper.data_ptr(), 0.9, 0.999, 1e-8, pts, e.ws_obj.data_ptr(), e.ws_obj.numel(),
+               L.stream_ptr(dev))
+    # a moment buffer off its 8-byte alignment: rejected
+    with pytest.raises(RuntimeError, match="aligned"):
+        L.call("pty_inr_bwd_finish_adam", e.obj_grid, e.obj_mlp, e.obj_gmlp, p.data_ptr(), mm.data_ptr() + 4,
+               v.data_ptr(), e.hyper.data_ptr(), 0.9, 0.999, 1e-8, e.obj_pdesc, e.ws_obj.data_ptr(), e.ws_obj.numel(),
+               L.stream_ptr(dev))
+    torch.cuda.synchronize()
+
+
 def _mlp_grads_vs_oracle(model, params):
     """(name, error) per MLP tensor.  Weights and biases: relative error (norm).
     omega: omega * dL/domega = Re<G_W, W> + Re<G_b, b> sums terms that cancel,
