@@ -16,12 +16,12 @@ are epochs = ceil(positions/batch) steps.  value = iters/s = epochs per
 second.
 
 Two timed regions of K steps each, both bracketed by a synchronisation:
-(1) eager steps (the host enqueues every launch), with HIP events on the
-launch streams around each launch of the dominant kernel; (2) the same
-steps replayed from the captured HIP graph, which is how
-FusedReconstruction.run_epoch (the reconstruction loop) runs full batches;
-value and ms_per_step come from (2) (from (1) with --no-graph), and
-eager_ms_per_step reports (1).
+(1) eager steps (the host enqueues every launch) with the object and probe
+chains on one stream and HIP events around each launch of the dominant
+kernel; (2) the same steps replayed from the captured HIP graph with the
+probe chain on its own stream, which is how FusedReconstruction.run_epoch
+(the reconstruction loop) runs full batches; value and ms_per_step come from
+(2) (from (1) with --no-graph), and eager_ms_per_step reports (1).
 
 Also reported: the roofline of the dominant kernel (the MLP backward with
 recompute, k_inr_bwd16 in FP32, launched once for the object grid and once for the
@@ -92,14 +92,19 @@ def pmc_summary(fetch_csv, write_csv, workload, precision, command):
         factor = true_kib / (sum(cal) / len(cal))
         import re
         key = re.compile(r"\b" + re.escape(MLP_BWD_KERNEL[precision]) + r"\(")
-        # each step launches the kernel for the object grid, then the probe grid
+        # each step launches the kernel once for the object grid and once for
+        # the probe grid (on two streams, so in either order); the object grid
+        # has the larger point count, so the larger dispatch of each pair is
+        # the object's
         per = {"object": [], "probe": []}
         hits = [r for r in rows if key.search(r["Kernel_Name"])]
         if not hits or len(hits) % 2:
             raise SystemExit(f"{path}: expected pairs of {key.pattern} dispatches, found {len(hits)}")
         hits.sort(key=lambda r: int(r["Dispatch_Id"]))
-        for i, r in enumerate(hits):
-            per["object" if i % 2 == 0 else "probe"].append(float(r["Counter_Value"]) * factor * 1024)
+        for i in range(0, len(hits), 2):
+            a, b = (float(hits[i + q]["Counter_Value"]) * factor * 1024 for q in range(2))
+            per["object"].append(max(a, b))
+            per["probe"].append(min(a, b))
         return factor, {g: sum(v) / len(v) for g, v in per.items()}, len(hits)
 
     ff, fetch, nf = read(fetch_csv, "FETCH_SIZE")
@@ -859,7 +864,11 @@ def main():
         eng.step(next_batch())
     torch.cuda.synchronize()
     # timed region 1: K eager steps (one launch sequence per step from the
-    # host), HIP events on the launch streams around each MLP-backward launch
+    # host), HIP events on the launch stream around each MLP-backward launch.
+    # The object and probe chains share one stream here, so each launch's
+    # duration is the kernel's own (with two streams a launch shares the GPU
+    # with the other chain's kernels for a run-dependent part of its time).
+    eng.overlap = False
     eng.enable_timing(["obj_mlp_bwd", "prb_mlp_bwd"])
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
@@ -873,6 +882,7 @@ def main():
     eager_ms = start.elapsed_time(stop) / args.steps
     tim = eng.timings_ms()
     eng.enable_timing([])
+    eng.overlap = True
     # the MLP-backward kernel runs twice a step (object grid, probe grid);
     # average launch duration and average points per launch over both
     launches = tim["obj_mlp_bwd"] + tim["prb_mlp_bwd"]

@@ -230,8 +230,10 @@ class FusedReconstruction:
                 self.far[b] = (d, _lib.workspace(q("pty_farfield_workspace", d), dev))
         self.epoch = 0
         self._timers = None
-        # the probe chain's stream (default priority, like every stream here)
+        # the probe chain's stream (default priority, like every stream here);
+        # overlap = False runs both chains on the caller's stream
         self.side = torch.cuda.Stream(device=dev)
+        self.overlap = True
         self.graph = None
         self._grad_clear = True   # FlatParams starts with zero gradients
         if self.fp8_states:
@@ -272,12 +274,17 @@ class FusedReconstruction:
             if tab.dtype == torch.float16:
                 _lib.call("pty_cast_f16", enc.params.data_ptr(), enc.params.numel(), tab.data_ptr(), s)
 
+    def _streams(self):
+        main = torch.cuda.current_stream(self.device)
+        return main, (self.side if self.overlap else main)
+
     def forward_fields(self):
-        main, side = torch.cuda.current_stream(self.device), self.side
+        main, side = self._streams()
         C = _lib.call
         m = self.model
         self._refresh_tables(main.cuda_stream)
-        side.wait_stream(main)
+        if self.overlap:
+            side.wait_stream(main)
         # object chain on the caller's stream
         s = main.cuda_stream
         self._mark("obj_inr_fwd", True, main)
@@ -298,7 +305,8 @@ class FusedReconstruction:
         if m.has_object_residual:
             C("pty_residual_fwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), m.object_initial.data_ptr(),
               self.obj_up.numel(), self.obj.data_ptr(), s)
-        main.wait_stream(side)
+        if self.overlap:
+            main.wait_stream(side)
 
     def _inr_bwd(self, which, grid, mlp, gmlp, table, dtable, pdesc, dout, ws, stream, adam=False):
         """INR backward on ``stream``; with timing on for '<which>_mlp_bwd' the
@@ -328,7 +336,7 @@ class FusedReconstruction:
 
     def step(self, batch_idx: torch.Tensor, update: bool = True):
         """One optimizer step on the positions ``batch_idx`` (int32, device)."""
-        main, side = torch.cuda.current_stream(self.device), self.side
+        main, side = self._streams()
         s = main.cuda_stream
         C = _lib.call
         m = self.model
@@ -349,7 +357,8 @@ class FusedReconstruction:
         if self.sparsity_weight:
             C("pty_haar_sparsity", self.obj.data_ptr(), self.H, self.W, 1.0, self.sparsity_weight,
               self.sparsity.data_ptr(), self.dobj.data_ptr(), self.ws_haar.data_ptr(), self.ws_haar.numel(), s)
-        side.wait_stream(main)
+        if self.overlap:
+            side.wait_stream(main)
         # object adjoint of the upsample first (the object chain is the longer
         # one), then the probe chain on the side stream, then the object INR
         if m.has_object_residual:
@@ -373,7 +382,8 @@ class FusedReconstruction:
             st_o, st_p = self.fp8_states
             C("pty_fp8_roll", st_o.data_ptr(), s)
             C("pty_fp8_roll", st_p.data_ptr(), p)
-        main.wait_stream(side)
+        if self.overlap:
+            main.wait_stream(side)
         if update:
             # Adam clears the trainable gradients it consumes; frozen ones are
             # cleared with them when they exist

@@ -633,6 +633,27 @@ def test_table_adam_in_place_matches_flat_adam(pkg, dev):
     assert torch.equal(f.hyper, u.hyper)
 
 
+def test_one_stream_matches_two_streams(pkg, dev):
+    """The probe chain on its own stream (default) or on the caller's stream
+    (overlap = False) gives the same bits: the two chains share no buffers."""
+    inst = pkg.synthetic.make_named("small", seed=10)
+    M = inst.probe.shape[0]
+    pinit = torch.from_numpy(inst.probe.astype(np.complex64))
+    engines = []
+    for overlap in (True, False):
+        m = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=21)
+        e = pkg.FusedReconstruction(m, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX, batch_size=5,
+                                    loss_type="Poisson_likelihood_loss")
+        e.overlap = overlap
+        e.set_lr(1e-3)
+        engines.append(e)
+    for b in ([0, 1, 2, 3, 4], [9, 8, 7, 6, 5], [30, 5, 17, 2, 21]):
+        bt = torch.tensor(b, dtype=torch.int32, device=dev)
+        losses = [e.step(bt).clone() for e in engines]
+        assert torch.equal(losses[0], losses[1])
+    assert torch.equal(engines[0].flat.data, engines[1].flat.data)
+
+
 def test_table_adam_in_place_contract(pkg, dev):
     """pty_inr_bwd_finish_adam: after an updating fused step the tables' slots
     of the flat gradient stay zero (the MLP slots are consumed by Adam too);
