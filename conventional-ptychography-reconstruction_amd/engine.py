@@ -424,6 +424,12 @@ class FusedReconstruction:
 
     def graph_step(self, batch_idx: torch.Tensor):
         """Replay the captured step on ``batch_idx`` (int32, device, length batch_size)."""
+        if not self._grad_clear:
+            # the captured step starts from cleared gradients; an eager step
+            # without the update since the last one left some behind
+            _lib.call("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0,
+                      _lib.stream_ptr(self.device))
+            self._grad_clear = True
         self.batch_buf.copy_(batch_idx, non_blocking=True)
         self.graph.replay()
         return self.loss

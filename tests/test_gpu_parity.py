@@ -589,6 +589,31 @@ def test_graph_replay_matches_eager(pkg, dev):
         assert torch.equal(o1, o2), it
 
 
+def test_graph_replay_after_step_without_update(pkg, dev):
+    """An eager step without the update (gradients left in the buffer) followed
+    by graph replays gives the bits of the same sequence run eagerly."""
+    inst = pkg.synthetic.make_named("small", seed=12)
+    M = inst.probe.shape[0]
+    pinit = torch.from_numpy(inst.probe.astype(np.complex64))
+    engines = []
+    for _ in range(2):
+        m = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=23)
+        e = pkg.FusedReconstruction(m, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX, batch_size=6)
+        e.set_lr(1e-3)
+        engines.append(e)
+    e1, e2 = engines
+    e2.capture()
+    b0 = torch.arange(6, dtype=torch.int32, device=dev)
+    e1.step(b0, update=False)
+    e2.step(b0, update=False)
+    for it in range(3):
+        b = (b0 + 3 * it) % inst.stack.shape[2]
+        l1 = e1.step(b).clone()
+        l2 = e2.graph_step(b).clone()
+        assert torch.equal(l1, l2), it
+    assert torch.equal(e1.flat.data, e2.flat.data)
+
+
 def test_fused_step_is_deterministic(pkg, dev):
     """Same inputs, same bits: two engines from the same seed stay identical over several updates."""
     inst = pkg.synthetic.make_named("small", seed=7)
