@@ -238,9 +238,8 @@ static int check_bwd_args(const PtyGrid* g, const PtyMlp* mlp, const PtyPoints* 
   return PTY_OK;
 }
 
-// Workspace layout shared by the two stages: per-block MLP gradient partials
-// (+ the chunk sums of the first reduce stage), the feature gradient
-// [n_levels][n] c64, then the table-gradient scratch.
+// Workspace layout shared by the two stages: per-block MLP gradient partials,
+// the feature gradient [n_levels][n] c64, then the table-gradient scratch.
 struct BwdWork {
   float* partial;
   float2* gfeat;

@@ -30,8 +30,9 @@
 //   g_x = -i w A^H g_w
 //   layer 0: Im(g_z conj(z0)) = g_u0 / w0;  P0 += Im(.) h^T, q0 += Im(.);
 //   g_h = w0 W0^T Im(.), written level-major for the table-gradient kernels.
-// The reduce stage of inr.hip applies -i w (w0) to P, q (P0, q0) and forms the
-// frequency gradients from the totals.
+// Each workgroup forms its share of the frequency gradients from its partial
+// sums (omega_terms); the finish kernel of inr.hip applies -i w (w0) to the
+// totals of P, q (P0, q0).
 #include "inr_common.h"
 
 namespace pty {

@@ -543,7 +543,7 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
     wave_sync();
     // ---------------- hidden layers, last to first (the staged gradient is
     // g_w = g_z conj(z); the factor -i w of g_v is applied to g_x here and
-    // to the weight gradients by the reduce stage, which also forms dw)
+    // to the weight gradients by the finish kernel; dw comes from omega_terms)
 #pragma unroll
     for (int k = H - 1; k >= 0; --k) {
       const float w = s.om[k + 1];
