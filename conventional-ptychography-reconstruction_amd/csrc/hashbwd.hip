@@ -358,9 +358,10 @@ __global__ __launch_bounds__(256) void k_dense_pass1_rows(PtyGrid g, PlanLayout 
   }
 }
 
-// G(cx, cy) from A (dense levels)
+// G(cx, cy) from A (dense levels), or the part of it over the rows i with
+// i - st[cell] = s mod S (lane s of an entry split over S lanes)
 __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanPtrs& P, int l, int res,
-                                                const float2* __restrict__ Al, int cx, int cy) {
+                                                const float2* __restrict__ Al, int cx, int cy, int s, int S) {
   const int* st = P.s1 + L.off_s[l];
   const float* fyl = P.fy + (int64_t)l * L.n1;
   float ax = 0.f, ay = 0.f;
@@ -370,7 +371,7 @@ __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanP
     if (cell < 0 || cell > res) continue;
     const int i1 = st[cell + 1];
 #pragma unroll 8
-    for (int i = st[cell]; i < i1; ++i) {
+    for (int i = st[cell] + s; i < i1; i += S) {
       const float w = dy ? fyl[i] : 1.f - fyl[i];
       const float2 v = Al[(int64_t)i * (res + 1) + cx];
       ax += w * v.x;
@@ -381,34 +382,59 @@ __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanP
 }
 
 // Dense levels: one thread per table entry (entries of hashed levels are
-// skipped here and handled by k_table_hashed).
+// skipped here and handled by k_table_hashed).  The coarse levels, whose cells
+// hold many rows of points, come first and get kDenseSplit lanes per entry:
+// lane s sums the rows s, s + kDenseSplit, ... of every corner and the lanes
+// of an entry add up by a fixed butterfly, so a coarse entry's chain of
+// dependent loads is kDenseSplit times shorter and the result deterministic.
+constexpr int kDenseSplit = 8;
+
 __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, PlanPtrs P,
                                                      const float2* __restrict__ A, uint32_t n_dense_entries,
-                                                     float2* __restrict__ dtable, TableAdam ad) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_dense_entries) return;
+                                                     uint32_t n_split_entries, float2* __restrict__ dtable,
+                                                     TableAdam ad) {
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n_split_lanes = n_split_entries * kDenseSplit;
+  const bool split = gt < n_split_lanes;
+  const uint32_t t = split ? gt / kDenseSplit : n_split_entries + (gt - n_split_lanes);
+  const int s = split ? (int)(gt % kDenseSplit) : 0;
+  const int S = split ? kDenseSplit : 1;
+  const bool active = t < n_dense_entries;
+  const bool owner = active && s == 0;
   EntryAdam st{};
-  if (ad.p) st = table_adam_load(ad, t);
-  int l = 0;
-  while (l + 1 < g.n_levels && t >= g.offset[l + 1]) ++l;
-  const uint32_t e = t - g.offset[l];
-  const int res = (int)g.res[l];
-  const uint32_t size = g.size[l];
-  const float2* Al = A + L.off_a[l];
-  const int64_t top = (int64_t)res * res + res;  // largest cx + cy*res
+  if (ad.p && owner) st = table_adam_load(ad, t);
   float2 acc = make_float2(0.f, 0.f);
-  for (int k = 0; k < 2; ++k) {
-    const int64_t target = (int64_t)e + (int64_t)k * size;
-    if (target > top) break;
-    // cy with cx = target - cy*res in [0, res]
-    const int64_t cy_lo = target > res ? (target - res + res - 1) / res : 0;
-    const int64_t cy_hi = target / res;
-    for (int64_t cy = cy_lo; cy <= cy_hi && cy <= res; ++cy) {
-      const float2 v = corner_from_A(L, P, l, res, Al, (int)(target - cy * res), (int)cy);
-      acc.x += v.x;
-      acc.y += v.y;
+  if (active) {
+    int l = 0;
+    while (l + 1 < g.n_levels && t >= g.offset[l + 1]) ++l;
+    const uint32_t e = t - g.offset[l];
+    const int res = (int)g.res[l];
+    const uint32_t size = g.size[l];
+    const float2* Al = A + L.off_a[l];
+    const int64_t top = (int64_t)res * res + res;  // largest cx + cy*res
+    for (int k = 0; k < 2; ++k) {
+      const int64_t target = (int64_t)e + (int64_t)k * size;
+      if (target > top) break;
+      // cy with cx = target - cy*res in [0, res]
+      const int64_t cy_lo = target > res ? (target - res + res - 1) / res : 0;
+      const int64_t cy_hi = target / res;
+      for (int64_t cy = cy_lo; cy <= cy_hi && cy <= res; ++cy) {
+        const float2 v = corner_from_A(L, P, l, res, Al, (int)(target - cy * res), (int)cy, s, S);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
     }
   }
+  // the butterfly runs with every lane of the wave converged; the groups of
+  // kDenseSplit lanes are aligned, so split and unsplit lanes never mix
+  float2 sum = acc;
+#pragma unroll
+  for (int d = 1; d < kDenseSplit; d <<= 1) {
+    sum.x += __shfl_xor(sum.x, d);
+    sum.y += __shfl_xor(sum.y, d);
+  }
+  if (!owner) return;
+  if (split) acc = sum;
   if (ad.p) {
     table_adam_store(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), t, st, acc);
   } else if (acc.x != 0.f || acc.y != 0.f) {
@@ -547,6 +573,7 @@ __global__ __launch_bounds__(256) void k_csr_fill(PtyGrid g, PlanLayout L, PlanP
 // LDS, then each entry's thread adds its own terms from LDS in plan order.
 // The sum order per entry is fixed, so the result is deterministic.
 constexpr int kHashWin = 2048;
+constexpr int kHashPer = kHashWin / 256;   // terms per thread and window
 
 __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                       const float2* __restrict__ gfeat,
@@ -574,14 +601,29 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
   float ax = 0.f, ay = 0.f;
   for (int w0 = k0; w0 < k1; w0 += kHashWin) {
     const int wn = min(kHashWin, k1 - w0);
-    for (int t = tid; t < wn; t += 256) {
-      const uint32_t c = terms[w0 + t];
-      const int i = (int)(c >> 17), j = (int)((c >> 2) & 0x7FFFu);
-      const float fx = fxl[j], fy = fyl[i];
-      const float wx = (c & 1u) ? fx : 1.f - fx;
-      const float w = wx * ((c & 2u) ? fy : 1.f - fy);
-      const float2 v = gl[(int64_t)i * L.n0 + j];
-      s_val[t] = make_float2(w * v.x, w * v.y);
+    const int nu = (wn + 255) / 256;   // term slots in use, uniform across the workgroup
+    // all of a thread's term loads first, then all of their gathers, so that
+    // each phase is one round trip to memory instead of one per term; a slot
+    // past the window's end reads its last term again and stores nothing
+    uint32_t c[kHashPer];
+#pragma unroll
+    for (int u = 0; u < kHashPer; ++u)
+      if (u < nu) c[u] = terms[w0 + min(tid + 256 * u, wn - 1)];
+    float wt[kHashPer];
+    float2 v[kHashPer];
+#pragma unroll
+    for (int u = 0; u < kHashPer; ++u)
+      if (u < nu) {
+        const int i = (int)(c[u] >> 17), j = (int)((c[u] >> 2) & 0x7FFFu);
+        const float fx = fxl[j], fy = fyl[i];
+        const float wx = (c[u] & 1u) ? fx : 1.f - fx;
+        wt[u] = wx * ((c[u] & 2u) ? fy : 1.f - fy);
+        v[u] = gl[(int64_t)i * L.n0 + j];
+      }
+#pragma unroll
+    for (int u = 0; u < kHashPer; ++u) {
+      const int t = tid + 256 * u;
+      if (u < nu && t < wn) s_val[t] = make_float2(wt[u] * v[u].x, wt[u] * v[u].y);
     }
     __syncthreads();
     const int a = max(my0, w0), b = min(my1, w0 + wn);
@@ -691,14 +733,20 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
       hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, P,
                          gfeat, A);
   }
-  uint32_t n_dense_entries = 0, hashed_chunks = 0;
+  uint32_t n_dense_entries = 0, hashed_chunks = 0, n_split_entries = 0;
+  bool coarse = true;
   for (int l = 0; l < g.n_levels; ++l) {
     if ((g.dense_mask >> l) & 1u) n_dense_entries = g.offset[l] + g.size[l];
     else hashed_chunks += (g.size[l] + 255) / 256;
+    // leading dense levels with at least kDenseSplit rows of points per cell
+    coarse = coarse && ((g.dense_mask >> l) & 1u) && L.n1 >= kDenseSplit * ((int)g.res[l] + 1);
+    if (coarse) n_split_entries = g.offset[l] + g.size[l];
   }
-  if (n_dense_entries > 0)
-    hipLaunchKernelGGL(k_table_dense, dim3((n_dense_entries + 255) / 256), dim3(256), 0, stream, g, L, P,
-                       (const float2*)A, n_dense_entries, (float2*)dtable, ad);
+  if (n_dense_entries > 0) {
+    const uint32_t lanes = n_split_entries * kDenseSplit + (n_dense_entries - n_split_entries);
+    hipLaunchKernelGGL(k_table_dense, dim3((lanes + 255) / 256), dim3(256), 0, stream, g, L, P, (const float2*)A,
+                       n_dense_entries, n_split_entries, (float2*)dtable, ad);
+  }
   if (hashed_chunks > 0)
     hipLaunchKernelGGL(k_table_hashed, dim3(hashed_chunks), dim3(256), 0, stream, g, L, pts, P, gfeat,
                        (float2*)dtable, ad);

@@ -495,15 +495,17 @@ def test_full_size_properties(pkg, dev):
         assert abs(I[b].double().sum() / (256 * 256) - ew) < 1e-5 * ew
 
 
-@pytest.mark.parametrize("log2T", [10, 12, 14, 18])
-def test_table_gradient_paths_agree(pkg, dev, log2T):
+@pytest.mark.parametrize("canvas,log2T", [(128, 10), (128, 12), (128, 14), (128, 18), (512, 14)])
+def test_table_gradient_paths_agree(pkg, dev, canvas, log2T):
     """Regular-grid owner-computes (cached plan / plan built per call) == explicit-coords scatter == oracle.
 
     log2T 10 and 12 have hashed levels with fewer than 2^8 columns per hash
     bucket (the per-entry path of k_table_hashed), 14 and 18 the shared-bucket
-    path; the grid path runs three times and must give the same bits."""
+    path; the INR grid of canvas 512 (256^2 points) has at least eight rows of
+    points per cell on levels 0 and 1, whose entries k_table_dense splits over
+    eight lanes.  The grid path runs three times and must give the same bits."""
     torch.manual_seed(3)
-    model = _model(pkg, dev, 128, 16, log2T=log2T, residual=False)
+    model = _model(pkg, dev, canvas, 16, log2T=log2T, residual=False)
     with torch.no_grad():
         model.xy_hash_encoder.params.uniform_(-1, 1)
     spec = model.xy_hash_encoder.spec
@@ -526,7 +528,8 @@ def test_table_gradient_paths_agree(pkg, dev, log2T):
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])   # deterministic
     assert rel_err(outs[0], outs[3]) < 1e-5           # gather == scatter
     params = oracle_params(model)
-    ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], oracle_table(model), params["obj_mlp"], 64, 64)
+    side = canvas // 2
+    ref = orc.inr_field(model.xy_coords.cpu(), params["obj_table"], oracle_table(model), params["obj_mlp"], side, side)
     (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref.reshape(-1))).sum().backward()
     assert rel_err(outs[0], params["obj_table"].grad) < 1e-4
 
@@ -659,11 +662,14 @@ def test_fused_step_is_deterministic(pkg, dev):
     assert torch.equal(engines[0].flat.data, engines[1].flat.data)
 
 
-def test_table_adam_in_place_matches_flat_adam(pkg, dev):
+@pytest.mark.parametrize("shape", ["small", "cfg1"])
+def test_table_adam_in_place_matches_flat_adam(pkg, dev, shape):
     """The table-gradient kernels' in-place Adam step gives the bits of
     storing the table gradient and running the flat Adam over it (same
-    element update, same gradient), over steps with and without the update."""
-    inst = pkg.synthetic.make_named("small", seed=8)
+    element update, same gradient), over steps with and without the update.
+    The canvas-1024 case (INR grid 512^2) has the coarse dense levels whose
+    entries are split over several lanes."""
+    inst = pkg.synthetic.make_named(shape, seed=8)
     M = inst.probe.shape[0]
     pinit = torch.from_numpy(inst.probe.astype(np.complex64))
     engines = []
