@@ -306,6 +306,52 @@ __device__ __forceinline__ v4f mfma_pts(const float4& a, const float4& b, v4f ac
 }
 __device__ __forceinline__ float4 neg4(const float4& a) { return make_float4(-a.x, -a.y, -a.z, -a.w); }
 
+// One wave's share of the weight-gradient sums into the GradLayout slots at R
+// (every slot but the frequency terms): stored (ADD false) or added.
+template <bool ADD>
+__device__ __forceinline__ void put_slot(float& r, float v) {
+  if (ADD) r += v;
+  else r = v;
+}
+
+template <int H, bool ADD>
+__device__ __forceinline__ void put_wave_sums(float* R, const v4f (&P0)[2][2], const v4f (&Pr)[H][2][2],
+                                              const v4f (&Pi)[H][2][2], const float (&q0s)[2],
+                                              const float (&qrs)[H][2], const float (&qis)[H][2], float acc_wo,
+                                              float br, float bi, int lane, int g, int j) {
+  using GL = GradLayout<H>;
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int o = 16 * mb + 4 * g + v, in = 16 * nb + j;
+        put_slot<ADD>(R[GL::w0 + o * 32 + in], P0[mb][nb][v]);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          put_slot<ADD>(R[GL::A(k) + 2 * (o * 32 + in)], Pr[k][mb][nb][v]);
+          put_slot<ADD>(R[GL::A(k) + 2 * (o * 32 + in) + 1], Pi[k][mb][nb][v]);
+        }
+      }
+  if (g == 0) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      put_slot<ADD>(R[GL::b0 + 16 * mb + j], q0s[mb]);
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        put_slot<ADD>(R[GL::c(k) + 2 * (16 * mb + j)], qrs[k][mb]);
+        put_slot<ADD>(R[GL::c(k) + 2 * (16 * mb + j) + 1], qis[k][mb]);
+      }
+    }
+  }
+  put_slot<ADD>(R[GL::wo + 2 * (lane & 31) + (lane >> 5)], acc_wo);
+  if (lane == 0) {
+    put_slot<ADD>(R[GL::bo], br);
+    put_slot<ADD>(R[GL::bo + 1], bi);
+  }
+}
+
 template <int H>
 __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, const float2* __restrict__ table,
                                                          float2* __restrict__ gfeat, PtyPoints pts,
@@ -498,12 +544,14 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
   }
 
   // ---------------- workgroup reduction into partial[blockIdx.x][GradLayout::total]
+  // Waves 0-3 store their sums into regions 0-3 of the staging area, waves 4-7
+  // add theirs into region wave - 4, then every thread totals its slots over
+  // the four regions in a fixed order: three barriers instead of one per wave.
   using GL = GradLayout<H>;
-  static_assert(sizeof(st) / sizeof(float) >= (size_t)GL::total, "reduction buffer does not fit");
+  constexpr int kRegions = kW16 / 2;
+  static_assert(sizeof(st) / sizeof(float) >= (size_t)kRegions * GL::total, "reduction buffer does not fit");
   __syncthreads();
   float* red = &st[0].X[0][0];
-  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) red[t] = 0.f;
-  __syncthreads();
   // bias sums over the four lane groups (same unit, different points)
   float q0s[2], qrs[H][2], qis[H][2];
 #pragma unroll
@@ -519,38 +567,18 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     }
   }
   const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
-  for (int w = 0; w < kW16; ++w) {
-    if (wave == w) {
-#pragma unroll
-      for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int o = 16 * mb + 4 * g + v, in = 16 * nb + j;
-            red[GL::w0 + o * 32 + in] += P0[mb][nb][v];
-#pragma unroll
-            for (int k = 0; k < H; ++k) {
-              red[GL::A(k) + 2 * (o * 32 + in)] += Pr[k][mb][nb][v];
-              red[GL::A(k) + 2 * (o * 32 + in) + 1] += Pi[k][mb][nb][v];
-            }
-          }
-      if (g == 0) {
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-          red[GL::b0 + 16 * mb + j] += q0s[mb];
-#pragma unroll
-          for (int k = 0; k < H; ++k) {
-            red[GL::c(k) + 2 * (16 * mb + j)] += qrs[k][mb];
-            red[GL::c(k) + 2 * (16 * mb + j) + 1] += qis[k][mb];
-          }
-        }
-      }
-      red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
-      if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
-    }
-    __syncthreads();
-  }
+  float* R = red + (wave % kRegions) * GL::total;
+  if (wave < kRegions)
+    put_wave_sums<H, false>(R, P0, Pr, Pi, q0s, qrs, qis, acc_wo, br, bi, lane, g, j);
+  __syncthreads();
+  if (wave >= kRegions)
+    put_wave_sums<H, true>(R, P0, Pr, Pi, q0s, qrs, qis, acc_wo, br, bi, lane, g, j);
+  __syncthreads();
+  static_assert(kRegions == 4, "the fixed summation order below assumes four regions");
+  for (int t = threadIdx.x; t < GL::total; t += blockDim.x)
+    if (t < GL::om || t >= GL::wo)
+      red[t] = (red[t] + red[GL::total + t]) + (red[2 * GL::total + t] + red[3 * GL::total + t]);
+  __syncthreads();
   omega_terms<H>(red, m);
   for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
 }
