@@ -98,6 +98,8 @@ _SIGNATURES = {
     "pty_adam": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_float, c_float,
                            c_float, c_int32, c_void_p]),
     "pty_adam_tick": (c_int32, [c_void_p, c_void_p]),
+    "pty_batch_next": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_int64, c_void_p]),
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
     "pty_cast_f16": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "pty_fp8_roll": (c_int32, [c_void_p, c_void_p]),

@@ -153,6 +153,32 @@ __global__ void k_stack_prepare(const float* __restrict__ in, int M, int N, int 
 
 __global__ void k_adam_tick(float* hyper) { hyper[1] += 1.f; }
 
+// Next batch of an epoch's position order (one workgroup): with c = cursor[0]
+// and start = c * batch (0, and c = 0, once start + batch would pass n_order),
+// idx[k] = order[start + k]; the previous batch's per-pattern losses
+// loss[0..batch) go to hist[(c - 1) * batch + k] when c >= 1 and that range
+// lies inside hist; then cursor[0] = c + 1 and, with hyper, hyper[1] += 1.
+__global__ void k_batch_next(const int32_t* __restrict__ order, int32_t n_order, int32_t* cursor, int32_t batch,
+                             int32_t* __restrict__ idx, float* hyper, const float* __restrict__ loss,
+                             float* __restrict__ hist, int64_t n_hist) {
+  __shared__ int s_c;
+  if (threadIdx.x == 0) {
+    int c = cursor[0];
+    if (c < 0 || (int64_t)c * batch + batch > n_order) c = 0;
+    s_c = c;
+  }
+  __syncthreads();
+  const int c = s_c;
+  const int64_t start = (int64_t)c * batch;
+  for (int k = threadIdx.x; k < batch; k += blockDim.x) idx[k] = order[start + k];
+  if (loss && hist && c >= 1 && (int64_t)c * batch <= n_hist)
+    for (int k = threadIdx.x; k < batch; k += blockDim.x) hist[(int64_t)(c - 1) * batch + k] = loss[k];
+  if (threadIdx.x == 0) {
+    cursor[0] = c + 1;
+    if (hyper) hyper[1] += 1.f;
+  }
+}
+
 static unsigned grid_for(int64_t n, int block) {
   int64_t b = (n + block - 1) / block;
   if (b > 65536) b = 65536;
@@ -238,6 +264,17 @@ extern "C" int pty_adam_tick(float* hyper, hipStream_t stream) {
   PTY_REQUIRE(hyper, PTY_EINVAL, "adam_tick: null pointer");
   hipLaunchKernelGGL(k_adam_tick, dim3(1), dim3(1), 0, stream, hyper);
   return check_launch("adam_tick");
+}
+
+extern "C" int pty_batch_next(const int32_t* order, int32_t n_order, int32_t* cursor, int32_t batch, int32_t* idx,
+                              float* hyper, const float* loss, float* hist, int64_t n_hist, hipStream_t stream) {
+  PTY_REQUIRE(order && cursor && idx, PTY_EINVAL, "batch_next: null pointer");
+  PTY_REQUIRE(batch > 0 && n_order >= batch, PTY_EINVAL, "batch_next: batch %d with an order of %d positions",
+              batch, n_order);
+  PTY_REQUIRE(n_hist >= 0 && (n_hist == 0 || hist), PTY_EINVAL, "batch_next: bad loss history");
+  hipLaunchKernelGGL(k_batch_next, dim3(1), dim3(256), 0, stream, order, n_order, cursor, batch, idx, hyper,
+                     hist ? loss : nullptr, hist, n_hist);
+  return check_launch("batch_next");
 }
 
 extern "C" int pty_stack_prepare(const float* stack_mnf, int32_t M, int32_t N, int32_t frames, float* out,

@@ -633,8 +633,13 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
   // ---------------- workgroup reduction into partial[blockIdx.x][GradLayout::total]
   // (scaled formats: the MFMA sums of staged operands are multiplied by
   // 1 / (s_G s_X), the bias sums by 1 / s_G)
+  // Waves 0-1 store their sums into two LDS regions, waves 2-3 add theirs
+  // into region wave - 2, then every thread totals its slots over the two
+  // regions: two barriers instead of one per wave.
   using GL = GradLayout<H>;
-  static_assert(sizeof(st) / (sizeof(float)) >= (size_t)GL::total, "reduction buffer does not fit");
+  constexpr int kRegions = kWaves / 2;
+  static_assert(kRegions == 2, "the summation below assumes two regions");
+  static_assert(sizeof(st) / (sizeof(float)) >= (size_t)kRegions * GL::total, "reduction buffer does not fit");
   float dwi0 = 1.f, dwik[H], cik[H], b0i = 1.f;
 #pragma unroll
   for (int k = 0; k < H; ++k) dwik[k] = cik[k] = 1.f;
@@ -647,39 +652,54 @@ __global__ __launch_bounds__(kThreads) void k_inr_bwd_lp(PtyGrid g, PtyMlp m, co
       cik[k] = 1.f / s.xs[slot_gv(k)];
     }
   }
+  // cross-lane sums first, with every lane of the wave taking part
+  float cr[H], ci[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    cr[k] = acc_c[k][0] + __shfl_xor(acc_c[k][0], 32);
+    ci[k] = acc_c[k][1] + __shfl_xor(acc_c[k][1], 32);
+  }
+  const float b0s = acc_b0 + __shfl_xor(acc_b0, 32);
+  const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
   __syncthreads();
   float* red = (float*)&st[0];
-  for (int t = threadIdx.x; t < GL::total; t += blockDim.x) red[t] = 0.f;
-  __syncthreads();
-  for (int w = 0; w < kWaves; ++w) {
-    if (wave == w) {
+  float* R = red + (wave % kRegions) * GL::total;
+  for (int phase = 0; phase < 2; ++phase) {
+    if (wave / kRegions == phase) {
+      const bool add = phase == 1;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int oo = crow(v, h);
-        red[GL::w0 + oo * 32 + j] += dW0[v] * dwi0;
+        float* r0 = &R[GL::w0 + oo * 32 + j];
+        *r0 = (add ? *r0 : 0.f) + dW0[v] * dwi0;
 #pragma unroll
         for (int k = 0; k < H; ++k) {
-          red[GL::A(k) + 2 * (oo * 32 + j)] += dWr[k][v] * dwik[k];
-          red[GL::A(k) + 2 * (oo * 32 + j) + 1] += dWi[k][v] * dwik[k];
+          float* rr = &R[GL::A(k) + 2 * (oo * 32 + j)];
+          rr[0] = (add ? rr[0] : 0.f) + dWr[k][v] * dwik[k];
+          rr[1] = (add ? rr[1] : 0.f) + dWi[k][v] * dwik[k];
         }
       }
+      if (h == 0) {
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const float cr = acc_c[k][0] + __shfl_xor(acc_c[k][0], 32);
-        const float ci = acc_c[k][1] + __shfl_xor(acc_c[k][1], 32);
-        if (h == 0) {
-          red[GL::c(k) + 2 * j] += cr * cik[k];
-          red[GL::c(k) + 2 * j + 1] += ci * cik[k];
+        for (int k = 0; k < H; ++k) {
+          float* rc = &R[GL::c(k) + 2 * j];
+          rc[0] = (add ? rc[0] : 0.f) + cr[k] * cik[k];
+          rc[1] = (add ? rc[1] : 0.f) + ci[k] * cik[k];
         }
+        R[GL::b0 + j] = (add ? R[GL::b0 + j] : 0.f) + b0s * b0i;
       }
-      const float b0s = acc_b0 + __shfl_xor(acc_b0, 32);
-      if (h == 0) red[GL::b0 + j] += b0s * b0i;
-      red[GL::wo + 2 * (lane & 31) + (lane >> 5)] += acc_wo;
-      const float br = wave_sum(acc_bor), bi = wave_sum(acc_boi);
-      if (lane == 0) { red[GL::bo] += br; red[GL::bo + 1] += bi; }
+      float* rw = &R[GL::wo + 2 * (lane & 31) + (lane >> 5)];
+      *rw = (add ? *rw : 0.f) + acc_wo;
+      if (lane == 0) {
+        R[GL::bo] = (add ? R[GL::bo] : 0.f) + br;
+        R[GL::bo + 1] = (add ? R[GL::bo + 1] : 0.f) + bi;
+      }
     }
     __syncthreads();
   }
+  for (int t = threadIdx.x; t < GL::total; t += blockDim.x)
+    if (t < GL::om || t >= GL::wo) red[t] += red[GL::total + t];
+  __syncthreads();
   omega_terms<H>(red, m);
   for (int t = threadIdx.x; t < GL::total; t += blockDim.x) partial[(int64_t)blockIdx.x * GL::total + t] = red[t];
 }

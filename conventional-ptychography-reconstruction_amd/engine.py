@@ -235,6 +235,13 @@ class FusedReconstruction:
         self.side = torch.cuda.Stream(device=dev)
         self.overlap = True
         self.graph = None
+        # epoch order on the device for graph replays: the captured step takes
+        # its batch from order at cursor (pty_batch_next) and records the
+        # previous batch's losses in loss_hist (one slot per position)
+        self.order = torch.zeros(self.frames, dtype=torch.int32, device=dev)
+        self.cursor = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.loss_hist = torch.zeros(self.frames, dtype=torch.float32, device=dev)
+        self._from_order = False
         self._grad_clear = True   # FlatParams starts with zero gradients
         if self.fp8_states:
             # calibration: one step without the update records the amax of
@@ -345,7 +352,13 @@ class FusedReconstruction:
         fused = update and self.fuse_table_adam
         if not self._grad_clear:
             C("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, s)
-        if fused:
+        if self._from_order:
+            # (capture only) the batch comes from the epoch order on the
+            # device, and Adam's count advances in the same launch
+            C("pty_batch_next", self.order.data_ptr(), self.order.numel(), self.cursor.data_ptr(), b,
+              batch_idx.data_ptr(), self.hyper.data_ptr() if fused else None, self.loss.data_ptr(),
+              self.loss_hist.data_ptr(), self.loss_hist.numel(), s)
+        elif fused:
             # the table-gradient kernels apply Adam with this step's count
             C("pty_adam_tick", self.hyper.data_ptr(), s)
         self.forward_fields()
@@ -403,9 +416,11 @@ class FusedReconstruction:
     def capture(self):
         """Capture one full training step (batch size ``batch_size``) in a HIP graph.
 
-        Every launch of the step goes to the capture stream; the batch indices
-        are read from a persistent device buffer that ``graph_step`` fills
-        before each replay.  The graph holds no memset nodes (clears are fill
+        Every launch of the step goes to the capture stream.  The captured step
+        starts with ``pty_batch_next``: it reads its batch from the epoch order
+        (``order``) at ``cursor`` and advances the cursor, so consecutive replays
+        walk an epoch with no host work in between (``begin_order`` +
+        ``graph_next``).  The graph holds no memset nodes (clears are fill
         kernels) and no host synchronisation.
         """
         self.batch_buf = torch.zeros(self.batch_size, dtype=torch.int32, device=self.device)
@@ -416,45 +431,76 @@ class FusedReconstruction:
         torch.cuda.synchronize(self.device)
         timers, self._timers = self._timers, None
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.step(self.batch_buf, update=True)
+        self._from_order = True
+        try:
+            with torch.cuda.graph(self.graph):
+                self.step(self.batch_buf, update=True)
+        finally:
+            self._from_order = False
         self._timers = timers
         torch.cuda.synchronize(self.device)
         return self.graph
 
-    def graph_step(self, batch_idx: torch.Tensor):
-        """Replay the captured step on ``batch_idx`` (int32, device, length batch_size)."""
+    def begin_order(self, order: torch.Tensor):
+        """Load an epoch's position order (int32, device, length <= frames) for
+        ``graph_next``; the next replay takes order[0:batch_size]."""
+        n = order.numel()
+        if n < self.batch_size or n > self.frames:
+            raise ValueError(f"order of {n} positions for batch {self.batch_size} and {self.frames} frames")
+        self.order[:n].copy_(order, non_blocking=True)
+        self.cursor.zero_()
+
+    def graph_next(self):
+        """Replay the captured step on the next batch of the loaded order
+        (``begin_order``); returns the loss buffer (valid until the next step)."""
+        if self.graph is None:
+            self.capture()
         if not self._grad_clear:
             # the captured step starts from cleared gradients; an eager step
             # without the update since the last one left some behind
             _lib.call("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0,
                       _lib.stream_ptr(self.device))
             self._grad_clear = True
-        self.batch_buf.copy_(batch_idx, non_blocking=True)
         self.graph.replay()
         return self.loss
+
+    def graph_step(self, batch_idx: torch.Tensor):
+        """Replay the captured step on ``batch_idx`` (int32, device, length batch_size)."""
+        if batch_idx.numel() != self.batch_size:
+            raise ValueError(f"graph_step takes full batches of {self.batch_size}")
+        self.begin_order(batch_idx)
+        return self.graph_next()
 
     def run_epoch(self, generator: torch.Generator | None = None, use_graph: bool = True):
         """One epoch: randperm over frames, ceil(frames/batch) steps (Recovery.py:243-350).
 
-        Full batches replay the captured step (captured on first use); the
-        trailing partial batch, if any, runs eagerly.  Both give the same
-        bits (the graph holds the same launches)."""
+        Full batches replay the captured step (captured on first use), which
+        takes each batch from the permutation on the device; the trailing
+        partial batch, if any, runs eagerly.  Returns the sum of the epoch's
+        per-pattern losses (a device scalar).  Both paths give the same bits
+        (the graph holds the same launches, and both total the same per-position
+        loss record)."""
         lr = self.lr0 * self.gamma ** (self.epoch // self.decay_step)
         if self.epoch == 0 or self.epoch % self.decay_step == 0:
             self.set_lr(lr)
         perm = torch.randperm(self.frames, device=self.device, generator=generator).to(torch.int32)
-        total = torch.zeros((), dtype=torch.float32, device=self.device)
-        for start in range(0, self.frames, self.batch_size):
-            bidx = perm[start:start + self.batch_size]
-            if use_graph and bidx.numel() == self.batch_size:
-                if self.graph is None:
-                    self.capture()
-                total += self.graph_step(bidx).sum()
-            else:
-                total += self.step(bidx).sum()
+        B = self.batch_size
+        n_full = self.frames // B
+        if use_graph and n_full > 0:
+            if self.graph is None:
+                self.capture()
+            self.begin_order(perm)
+            for _ in range(n_full):
+                self.graph_next()
+            # each replay recorded the previous batch's losses; the last is left
+            self.loss_hist[(n_full - 1) * B:n_full * B].copy_(self.loss[:B])
+        else:
+            for i in range(n_full):
+                self.loss_hist[i * B:(i + 1) * B].copy_(self.step(perm[i * B:(i + 1) * B]))
+        if n_full * B < self.frames:
+            self.loss_hist[n_full * B:].copy_(self.step(perm[n_full * B:]))
         self.epoch += 1
-        return total
+        return self.loss_hist.sum()
 
     def current_fields(self):
         """(object [H,W], normalised probe [M,N]) of the current parameters."""

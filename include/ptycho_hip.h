@@ -270,6 +270,21 @@ int pty_adam(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64
              hipStream_t stream);
 int pty_adam_tick(float* hyper, hipStream_t stream);
 
+/* ---- reconstruction loop (Recovery.py:243-248, 300-346) ---- */
+/* The loop's batch slicing batch_idx = perm[batch_start : batch_start + batch]
+ * and its per-batch loss bookkeeping, on the device, so that a captured step
+ * needs no host work between replays.  order: the epoch's permutation
+ * (n_order int32); cursor (1 int32): the batch number, 0 at the epoch start.
+ * With c = *cursor and start = c * batch (start = 0 and c = 0 once start + batch
+ * would pass n_order): idx[k] = order[start + k] for k < batch; when hist is
+ * non-null, c >= 1 and c * batch <= n_hist, the previous batch's per-pattern
+ * losses loss[0..batch) are copied to hist[(c - 1) * batch + k]; then
+ * *cursor = c + 1, and with hyper non-null Adam's step count advances as in
+ * pty_adam_tick (one launch instead of two).  One workgroup; batch > 0 and
+ * n_order >= batch. */
+int pty_batch_next(const int32_t* order, int32_t n_order, int32_t* cursor, int32_t batch, int32_t* idx,
+                   float* hyper, const float* loss, float* hist, int64_t n_hist, hipStream_t stream);
+
 /* fill kernel (graph-safe replacement for hipMemsetAsync) */
 int pty_fill(float* p, int64_t n, float value, hipStream_t stream);
 

@@ -40,6 +40,13 @@ def test_errors_are_reported_without_a_gpu(pkg):
     assert pkg._lib.query("pty_farfield_workspace", far) > 0
     with pytest.raises(ValueError):
         pkg.farfield.far_descriptor(64, 64, 256, 256, 10, 100, "l2")
+    # batch_next: a batch longer than the order, an empty batch, a missing cursor
+    with pytest.raises(RuntimeError, match="batch 8 with an order of 5"):
+        pkg._lib.call("pty_batch_next", 1, 5, 1, 8, 1, None, None, None, 0, None)
+    with pytest.raises(RuntimeError, match="batch 0"):
+        pkg._lib.call("pty_batch_next", 1, 5, 1, 0, 1, None, None, None, 0, None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        pkg._lib.call("pty_batch_next", 1, 5, None, 2, 1, None, None, None, 0, None)
 
 
 def test_table_precision_plumbing(pkg):

@@ -617,6 +617,40 @@ def test_graph_replay_after_step_without_update(pkg, dev):
     assert torch.equal(e1.flat.data, e2.flat.data)
 
 
+def test_batch_next_walks_an_order(pkg, dev):
+    """pty_batch_next: batch c of the order, the previous batch's losses into
+    the history, wrap to the start when a batch would pass the end, and the
+    Adam count advancing only when hyper is given."""
+    L = pkg._lib
+    s = L.stream_ptr(dev)
+    n, B = 23, 5
+    order = torch.randperm(n, generator=torch.Generator().manual_seed(1)).to(torch.int32).to(dev)
+    cursor = torch.zeros(1, dtype=torch.int32, device=dev)
+    idx = torch.full((B,), -1, dtype=torch.int32, device=dev)
+    hyper = torch.tensor([1e-3, 0.0], device=dev)
+    loss = torch.zeros(B, device=dev)
+    hist = torch.full((n,), -1.0, device=dev)
+    starts = [0, 5, 10, 15, 0, 5]          # batch 4 (20..24) would pass 23: back to the start
+    expect_hist = torch.full((n,), -1.0)
+    for c, st in enumerate(starts):
+        L.call("pty_batch_next", order.data_ptr(), n, cursor.data_ptr(), B, idx.data_ptr(),
+               hyper.data_ptr() if c % 2 == 0 else None, loss.data_ptr(), hist.data_ptr(), n, s)
+        assert torch.equal(idx.cpu(), order[st:st + B].cpu()), c
+        prev = starts[c - 1] if c else None
+        if c and st != 0:
+            # the previous batch (number st / B - 1) recorded its losses
+            expect_hist[st - B:st] = torch.arange(B, dtype=torch.float32) + 10 * (c - 1)
+        assert torch.equal(hist.cpu(), expect_hist), (c, prev)
+        assert int(cursor.item()) == st // B + 1
+        loss.copy_(torch.arange(B, dtype=torch.float32, device=dev) + 10 * c)
+    assert float(hyper[1].item()) == 3.0 and float(hyper[0].item()) == pytest.approx(1e-3)
+    # without a history nothing is recorded
+    hist2 = hist.clone()
+    L.call("pty_batch_next", order.data_ptr(), n, cursor.data_ptr(), B, idx.data_ptr(), None, loss.data_ptr(),
+           None, 0, s)
+    assert torch.equal(hist, hist2) and torch.equal(idx.cpu(), order[10:15].cpu())
+
+
 def test_run_epoch_graph_matches_eager_with_partial_batch(pkg, dev):
     """run_epoch (graph replays for full batches, the trailing partial batch
     eager) gives the bits of the all-eager epoch, over two epochs."""
