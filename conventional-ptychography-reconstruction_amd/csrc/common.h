@@ -92,16 +92,27 @@ struct TableAdam {
 struct EntryAdam {
   float2 p, m, v;
 };
+// The moments are touched once per step, so they move with nontemporal loads
+// and stores (streamed past the caches, which keep the step's reused data);
+// the parameter is stored normally, since the next forward gathers it.
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 nt_load(const float2* q) {
+  const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2*>(q));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void nt_store(float2* q, float2 v) {
+  __builtin_nontemporal_store(nt_f2{v.x, v.y}, reinterpret_cast<nt_f2*>(q));
+}
 __device__ __forceinline__ EntryAdam table_adam_load(const TableAdam& a, uint32_t t) {
-  return EntryAdam{a.p[t], a.m[t], a.v[t]};
+  return EntryAdam{nt_load(a.p + t), nt_load(a.m + t), nt_load(a.v + t)};
 }
 __device__ __forceinline__ void table_adam_store(const TableAdam& a, const AdamCoef& c, uint32_t t, EntryAdam e,
                                                  float2 g) {
   adam_elem(e.p.x, e.m.x, e.v.x, g.x, c);
   adam_elem(e.p.y, e.m.y, e.v.y, g.y, c);
   a.p[t] = e.p;
-  a.m[t] = e.m;
-  a.v[t] = e.v;
+  nt_store(a.m + t, e.m);
+  nt_store(a.v + t, e.v);
 }
 
 // ---------------------------------------------------------------- FFT in LDS
