@@ -693,7 +693,8 @@ def kernels_mode(args):
     (pty_hashgrid_fwd, gathers of 4 corners x 16 levels per point), the
     Fourier upsample 1824 -> 3648 and its adjoint (pty_upsample_fwd/adj), and
     the far-field fused forward + adjoint for a batch of 64 probe-512
-    patterns.  Algorithmic bytes are the minimum the operation must move
+    patterns; then the object table's gradient + Adam stage of the fused step
+    at canvas 1024 and 3648, and a plain device copy for scale.  Algorithmic bytes are the minimum the operation must move
     (inputs read once, outputs written once); achieved = bytes / mean launch
     time, against the 8 TB/s HBM peak (table gathers mostly hit L2 / MALL, so
     the gather kernel's figure can exceed it)."""
@@ -774,6 +775,53 @@ def kernels_mode(args):
         # written once (8 B / canvas pixel); probe read + gradient 16 B / probe pixel
         nbytes = B * M * M * 12 + H * H * 8 + M * M * 16
         row(f"far field fwd+adjoint ({lt})", f"{B} x {M}^2 patterns, canvas {H}", nbytes, ms)
+    del obj, prb, stack, dobj, dprb, ws
+
+    # --- the table-gradient tail of the object INR's backward with the tables'
+    # Adam step (pty_inr_bwd_finish_adam: gradient finish, dense levels in two
+    # separable passes, hashed levels by CSR gather), in the fused step of both
+    # workloads (eager, one stream, events around the launch)
+    for wl in ("cfg1", "cfg3"):
+        shapes_w, _, batch_w, loss_w, wl_name_w = WORKLOADS[wl]
+        inst, model = build_problem(pkg, dev, workload=wl)
+        eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                      loss_type=loss_w, lr=1e-3, batch_size=batch_w)
+        eng.overlap = False
+        bi = torch.arange(batch_w, dtype=torch.int32, device=dev)
+        for _ in range(3):
+            eng.step(bi)
+        torch.cuda.synchronize()
+        eng.enable_timing(["obj_finish_adam"])
+        for _ in range(10):
+            eng.step(bi)
+        torch.cuda.synchronize()
+        t = sorted(eng.timings_ms()["obj_finish_adam"])
+        ms = t[len(t) // 2]
+        spec_w = model.xy_hash_encoder.spec
+        n_pts = model.downsampled_width * model.downsampled_height
+        n_hashed = sum(1 for d in spec_w.dense if not d)
+        n_dense_entries = sum(sz for sz, d in zip(spec_w.size, spec_w.dense) if d)
+        n_hashed_entries = spec_w.n_entries - n_dense_entries
+        # feature gradients read once (8 B per point and level), the hashed
+        # levels' CSR terms (4 per point and level, 4 B) and offsets (4 B per
+        # entry), the dense levels' row sums A written and read once, Adam's
+        # parameter and moments read and written (48 B per entry), the MLP
+        # partials read once
+        a_bytes = 8 * sum(model.downsampled_height * (r + 1) for r, d in zip(spec_w.res, spec_w.dense) if d)
+        nbytes = (8 * spec_w.n_levels * n_pts + 16 * n_hashed * n_pts + 4 * n_hashed_entries + 2 * a_bytes
+                  + 48 * spec_w.n_entries)
+        row(f"table gradient + Adam, object ({wl_name_w.split('_')[0]})",
+            f"{n_pts} points, {spec_w.n_entries} entries", nbytes, ms,
+            {"dense_entries": n_dense_entries, "hashed_entries": n_hashed_entries})
+        del eng, model, inst
+        torch.cuda.empty_cache()
+
+    # --- what a plain device copy achieves (read + write), for scale
+    src = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    ms = _time_launches(lambda: dst.copy_(src))
+    row("device copy (torch copy_, 1 GiB)", "read 1 GiB + write 1 GiB", 2 * src.numel() * 4, ms)
+    del src, dst
     out = {"mode": "kernels", "device": torch.cuda.get_device_name(0), "hbm_peak_GBs": HBM_PEAK_GBS,
            "rows": rows}
     print(json.dumps(out))

@@ -252,8 +252,10 @@ class FusedReconstruction:
     # ------------------------------------------------------------------ timing
     def enable_timing(self, names):
         """Record HIP events (on the launch stream) around the named launches
-        ('obj_inr_fwd', 'obj_inr_bwd', 'prb_inr_bwd', 'farfield', and 'obj_mlp_bwd' /
-        'prb_mlp_bwd', which time the MLP-backward kernel alone)."""
+        ('obj_inr_fwd', 'obj_inr_bwd', 'prb_inr_bwd', 'farfield', 'obj_mlp_bwd' /
+        'prb_mlp_bwd', which time the MLP-backward kernel alone, and
+        'obj_finish_adam' / 'prb_finish_adam', the gradient finish with the
+        tables' gradient and Adam step)."""
         self._timers = {n: [] for n in names}
 
     def _mark(self, name, begin, stream=None):
@@ -327,8 +329,10 @@ class FusedReconstruction:
             C("pty_inr_bwd_mlp", grid, mlp, table.data_ptr(), pdesc, dout.data_ptr(), ws.data_ptr(), ws.numel(), s)
             self._mark(name, False, stream)
             p, m, v = self._tab_adam[which]
+            self._mark(which + "_finish_adam", True, stream)
             C("pty_inr_bwd_finish_adam", grid, mlp, gmlp, p, m, v, self.hyper.data_ptr(), self.beta1, self.beta2,
               self.eps, pdesc, ws.data_ptr(), ws.numel(), s)
+            self._mark(which + "_finish_adam", False, stream)
             return
         if self._timers is None or name not in self._timers:
             self._mark(which + "_inr_bwd", True, stream)
