@@ -495,15 +495,16 @@ def test_full_size_properties(pkg, dev):
         assert abs(I[b].double().sum() / (256 * 256) - ew) < 1e-5 * ew
 
 
-@pytest.mark.parametrize("canvas,log2T", [(128, 10), (128, 12), (128, 14), (128, 18), (512, 14)])
+@pytest.mark.parametrize("canvas,log2T", [(128, 10), (128, 12), (128, 14), (128, 18), (512, 14), (1024, 14)])
 def test_table_gradient_paths_agree(pkg, dev, canvas, log2T):
     """Regular-grid owner-computes (cached plan / plan built per call) == explicit-coords scatter == oracle.
 
     log2T 10 and 12 have hashed levels with fewer than 2^8 columns per hash
     bucket (the per-entry path of k_table_hashed), 14 and 18 the shared-bucket
-    path; the INR grid of canvas 512 (256^2 points) has at least eight rows of
-    points per cell on levels 0 and 1, whose entries k_table_dense splits over
-    eight lanes.  The grid path runs three times and must give the same bits."""
+    path.  k_table_dense splits the entries of the coarse dense levels over
+    several lanes: 8 lanes on levels 0-3 of the 256^2 grid (canvas 512), 16 on
+    levels 0-1 and 8 on levels 2-5 of the 512^2 grid (canvas 1024).  The grid
+    path runs three times and must give the same bits."""
     torch.manual_seed(3)
     model = _model(pkg, dev, canvas, 16, log2T=log2T, residual=False)
     with torch.no_grad():
