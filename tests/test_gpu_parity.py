@@ -501,10 +501,10 @@ def test_table_gradient_paths_agree(pkg, dev, canvas, log2T):
 
     log2T 10 and 12 have hashed levels with fewer than 2^8 columns per hash
     bucket (the per-entry path of k_table_hashed), 14 and 18 the shared-bucket
-    path.  k_table_dense splits the entries of the coarse dense levels over
-    several lanes: 8 lanes on levels 0-3 of the 256^2 grid (canvas 512), 16 on
-    levels 0-1 and 8 on levels 2-5 of the 512^2 grid (canvas 1024).  The grid
-    path runs three times and must give the same bits."""
+    path.  k_table_dense splits each entry of the leading dense levels with at
+    least eight rows of points per cell over eight lanes: levels 0-1 of the
+    256^2 grid (canvas 512), levels 0-3 of the 512^2 grid (canvas 1024).  The
+    grid path runs three times and must give the same bits."""
     torch.manual_seed(3)
     model = _model(pkg, dev, canvas, 16, log2T=log2T, residual=False)
     with torch.no_grad():
