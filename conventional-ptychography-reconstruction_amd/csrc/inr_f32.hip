@@ -404,9 +404,15 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
 #pragma unroll
     for (int c = 0; c < 8; ++c) S.F[feat16(c, g) * PS + j] = feat[c];
     float zr[8], zi[8];
+    // The two waves of a SIMD alternate MFMA blocks with transcendental
+    // activation work; a wave raises its issue priority while it feeds the
+    // matrix core, so that the other wave's VALU work fills in around it
+    // rather than the reverse.
     {
       v4f u[2];
+      __builtin_amdgcn_s_setprio(2);
       layer0_16<H>(s, feat, j, g, u);
+      __builtin_amdgcn_s_setprio(0);
 #pragma unroll
       for (int q = 0; q < 8; ++q) cis(s.om[0] * u[q >> 2][q & 3], zr[q], zi[q]);
     }
@@ -418,7 +424,9 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
         S.X[k][(32 + unit16(q, g)) * PS + j] = zi[q];
       }
       v4f vr[2], vi[2];
+      __builtin_amdgcn_s_setprio(2);
       hidden16<H>(s, k, zr, zi, j, g, vr, vi);
+      __builtin_amdgcn_s_setprio(0);
       act16(s.om[k + 1], vr, vi, zr, zi);
     }
     // ---------------- output layer
@@ -455,6 +463,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
       }
       wave_sync();
       // P += G X^H: Pr = Gr Xr^T + Gi Xi^T, Pi = Gi Xr^T - Gr Xi^T
+      __builtin_amdgcn_s_setprio(2);
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
         const float4 ar = lds4(&S.G[(16 * mb + j) * PS + 4 * g]);
@@ -491,6 +500,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
           t3[mb] = mfma4(a_r - a_i, gs, t3[mb]);
         }
       }
+      __builtin_amdgcn_s_setprio(0);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const float a1 = t1[q >> 2][q & 3], a2 = t2[q >> 2][q & 3], a3 = t3[q >> 2][q & 3];
