@@ -521,6 +521,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
         S.G[unit16(q, g) * PS + j] = gu[q];
       }
       wave_sync();
+      __builtin_amdgcn_s_setprio(2);
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
         const float4 a = lds4(&S.G[(16 * mb + j) * PS + 4 * g]);
@@ -538,6 +539,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
 #pragma unroll
         for (int fb = 0; fb < 2; ++fb) gh[fb] = mfma4(s.w0[u * WS36 + 16 * fb + j], gu[q], gh[fb]);
       }
+      __builtin_amdgcn_s_setprio(0);
       wave_sync();
       // feature 16 fb + 4 g + 2 h + e is feature e of level 8 fb + 2 g + h
       if (valid) {
