@@ -61,6 +61,10 @@ class ComplexMLP(nn.Module):
             self._fp8_state = fp8_state_tensor(device)
         return self._fp8_state
 
+    def n_hidden(self) -> int:
+        """Number of hidden complex Gabor layers."""
+        return len(self.net) - 2
+
     def tensors(self):
         """[w0, b0, omega0, (A_k, c_k, omega_k)*, wo, bo] in kernel order."""
         layers = list(self.net)

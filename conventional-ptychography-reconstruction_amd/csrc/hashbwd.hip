@@ -148,8 +148,7 @@ static PlanPtrs plan_ptrs(const PlanLayout& L, const void* plan) {
 }
 
 __device__ __forceinline__ uint32_t axis_cell(const PtyGrid& g, int l, float x, float& frac) {
-#pragma clang fp contract(off)
-  const float p = x * g.scale[l] + 0.5f;
+  const float p = __builtin_fmaf(x, g.scale[l], 0.5f);   // one rounding, as level_pos
   const float f = floorf(p);
   frac = p - f;
   return (uint32_t)(int)f;

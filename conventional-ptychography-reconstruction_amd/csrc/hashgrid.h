@@ -3,7 +3,8 @@
 //
 // Restates the multiresolution hash encoding that the reference obtains from
 // tiny-cuda-nn's HashGrid (models/complex_inr.py:130-145): per level l,
-// pos = x*scale_l + 0.5 (two separately rounded f32 ops), g = floor(pos),
+// pos = fma(x, scale_l, 0.5) (one rounding, as tiny-cuda-nn's pos_fract),
+// g = floor(pos),
 // frac = pos - g; corner c = (c&1, c>>1); dense index (cx + cy*res) reduced
 // modulo the level size (the edge corner cx = res wraps into the next row),
 // hashed index (cx ^ cy*2654435761) mod 2^32 mod 2^log2T; bilinear weights
@@ -28,12 +29,11 @@ struct LevelPos {
 };
 
 __device__ __forceinline__ LevelPos level_pos(const PtyGrid& g, int l, float x0, float x1) {
-  // x*scale and +0.5 are two separately rounded operations: no FMA contraction
-  // (HIP's __fmul_rn is a plain multiply and would otherwise be fused).
-#pragma clang fp contract(off)
+  // x*scale + 0.5 with a single rounding (fused multiply-add), the form of
+  // tiny-cuda-nn's pos_fract
   const float s = g.scale[l];
-  float p0 = x0 * s + 0.5f;
-  float p1 = x1 * s + 0.5f;
+  float p0 = __builtin_fmaf(x0, s, 0.5f);
+  float p1 = __builtin_fmaf(x1, s, 0.5f);
   float f0 = floorf(p0), f1 = floorf(p1);
   LevelPos r;
   r.gx = (uint32_t)(int)f0;
@@ -89,9 +89,8 @@ __device__ __forceinline__ LevelC level_c(const PtyGrid& g, int la, int lb, bool
 }
 
 __device__ __forceinline__ LevelPos level_pos_c(const LevelC& c, float x0, float x1) {
-#pragma clang fp contract(off)
-  float p0 = x0 * c.scale + 0.5f;
-  float p1 = x1 * c.scale + 0.5f;
+  float p0 = __builtin_fmaf(x0, c.scale, 0.5f);
+  float p1 = __builtin_fmaf(x1, c.scale, 0.5f);
   float f0 = floorf(p0), f1 = floorf(p1);
   LevelPos r;
   r.gx = (uint32_t)(int)f0;

@@ -17,7 +17,8 @@ Per step (batch of positions ``batch_idx`` on the device):
   loss_b, dobject, dprobe_n = far field + loss + adjoint over the batch
   dobject += sparsity_weight * d Haar(object)   (only when the weight is set)
   dprobe  = normalise^T(dprobe_n);  residual^T;  dsmall = Upsample^T(dobject)
-  grads   = INR^T (recomputed activations, hash-table atomics)
+  grads   = INR^T (recomputed activations; the table gradient gathered per
+            entry from a static plan of the grid, no atomics)
   Adam over all trainable parameters: the hash tables inside their
   table-gradient kernels (the table gradient is never stored), the rest in
   one flat launch
@@ -91,12 +92,52 @@ class FlatParams:
         return self.offsets[id(p)], _real_numel(p)
 
 
+def _check_initial(name, t, shape):
+    """A residual's initial field as the kernels read it: complex, of the
+    field's shape; returned as a private contiguous complex64 copy."""
+    if not torch.is_complex(t):
+        raise ValueError(f"{name} must be a complex tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, the field is {tuple(shape)}")
+    return t.detach().to(torch.complex64).contiguous().clone()
+
+
+def check_inputs(model, stack_mnf, tlY, tlX, batch_size):
+    """Host-side shape checks of the fused step's inputs (the kernels index
+    the stack, the scan windows and the initial fields without bounds
+    checks).  Raises ValueError; returns (frames, tlY, tlX) as int64 arrays."""
+    M, N = model.probe_width, model.probe_height
+    if stack_mnf.dim() != 3 or tuple(stack_mnf.shape[:2]) != (M, N):
+        raise ValueError(f"measured stack has shape {tuple(stack_mnf.shape)}; expected [{M}, {N}, frames] "
+                         f"(the reference layout, Recovery.py:103)")
+    frames = int(stack_mnf.shape[2])
+    tlY = np.asarray(tlY).reshape(-1)
+    tlX = np.asarray(tlX).reshape(-1)
+    if frames < 1 or tlY.size != frames or tlX.size != frames:
+        raise ValueError(f"{tlY.size} / {tlX.size} scan windows for {frames} measured patterns")
+    if int(batch_size) < 1:
+        raise ValueError("batch_size must be positive")
+    check_windows(tlY, tlX, M, N, model.output_width, model.output_height)
+    if model.has_object_residual:
+        _check_initial("object_initial", model.object_initial, (model.output_width, model.output_height))
+    if model.has_probe_residual:
+        _check_initial("probe_initial", model.probe_initial, (M, N))
+    return frames, tlY.astype(np.int64), tlX.astype(np.int64)
+
+
 class FusedReconstruction:
-    """Fused training step for a ComplexINRModel2D on a measured stack."""
+    """Fused training step for a ComplexINRModel2D on a measured stack.
+
+    The residual initial fields (``object_initial`` / ``probe_initial``) are
+    buffers of the model that the step reads as complex64: they are copied at
+    construction (a complex128 initial is rounded once), so later changes to
+    the model's buffers do not reach a constructed step.
+    """
 
     def __init__(self, model: ComplexINRModel2D, stack_mnf: torch.Tensor, tlY, tlX, loss_type="smooth_L1_loss",
                  lr=1e-3, batch_size=10, lr_decay_step=10000, lr_decay_gamma=0.1, betas=(0.9, 0.999),
                  eps=1e-8, sparsity_weight=0.0, fuse_table_adam=True):
+        frames, tlY, tlX = check_inputs(model, stack_mnf, tlY, tlX, batch_size)
         dev = model.xy_hash_encoder.params.device
         if dev.type != "cuda":
             raise RuntimeError("FusedReconstruction needs the model on the GPU")
@@ -121,8 +162,12 @@ class FusedReconstruction:
         self.sparsity_weight = float(sparsity_weight)
         if self.sparsity_weight and (self.H % 2 or self.W % 2):
             raise ValueError("the Haar sparsity term needs an even canvas")
-        self.frames = stack_mnf.shape[2]
+        self.frames = frames
         self.stack = prepare_stack(stack_mnf.to(dev))
+        self.obj_init = (_check_initial("object_initial", model.object_initial, (self.H, self.W)).to(dev)
+                         if model.has_object_residual else None)
+        self.prb_init = (_check_initial("probe_initial", model.probe_initial, (self.M, self.N)).to(dev)
+                         if model.has_probe_residual else None)
         self.tl = positions_tensor(tlY, tlX, dev)
 
         for mlp in (model.object_complex_mlp, model.probe_complex_mlp):
@@ -242,12 +287,21 @@ class FusedReconstruction:
         self.cursor = torch.zeros(1, dtype=torch.int32, device=dev)
         self.loss_hist = torch.zeros(self.frames, dtype=torch.float32, device=dev)
         self._from_order = False
+        self._order_len = self._order_used = 0
         self._grad_clear = True   # FlatParams starts with zero gradients
         if self.fp8_states:
-            # calibration: one step without the update records the amax of
-            # every FP8 tensor, so the first real step has scales (its
-            # gradients are cleared by the next step)
-            self.step(torch.arange(min(self.batch_size, self.frames), dtype=torch.int32, device=dev), update=False)
+            # calibration: steps without the update record the amax of every
+            # FP8 tensor, so the first real step has scales (their gradients
+            # are cleared by the next step).  A layer's gradient amax is only
+            # seen once the layer after it has a scale: with the initial
+            # scale 1 a gradient below e4m3's smallest subnormal (2^-9), such
+            # as the probe MLP's behind k_probe = 1e-8, converts to zero and
+            # so does every gradient before it.  Each pass calibrates one more
+            # layer of the backward; n_hidden + 2 passes cover all of them.
+            calib = torch.arange(min(self.batch_size, self.frames), dtype=torch.int32, device=dev)
+            n_pass = max(model.object_complex_mlp.n_hidden(), model.probe_complex_mlp.n_hidden()) + 2
+            for _ in range(n_pass):
+                self.step(calib, update=False)
 
     # ------------------------------------------------------------------ timing
     def enable_timing(self, names):
@@ -305,14 +359,14 @@ class FusedReconstruction:
         C("pty_inr_fwd", self.prb_grid, self.prb_mlp, self.prb_tab.data_ptr(), self.prb_pdesc,
           self.prb_mlp_out.data_ptr(), p)
         if m.has_probe_residual:
-            C("pty_residual_fwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), m.probe_initial.data_ptr(),
+            C("pty_residual_fwd", m.k_probe.data_ptr(), self.prb_mlp_out.data_ptr(), self.prb_init.data_ptr(),
               self.prb_mlp_out.numel(), self.probe.data_ptr(), p)
         C("pty_probe_normalize_fwd", self.probe.data_ptr(), self.M, self.N, self.probe_n.data_ptr(),
           self.stats.data_ptr(), self.ws_probe.data_ptr(), self.ws_probe.numel(), p)
         C("pty_upsample_fwd", self.obj_small.data_ptr(), self.n_small, self.ds, self.obj_up.data_ptr(),
           self.ws_up.data_ptr(), self.ws_up.numel(), s)
         if m.has_object_residual:
-            C("pty_residual_fwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), m.object_initial.data_ptr(),
+            C("pty_residual_fwd", m.k_object.data_ptr(), self.obj_up.data_ptr(), self.obj_init.data_ptr(),
               self.obj_up.numel(), self.obj.data_ptr(), s)
         if self.overlap:
             main.wait_stream(side)
@@ -428,8 +482,13 @@ class FusedReconstruction:
         kernels) and no host synchronisation.
         """
         self.batch_buf = torch.zeros(self.batch_size, dtype=torch.int32, device=self.device)
-        # one eager pass without the update loads every kernel before capture
+        # one eager pass without the update loads every kernel before capture;
+        # it must not move the FP8 scaling state (its batch is position 0
+        # repeated), so the state is put back afterwards
+        saved = [st.clone() for st in self.fp8_states]
         self.step(self.batch_buf, update=False)
+        for st, sv in zip(self.fp8_states, saved):
+            st.copy_(sv)
         _lib.call("pty_fill", self.flat.grad.data_ptr(), self.flat.grad.numel(), 0.0, _lib.stream_ptr(self.device))
         self._grad_clear = True
         torch.cuda.synchronize(self.device)
@@ -453,12 +512,20 @@ class FusedReconstruction:
             raise ValueError(f"order of {n} positions for batch {self.batch_size} and {self.frames} frames")
         self.order[:n].copy_(order, non_blocking=True)
         self.cursor.zero_()
+        # the captured step reads the order buffer of all ``frames`` entries;
+        # replays stop at the end of what was loaded
+        self._order_len = n
+        self._order_used = 0
 
     def graph_next(self):
         """Replay the captured step on the next batch of the loaded order
         (``begin_order``); returns the loss buffer (valid until the next step)."""
         if self.graph is None:
             self.capture()
+        if self._order_used + self.batch_size > self._order_len:
+            raise RuntimeError(f"the loaded order ({self._order_len} positions) has no full batch left after "
+                               f"{self._order_used}; load the next one with begin_order")
+        self._order_used += self.batch_size
         if not self._grad_clear:
             # the captured step starts from cleared gradients; an eager step
             # without the update since the last one left some behind

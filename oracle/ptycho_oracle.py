@@ -15,6 +15,9 @@ Pinning (what checks each function, see tests/test_oracle.py):
   level_table          Appendix-A values (integer scales, dense/hash split),
                        next-multiple-of-8 sizes, totals per log2 size.
   grid_coords          torch.linspace (library) + meshgrid transpose identity.
+  fma_f32              exact rational arithmetic (fractions) with explicit
+                       round-to-nearest-even, on random inputs and on
+                       positions within a few ulps of cell boundaries.
   hash_corners         brute force on tiny grids, dense identity x+y*res,
                        uint32 wrap of the prime hash, edge-corner wrap.
   hashgrid_encode      vertex interpolation identity, weights sum to 1,
@@ -97,11 +100,35 @@ def grid_coords(width: int, height: int) -> torch.Tensor:
     return torch.stack((yv.flatten(), xv.flatten())).t()
 
 
+def fma_f32(x: np.ndarray, s, c) -> np.ndarray:
+    """f32(x*s + c) with ONE rounding (fused multiply-add) for f32 x, s, c.
+
+    The product of two f32 values is exact in f64; TwoSum gives the sum and its
+    exact error; the f64 sum rounded to f32 is the correctly rounded result
+    except when it sits exactly on a midpoint between two f32 values with a
+    nonzero error (double rounding), where the error decides the side.
+    """
+    f64 = np.float64
+    x = np.asarray(x, np.float32)
+    p = x.astype(f64) * f64(np.float32(s))
+    cc = f64(np.float32(c))
+    t = p + cc
+    bp = t - cc
+    e = (p - bp) + (cc - (t - bp))
+    r = t.astype(np.float32)
+    r64 = r.astype(f64)
+    other = np.nextafter(r, np.where(t > r64, np.float32(np.inf), np.float32(-np.inf)).astype(np.float32))
+    mid = (r64 + other.astype(f64)) * 0.5
+    flip = (t == mid) & (t != r64) & (e != 0) & (np.sign(e) == np.sign(other.astype(f64) - r64))
+    return np.where(flip, other, r).astype(np.float32)
+
+
 def hash_corners(coords_f32: np.ndarray, table: dict):
     """Corner entry indices [N, L, 4] and fractional positions [N, L, 2].
 
-    pos = f32(f32(x*scale) + 0.5) (two separately rounded f32 operations),
-    g = floor(pos), frac = pos - g.  Corner c has offsets (c&1, c>>1).
+    pos = fma_f32(x, scale, 0.5) (one rounding, the form of tiny-cuda-nn's
+    pos_fract), g = floor(pos), frac = pos - g (exact in f32).  Corner c has
+    offsets (c&1, c>>1).
     Dense index (cx + cy*res) % size, hashed index
     ((cx * 1) ^ (cy * 2654435761)) mod 2^32 % size.
     """
@@ -112,8 +139,7 @@ def hash_corners(coords_f32: np.ndarray, table: dict):
     frac = np.zeros((N, L, 2), np.float32)
     for l in range(L):
         s = table["scale"][l]
-        pos = (x * s).astype(np.float32) + np.float32(0.5)
-        pos = pos.astype(np.float32)
+        pos = fma_f32(x, s, 0.5)
         g = np.floor(pos)
         frac[:, l, :] = (pos - g).astype(np.float32)
         gi = g.astype(np.int64)

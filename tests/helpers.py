@@ -83,3 +83,23 @@ def rel_err(a, b) -> float:
     den = b.abs().pow(2).sum().sqrt().item()
     num = (a - b).abs().pow(2).sum().sqrt().item()
     return num / den if den > 0 else num
+
+
+def boundary_coords(table, per_level=40, seed=0):
+    """f32 inputs whose position x*scale + 0.5 lies within a few ulps of an
+    integer (a cell boundary) on some level: where a second rounding of the
+    product could move floor() to the other cell."""
+    rng = np.random.default_rng(seed)
+    xs = []
+    for l in range(table["n_levels"]):
+        s = float(table["scale"][l])
+        k = rng.integers(1, max(2, int(s)), per_level)
+        x0 = ((k - 0.5) / s).astype(np.float32)
+        for d in range(-3, 4):
+            v = x0.copy()
+            for _ in range(abs(d)):
+                v = np.nextafter(v, np.float32(np.inf if d > 0 else -np.inf))
+            xs.append(v)
+    x = np.concatenate(xs)
+    x = x[(x > 0) & (x < 1)]
+    return x

@@ -10,19 +10,19 @@ import pytest
 import torch
 
 from oracle import ptycho_oracle as orc
-from tests.helpers import oracle_params, oracle_table, rel_err, step_reference
+from tests.helpers import boundary_coords, oracle_params, oracle_table, rel_err, step_reference
 
 pytestmark = pytest.mark.gpu
 
 
 def _model(pkg, dev, canvas, M, log2T=12, residual=True, probe_init=None, seed=0, hidden_layers=2,
-           table_precision="fp32"):
+           table_precision="fp32", mlp_precision="fp32"):
     torch.manual_seed(seed)
     init = probe_init if probe_init is not None else torch.ones(M, M, dtype=torch.complex64)
     return pkg.ComplexINRModel2D(canvas, canvas, 2, probe_width=M, probe_height=M, log2_hashmap_size=log2T,
                                  hidden_omega_0=1.0, trainable_omega0=True, use_residual=residual,
                                  probe_initial=init if residual else None, hidden_layers=hidden_layers,
-                                 table_precision=table_precision).to(dev)
+                                 table_precision=table_precision, mlp_precision=mlp_precision).to(dev)
 
 
 def _round_tables_to_fp16(params):
@@ -34,13 +34,23 @@ def _round_tables_to_fp16(params):
 
 
 # ---------------------------------------------------------------- hash grid
-@pytest.mark.parametrize("log2T", [14, 18])
+@pytest.mark.parametrize("log2T", [14, 18, 20])
 def test_hash_indices_bit_exact(pkg, dev, log2T):
+    """Corner indices bit-exact against the oracle: random points, a regular
+    grid, points within a few ulps of cell boundaries on every level (where
+    the single rounding of x*scale + 0.5 decides the cell) and, for the
+    canvas-3648 table (log2T 20), a row subsample of its 1824^2 INR grid."""
     spec = pkg.level_table(16, 2, log2T, 16, 1.5)
     table = orc.level_table(16, log2T, 16, 1.5)
     rng = np.random.default_rng(log2T)
-    pts = np.concatenate([rng.random((4000, 2)), orc.grid_coords(128, 128).numpy()[::7],
-                          np.array([[0.0, 0.0], [0.999999, 0.999999], [0.5, 0.25]])]).astype(np.float32)
+    xb = boundary_coords(table, per_level=20, seed=log2T)
+    parts = [rng.random((4000, 2)), orc.grid_coords(128, 128).numpy()[::7],
+             np.stack([xb, rng.permutation(xb)], 1), np.stack([rng.random(xb.size), xb], 1),
+             np.array([[0.0, 0.0], [0.999999, 0.999999], [0.5, 0.25]])]
+    if log2T == 20:
+        g = orc.grid_coords(1824, 1824).numpy().reshape(1824, 1824, 2)
+        parts.append(g[::97].reshape(-1, 2))
+    pts = np.concatenate(parts).astype(np.float32)
     idx = pkg.hashgrid_indices(torch.from_numpy(pts).to(dev), spec).cpu().numpy()
     ref, _ = orc.hash_corners(pts, table)
     assert np.array_equal(idx, ref)
@@ -189,7 +199,7 @@ def test_probe_normalize_fwd_bwd(pkg, dev):
 
 
 # ---------------------------------------------------------------- far field
-@pytest.mark.parametrize("M", [16, 64, 256])
+@pytest.mark.parametrize("M", [16, 64, 256, 512])
 def test_farfield_intensity(pkg, dev, M):
     rng = np.random.default_rng(M)
     H = 2 * M + 24
@@ -207,8 +217,9 @@ def test_farfield_intensity(pkg, dev, M):
         assert err < 1e-5, (b, float(err))
 
 
-@pytest.mark.parametrize("loss_type", ["smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss"])
-@pytest.mark.parametrize("M", [16, 64])
+@pytest.mark.parametrize("loss_type,M", [(lt, m) for m in (16, 64) for lt in
+                                         ("smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss")]
+                         + [("FD_loss", 512), ("smooth_L1_loss", 512)])
 def test_farfield_loss_and_grads(pkg, dev, loss_type, M):
     rng = np.random.default_rng(M + len(loss_type))
     H = 3 * M
@@ -652,9 +663,12 @@ def test_batch_next_walks_an_order(pkg, dev):
     assert torch.equal(hist, hist2) and torch.equal(idx.cpu(), order[10:15].cpu())
 
 
-def test_run_epoch_graph_matches_eager_with_partial_batch(pkg, dev):
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
+def test_run_epoch_graph_matches_eager_with_partial_batch(pkg, dev, precision):
     """run_epoch (graph replays for full batches, the trailing partial batch
-    eager) gives the bits of the all-eager epoch, over two epochs."""
+    eager) gives the bits of the all-eager epoch, over two epochs; for FP8 this
+    includes the delayed-scaling state, which the capture's warm-up pass must
+    leave as it found it."""
     inst = pkg.synthetic.make_named("small", seed=13)
     M = inst.probe.shape[0]
     frames = inst.stack.shape[2]
@@ -663,7 +677,7 @@ def test_run_epoch_graph_matches_eager_with_partial_batch(pkg, dev):
     pinit = torch.from_numpy(inst.probe.astype(np.complex64))
     engines = []
     for _ in range(2):
-        m = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=29)
+        m = _model(pkg, dev, inst.canvas, M, residual=True, probe_init=pinit, seed=29, mlp_precision=precision)
         engines.append(pkg.FusedReconstruction(m, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
                                                batch_size=batch))
     g1 = torch.Generator(device=dev)
@@ -676,6 +690,16 @@ def test_run_epoch_graph_matches_eager_with_partial_batch(pkg, dev):
         assert torch.equal(t1, t2)
     assert engines[0].graph is not None and engines[1].graph is None
     assert torch.equal(engines[0].flat.data, engines[1].flat.data)
+    for a, b in zip(engines[0].fp8_states, engines[1].fp8_states):
+        assert torch.equal(a, b)
+    # the epoch used every full batch of its order: one more replay has no
+    # batch left in it and is refused on the host
+    with pytest.raises(RuntimeError, match="no full batch left"):
+        engines[0].graph_next()
+    engines[0].begin_order(torch.arange(batch, dtype=torch.int32, device=dev))
+    engines[0].graph_next()
+    with pytest.raises(RuntimeError, match="no full batch left"):
+        engines[0].graph_next()
 
 
 def test_fused_step_is_deterministic(pkg, dev):

@@ -6,6 +6,7 @@ torch.nn.functional), brute force on tiny inputs and invariants, each chosen
 so that a plausible slip (dropped term, wrong sign, swapped axis) fails.
 """
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -13,6 +14,7 @@ import torch
 import torch.nn.functional as F
 
 from oracle import ptycho_oracle as orc
+from tests.helpers import boundary_coords
 
 
 # ------------------------------------------------------------------ level table
@@ -68,6 +70,20 @@ def test_grid_coords_match_product_model(pkg):
 
 
 # ------------------------------------------------------------------ hash indices
+def _f32_round_exact(q: Fraction) -> np.float32:
+    """Round a rational to the nearest f32 (ties to even mantissa) by comparing
+    the exact distances to the neighbouring f32 values."""
+    c = np.float32(float(q))
+    cands = [np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))]
+    best = min(cands, key=lambda v: (abs(Fraction(float(v)) - q), int(np.array(v).view(np.uint32)) & 1))
+    return np.float32(best)
+
+
+def _fma_exact(x, s, c=0.5) -> np.float32:
+    return _f32_round_exact(Fraction(float(np.float32(x))) * Fraction(float(np.float32(s)))
+                            + Fraction(float(np.float32(c))))
+
+
 def _brute_indices(x, table):
     """Pure-Python integer arithmetic per point (independent of the vectorised oracle)."""
     out = []
@@ -75,7 +91,7 @@ def _brute_indices(x, table):
         s = np.float32(table["scale"][l])
         row = []
         for c in range(4):
-            pos = [np.float32(np.float32(np.float32(x[d]) * s) + np.float32(0.5)) for d in range(2)]
+            pos = [_fma_exact(x[d], s) for d in range(2)]
             g = [int(math.floor(float(p))) for p in pos]
             cx, cy = g[0] + (c & 1), g[1] + ((c >> 1) & 1)
             res, size = int(table["res"][l]), int(table["size"][l])
@@ -98,12 +114,50 @@ def test_hash_indices_brute_force():
         assert idx[n].tolist() == _brute_indices(pts[n], table)
 
 
+def test_fma_f32_matches_exact_rounding():
+    """The oracle's single-rounding x*s + 0.5 against exact rationals, on random
+    inputs and next to cell boundaries; and the two-rounding form differs
+    from it on some of those inputs (so the choice is visible)."""
+    table = orc.level_table(16, 18, 16, 1.5)
+    rng = np.random.default_rng(5)
+    xb = boundary_coords(table, per_level=6)
+    xr = rng.random(300).astype(np.float32)
+    n_diff = 0
+    for l in range(16):
+        s = np.float32(table["scale"][l])
+        for x in (xr[:40], xb):
+            got = orc.fma_f32(x, s, 0.5)
+            want = np.array([_fma_exact(v, s) for v in x], np.float32)
+            assert np.array_equal(got, want), l
+        two = ((xb * s).astype(np.float32) + np.float32(0.5)).astype(np.float32)
+        n_diff += int((two != orc.fma_f32(xb, s, 0.5)).sum())
+    assert n_diff > 0
+    # a double-rounding case: x*s = 2^-24 + 2^-60 (2^36 + 1 = 4097 * 16773121),
+    # so x*s + 1 rounds in f64 onto the f32 midpoint 1 + 2^-24, whose tie goes
+    # to 1.0, while the exact value lies above it: one rounding gives 1 + 2^-23
+    x = np.float32(4097 * 2.0 ** -30)
+    s = np.float32(16773121 * 2.0 ** -30)
+    assert np.float32(np.float64(x) * np.float64(s) + 1.0) == np.float32(1.0)
+    got = orc.fma_f32(np.array([x]), s, 1.0)[0]
+    assert got == np.float32(1.0 + 2.0 ** -23) == _fma_exact(x, s, 1.0)
+
+
+def test_hash_indices_brute_force_at_cell_boundaries():
+    table = orc.level_table(16, 14, 16, 1.5)
+    xb = boundary_coords(table, per_level=3, seed=3)
+    rng = np.random.default_rng(4)
+    pts = np.stack([xb, rng.permutation(xb)], 1).astype(np.float32)[:120]
+    idx, _ = orc.hash_corners(pts, table)
+    for n in range(pts.shape[0]):
+        assert idx[n].tolist() == _brute_indices(pts[n], table)
+
+
 def test_hash_dense_edge_corner_wraps_into_next_row():
     table = orc.level_table(1, 14, 16, 1.5)   # level 0: scale 15, res 16, size 256
     # x0 near 1: pos0 = 15.5 -> g0 = 15 = res-1, the +1 corner is cx = 16 = res
     x = np.array([[0.999, 0.2]], np.float32)
     idx, frac = orc.hash_corners(x, table)
-    g1 = int(np.floor(np.float32(np.float32(0.2) * np.float32(15.0)) + np.float32(0.5)))
+    g1 = int(np.floor(_fma_exact(0.2, 15.0)))
     assert idx[0, 0, 1] == (16 + g1 * 16) % 256        # wraps to column 0 of the next row
     assert idx[0, 0, 1] == (g1 + 1) * 16
 
@@ -115,8 +169,8 @@ def test_hash_prime_uint32_wrap():
     x = np.array([[0.77, 0.93]], np.float32)
     idx, _ = orc.hash_corners(x, table)
     s = np.float32(table["scale"][l])
-    gx = int(np.floor(np.float32(np.float32(0.77) * s) + np.float32(0.5)))
-    gy = int(np.floor(np.float32(np.float32(0.93) * s) + np.float32(0.5)))
+    gx = int(np.floor(_fma_exact(0.77, s)))
+    gy = int(np.floor(_fma_exact(0.93, s)))
     # 2654435761 * gy overflows 32 bits for gy >= 2: the wrap is part of the hash
     assert gy * 2654435761 >= 2**32
     expect = ((gx ^ ((gy * 2654435761) & 0xFFFFFFFF)) & 0xFFFFFFFF) % table["size"][l]
