@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libptycho_hip.so"
 ABI_VERSION = 6
 MAX_LEVELS = 16
 MAX_HIDDEN = 4
+FAR_SCRATCH_BYTES = 16 << 20   # PTY_FAR_SCRATCH_BYTES
 
 LOSS_CODES = {"smooth_L1_loss": 0, "FD_loss": 1, "Poisson_likelihood_loss": 2}
 PREC_CODES = {"fp32": 0, "bf16": 1, "fp8": 2}
@@ -93,6 +94,8 @@ _SIGNATURES = {
     "pty_farfield_workspace": (c_size_t, [P(PtyFar)]),
     "pty_farfield_fwd_bwd": (c_int32, [P(PtyFar), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_farfield_fwd_bwd_noalloc": (c_int32, [P(PtyFar), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                               c_void_p, c_void_p, c_void_p, c_void_p]),
     "pty_farfield_intensity": (c_int32, [P(PtyFar), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_size_t, c_void_p]),
     "pty_adam": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_float, c_float,

@@ -22,6 +22,8 @@
 //              F4 and F5 share one launch.
 #include "common.h"
 
+#include <mutex>
+
 namespace pty {
 
 struct FarArgs {
@@ -442,15 +444,13 @@ extern "C" size_t pty_farfield_workspace(const PtyFar* d) {
   return far_layout(d).total;
 }
 
-extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* probe_n, const float* stack,
-                                    const int32_t* batch_idx, const int32_t* tl, float* loss, float* dobject,
-                                    float* dprobe, void* work, size_t work_bytes, hipStream_t stream) {
-  int rc = check_far(d);
-  if (rc) return rc;
-  PTY_REQUIRE(object && probe_n && stack && batch_idx && tl && loss && dobject && dprobe, PTY_EINVAL,
-              "farfield_fwd_bwd: null pointer");
-  const FarLayout L = far_layout(d);
-  PTY_REQUIRE(work && work_bytes >= L.total, PTY_EWORK, "farfield_fwd_bwd: workspace %zu < %zu", work_bytes, L.total);
+namespace pty {
+
+// The far-field launch sequence over a given workspace (pty_farfield_fwd_bwd
+// and pty_farfield_fwd_bwd_noalloc).
+static int far_fwd_bwd_launch(const PtyFar* d, const FarLayout& L, const float* object, const float* probe_n,
+                              const float* stack, const int32_t* batch_idx, const int32_t* tl, float* loss,
+                              float* dobject, float* dprobe, void* work, hipStream_t stream) {
   FarArgs a = far_args(d, L, object, probe_n, stack, batch_idx, tl, loss, dobject, dprobe, work);
   launch_rows_fwd(a, stream);
   if (d->loss_type == PTY_LOSS_FD) {
@@ -469,6 +469,65 @@ extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const 
   const int64_t tiles = (int64_t)((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);
   hipLaunchKernelGGL(k_far_dgrad, dim3((unsigned)(probe_blocks + tiles)), dim3(256), 0, stream, a, probe_blocks);
   return check_launch("farfield_fwd_bwd");
+}
+
+// Scratch of pty_farfield_fwd_bwd_noalloc: a file-scope device array, one
+// copy per GPU (each holds it until the process ends).  Host code never
+// passes the array's name to a kernel (on the host it is a stand-in, not a
+// device address): its device address is looked up once per device with
+// hipGetSymbolAddress, with that device current, and kept here.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char far_scratch[PTY_FAR_SCRATCH_BYTES];
+static void* g_far_scratch[kMaxDevices];
+static std::mutex g_far_scratch_mu;
+
+static int far_scratch_ptr(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(PTY_ELAUNCH, "farfield_fwd_bwd_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  PTY_REQUIRE(dev >= 0 && dev < kMaxDevices, PTY_EUNSUP, "farfield_fwd_bwd_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_far_scratch_mu);
+  if (!g_far_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(far_scratch));
+    if (e != hipSuccess || !p)
+      return set_error(PTY_ELAUNCH, "farfield_fwd_bwd_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_far_scratch[dev] = p;
+  }
+  *out = g_far_scratch[dev];
+  return PTY_OK;
+}
+
+}  // namespace pty
+
+extern "C" int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* probe_n, const float* stack,
+                                    const int32_t* batch_idx, const int32_t* tl, float* loss, float* dobject,
+                                    float* dprobe, void* work, size_t work_bytes, hipStream_t stream) {
+  int rc = check_far(d);
+  if (rc) return rc;
+  PTY_REQUIRE(object && probe_n && stack && batch_idx && tl && loss && dobject && dprobe, PTY_EINVAL,
+              "farfield_fwd_bwd: null pointer");
+  const FarLayout L = far_layout(d);
+  PTY_REQUIRE(work && work_bytes >= L.total, PTY_EWORK, "farfield_fwd_bwd: workspace %zu < %zu", work_bytes, L.total);
+  return far_fwd_bwd_launch(d, L, object, probe_n, stack, batch_idx, tl, loss, dobject, dprobe, work, stream);
+}
+
+extern "C" int pty_farfield_fwd_bwd_noalloc(const PtyFar* d, const float* object, const float* probe_n,
+                                            const float* stack, const int32_t* batch_idx, const int32_t* tl,
+                                            float* loss, float* dobject, float* dprobe, hipStream_t stream) {
+  int rc = check_far(d);
+  if (rc) return rc;
+  PTY_REQUIRE(object && probe_n && stack && batch_idx && tl && loss && dobject && dprobe, PTY_EINVAL,
+              "farfield_fwd_bwd_noalloc: null pointer");
+  const FarLayout L = far_layout(d);
+  PTY_REQUIRE(L.total <= (size_t)PTY_FAR_SCRATCH_BYTES, PTY_EWORK,
+              "farfield_fwd_bwd_noalloc: this batch needs %zu bytes of scratch, more than the %zu of the static "
+              "scratch (use pty_farfield_fwd_bwd with a workspace, or a smaller batch)",
+              L.total, (size_t)PTY_FAR_SCRATCH_BYTES);
+  void* scratch = nullptr;
+  rc = far_scratch_ptr(&scratch);
+  if (rc) return rc;
+  return far_fwd_bwd_launch(d, L, object, probe_n, stack, batch_idx, tl, loss, dobject, dprobe, scratch, stream);
 }
 
 // Intensity output in the reference's fftshifted order.

@@ -242,6 +242,20 @@ int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* prob
                          const float* stack, const int32_t* batch_idx, const int32_t* tl,
                          float* loss, float* dobject, float* dprobe, void* work,
                          size_t work_bytes, hipStream_t stream);
+/* pty_farfield_fwd_bwd without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a static device array of PTY_FAR_SCRATCH_BYTES per GPU (held
+ * until the process ends); a batch whose pty_farfield_workspace exceeds it is
+ * rejected with PTY_EWORK (16 MiB: e.g. batch 12 at 256 x 256 with FD, batch 51
+ * at 128 x 128).  The library allocates nothing and the kernels need no
+ * cleared memory.  Every call on one GPU shares that GPU's scratch: calls
+ * that overlap on one GPU (different streams or threads) race.  Same kernels
+ * and same bits as pty_farfield_fwd_bwd. */
+#define PTY_FAR_SCRATCH_BYTES (16u << 20)
+int pty_farfield_fwd_bwd_noalloc(const PtyFar* d, const float* object, const float* probe_n,
+                                 const float* stack, const int32_t* batch_idx, const int32_t* tl,
+                                 float* loss, float* dobject, float* dprobe, hipStream_t stream);
 /* Intensities [n_batch, M, N] = |fftshift(fft2(probe_n * patch))|^2 (forward only). */
 int pty_farfield_intensity(const PtyFar* d, const float* object, const float* probe_n,
                            const int32_t* batch_idx, const int32_t* tl, float* intensity,
