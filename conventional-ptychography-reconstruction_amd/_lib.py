@@ -57,6 +57,13 @@ class PtyFar(ctypes.Structure):
     ]
 
 
+class PtyLayer(ctypes.Structure):
+    _fields_ = [
+        ("n_in", c_int32), ("n_out", c_int32), ("complex_io", c_int32), ("activation", c_int32),
+        ("weight", c_void_p), ("bias", c_void_p), ("omega", c_void_p),
+    ]
+
+
 P = ctypes.POINTER
 _SIGNATURES = {
     "pty_abi_version": (c_int32, []),
@@ -78,6 +85,10 @@ _SIGNATURES = {
     "pty_points_feature_floats": (c_size_t, [c_int64]),
     "pty_points_plan_build": (c_int32, [P(PtyGrid), P(PtyPoints), c_void_p, c_size_t, c_void_p]),
     "pty_mlp_fwd": (c_int32, [P(PtyMlp), c_void_p, c_int64, c_void_p, c_void_p]),
+    "pty_layer_fwd": (c_int32, [P(PtyLayer), c_void_p, c_int64, c_void_p, c_void_p]),
+    "pty_layer_bwd_workspace": (c_size_t, [P(PtyLayer), c_int64]),
+    "pty_layer_bwd": (c_int32, [P(PtyLayer), c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_size_t, c_void_p]),
     "pty_upsample_workspace": (c_size_t, [c_int32, c_int32]),
     "pty_upsample_fwd": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pty_upsample_adj": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),

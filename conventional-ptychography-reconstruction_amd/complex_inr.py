@@ -26,6 +26,63 @@ from .levels import GridSpec
 # ----------------------------------------------------------------------------
 # modules with the reference's parameters
 # ----------------------------------------------------------------------------
+class LayerFn(torch.autograd.Function):
+    """One layer of ComplexMLP.net on the device (``pty_layer_fwd/bwd``):
+    exp(1j * omega * (W x + b)) with ``activation``, W x + b without."""
+
+    @staticmethod
+    def forward(ctx, x, weight, bias, omega, activation: bool):
+        _lib.require_cuda(x, weight, bias, omega)
+        cplx = weight.is_complex()
+        n_out, n_in = weight.shape
+        if x.is_complex() != cplx:
+            raise RuntimeError(f"layer input is {x.dtype}, weight is {weight.dtype}")
+        if x.shape[-1] != n_in:
+            raise RuntimeError(f"layer input has {x.shape[-1]} features, the layer takes {n_in}")
+        x2 = x.detach().resolve_conj().reshape(-1, n_in).to(torch.complex64 if cplx else torch.float32).contiguous()
+        w = weight.detach().resolve_conj().contiguous()
+        b = bias.detach().resolve_conj().contiguous() if bias is not None else None
+        om = omega.detach().contiguous() if omega is not None else None
+        n = x2.shape[0]
+        out_dtype = torch.complex64 if (activation or cplx) else torch.float32
+        out = torch.empty(n, n_out, dtype=out_dtype, device=x.device)
+        d = _layer_descriptor(w, b, om, activation)
+        _lib.call("pty_layer_fwd", d, x2.data_ptr(), n, out.data_ptr(), _lib.stream_ptr(x.device))
+        ctx.save_for_backward(x2, w, b, om)
+        ctx.activation, ctx.lead = activation, x.shape[:-1]
+        return out.reshape(*x.shape[:-1], n_out)
+
+    @staticmethod
+    def backward(ctx, gout):
+        x2, w, b, om = ctx.saved_tensors
+        n, n_in = x2.shape
+        n_out = w.shape[0]
+        d = _layer_descriptor(w, b, om, ctx.activation)
+        g = gout.resolve_conj().reshape(n, n_out).contiguous()
+        need_x, need_w, need_b, need_om, _ = ctx.needs_input_grad
+        gx = torch.empty_like(x2) if need_x else None
+        gw = torch.empty_like(w) if need_w else None
+        gb = torch.empty_like(b) if (need_b and b is not None) else None
+        gom = torch.empty_like(om) if (need_om and om is not None) else None
+        ws = _lib.workspace(_lib.query("pty_layer_bwd_workspace", d, n), x2.device)
+        _lib.call("pty_layer_bwd", d, x2.data_ptr(), n, g.data_ptr(), _lib.ptr(gx), _lib.ptr(gw), _lib.ptr(gb),
+                  _lib.ptr(gom), ws.data_ptr(), ws.numel(), _lib.stream_ptr(x2.device))
+        if gx is not None:
+            gx = gx.reshape(*ctx.lead, n_in)
+        return gx, gw, gb, gom, None
+
+
+def _layer_descriptor(w, b, om, activation) -> _lib.PtyLayer:
+    d = _lib.PtyLayer()
+    d.n_out, d.n_in = w.shape
+    d.complex_io = int(w.is_complex())
+    d.activation = int(bool(activation))
+    d.weight = w.data_ptr()
+    d.bias = b.data_ptr() if b is not None else None
+    d.omega = om.data_ptr() if (activation and om is not None) else None
+    return d
+
+
 class ComplexGaborLayer2D(nn.Module):
     """exp(1j * omega_0 * Linear(x)); real Linear when is_first (complex_inr.py:6-40)."""
 
@@ -36,6 +93,18 @@ class ComplexGaborLayer2D(nn.Module):
         dtype = torch.float if is_first else torch.cfloat
         self.omega_0 = nn.Parameter(omega0 * torch.ones(1), trainable)
         self.linear = nn.Linear(in_features, out_features, bias=bias, dtype=dtype)
+
+    def forward(self, input):
+        """exp(1j * omega_0 * linear(input)) (complex_inr.py:38-40), one HIP layer launch."""
+        return LayerFn.apply(input, self.linear.weight, self.linear.bias, self.omega_0, True)
+
+
+class ComplexLinear(nn.Linear):
+    """The final ``nn.Linear(hidden, out, dtype=cfloat)`` of ComplexMLP.net
+    (complex_inr.py:73-78): same parameters, forward on the HIP layer kernel."""
+
+    def forward(self, input):
+        return LayerFn.apply(input, self.weight, self.bias, None, False)
 
 
 class ComplexMLP(nn.Module):
@@ -50,7 +119,7 @@ class ComplexMLP(nn.Module):
         for _ in range(hidden_layers):
             net.append(ComplexGaborLayer2D(hidden_features, hidden_features, omega0=hidden_omega_0,
                                            trainable=trainable_omega0))
-        net.append(nn.Linear(hidden_features, out_features, dtype=torch.cfloat))
+        net.append(ComplexLinear(hidden_features, out_features, dtype=torch.cfloat))
         self.net = nn.Sequential(*net)
         self.precision = "fp32"
         self._fp8_state = None
@@ -84,16 +153,28 @@ class ComplexMLP(nn.Module):
         if not 1 <= len(layers) - 2 <= 2:
             raise ValueError("the gfx950 MLP kernels implement hidden_layers in 1..2")
 
+    def _fused_ok(self, coords) -> bool:
+        try:
+            self.check_supported()
+        except ValueError:
+            return False
+        return coords.dim() == 2 and not coords.is_complex()
+
     def forward(self, coords):
         """ComplexMLP.forward: ``.real`` of the network output (complex_inr.py:80-82).
 
-        Forward-only on the HIP path (``pty_mlp_fwd``); the trained path of the
-        model is the fused encoder+MLP of ComplexINRModel2D.
+        Without autograd the whole network runs in one fused launch
+        (``pty_mlp_fwd``); with autograd (or a network shape the fused kernel
+        does not take) layer by layer (``pty_layer_fwd/bwd``).
         """
-        if torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters()):
-            raise NotImplementedError("backward through a standalone ComplexMLP is not provided on the "
-                                      "HIP path; differentiate ComplexINRModel2D.forward instead")
-        return mlp_forward(self, coords).real
+        return self.complex_forward(coords).real
+
+    def complex_forward(self, coords):
+        """The complex network output ``net(coords)`` (what _get_complex_output returns)."""
+        grad = torch.is_grad_enabled() and (coords.requires_grad or any(p.requires_grad for p in self.parameters()))
+        if not grad and self._fused_ok(coords):
+            return mlp_forward(self, coords)
+        return self.net(coords)
 
 
 FP8_STATE_FLOATS = 520   # PTY_FP8_STATE_FLOATS: 8 scales + 8 x 64 amax sub-slots
@@ -355,7 +436,8 @@ class ComplexINRModel2D(nn.Module):
         self.has_probe_residual = False
         if self.use_residual:
             self.init_residual_params(object_initial, probe_initial)
-        self._points = {}
+        self._grid_points = {}
+        self._grid_checked = {}
 
     def init_scale_grids(self, downsample_factor):
         downsampled_width = self.output_width // downsample_factor
@@ -407,18 +489,51 @@ class ComplexINRModel2D(nn.Module):
         """Device-resident regular grid of the object ('object') or probe ('probe')."""
         dev = self.xy_hash_encoder.params.device
         key = (which, str(dev))
-        if key not in self._points:
+        if key not in self._grid_points:
             if which == "object":
-                self._points[key] = GridPoints(self.downsampled_width, self.downsampled_height, dev)
+                self._grid_points[key] = GridPoints(self.downsampled_width, self.downsampled_height, dev)
             else:
-                self._points[key] = GridPoints(self.probe_width, self.probe_height, dev)
-        return self._points[key]
+                self._grid_points[key] = GridPoints(self.probe_width, self.probe_height, dev)
+        return self._grid_points[key]
+
+    def coords_are_grid(self, which: str) -> bool:
+        """Whether ``xy_coords`` ('object') / ``probe_coords`` ('probe') still
+        hold the regular grid of init_scale_grids.  Checked on the device when
+        the tensor has changed (new storage or an in-place write) since the
+        last check, cached otherwise."""
+        c = self.xy_coords if which == "object" else self.probe_coords
+        key = (which, c.data_ptr(), c._version, str(c.device))
+        hit = self._grid_checked.get(which)
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        gp = self.grid_points(which)
+        ok = tuple(c.shape) == (gp.n, 2)
+        if ok:
+            xs, ys = gp.xs.to(c.device), gp.ys.to(c.device)
+            ok = (torch.equal(c[:, 0].reshape(gp.width, gp.height), ys.expand(gp.width, gp.height))
+                  and torch.equal(c[:, 1].reshape(gp.width, gp.height), xs[:, None].expand(gp.width, gp.height)))
+        self._grid_checked[which] = (key, ok)
+        return ok
+
+    def _points(self, which: str):
+        """The regular grid (kernels generate its coordinates from two axis
+        vectors) while the coordinates are the default ones; the coordinate
+        list itself once a caller has changed them."""
+        if self.coords_are_grid(which):
+            return self.grid_points(which)
+        c = self.xy_coords if which == "object" else self.probe_coords
+        return CoordPoints(c.detach())
 
     def _field(self, encoder: HashGridEncoding, mlp: ComplexMLP, which: str):
         mlp.check_supported()
         state = mlp.fp8_state(encoder.params.device) if self.mlp_precision == "fp8" else None
-        return InrFieldFn.apply(encoder.spec, self.grid_points(which), self.mlp_precision, encoder.table_precision,
+        return InrFieldFn.apply(encoder.spec, self._points(which), self.mlp_precision, encoder.table_precision,
                                 state, encoder.params, *mlp.tensors())
+
+    def _get_complex_output(self, model, coords):
+        """``model.net(coords)``: the complex network output, bypassing the
+        ``.real`` of ComplexMLP.forward (complex_inr.py:308-314)."""
+        return model.complex_forward(coords)
 
     def fourier_upsample(self, x, scale_factor):
         return fourier_upsample(x, scale_factor)

@@ -118,6 +118,11 @@ def check_inputs(model, stack_mnf, tlY, tlX, batch_size):
     if int(batch_size) < 1:
         raise ValueError("batch_size must be positive")
     check_windows(tlY, tlX, M, N, model.output_width, model.output_height)
+    for which in ("object", "probe"):
+        if not model.coords_are_grid(which):
+            raise ValueError(f"the fused step evaluates the INR on the regular {which} grid of init_scale_grids; "
+                             f"the model's {which} coordinates were changed (use model.forward() for other "
+                             f"coordinates)")
     if model.has_object_residual:
         _check_initial("object_initial", model.object_initial, (model.output_width, model.output_height))
     if model.has_probe_residual:

@@ -126,15 +126,40 @@ def build(cfg: dict, data: dict, device="cuda", object_initial=None):
     return model, engine, geom
 
 
+def write_results(result_dir, engine, model):
+    """The files Recovery.py:377-383 writes every ``vis_interval`` epochs (its
+    PNG plots aside): ``result.mat`` with ``object_complex`` and
+    ``probe_complex`` -- the object and the centred, energy-normalised probe
+    of the epoch's last step, as the reference's loop variables hold them
+    then (computed before that step's Adam update) -- and
+    ``trained_models.pth`` (save_model: the requires_grad parameters in
+    named_parameters order)."""
+    from scipy.io import savemat
+    os.makedirs(result_dir, exist_ok=True)
+    savemat(os.path.join(result_dir, "result.mat"), {
+        "object_complex": engine.obj.detach().squeeze().cpu().numpy(),
+        "probe_complex": engine.probe_n.detach().squeeze().cpu().numpy(),
+    })
+    save_model(model, os.path.join(result_dir, "trained_models.pth"))
+
+
 def reconstruct(cfg: dict, data: dict, device="cuda", epochs: int | None = None, callback=None):
-    """Run ``iters`` epochs of the fused step; returns (model, engine, geom, losses)."""
+    """Run ``iters`` epochs of the fused step; returns (model, engine, geom, losses).
+
+    With ``result_dir`` in the config (Recovery.py:140-141), the results are
+    written there after epoch 0, every ``vis_interval`` epochs (default 100)
+    and after the last epoch (Recovery.py:352), as ``write_results`` does."""
     model, engine, geom = build(cfg, data, device)
     epochs = cfg.get("iters", 2000) if epochs is None else epochs
+    result_dir = cfg.get("result_dir")
+    vis_interval = int(cfg.get("vis_interval", 100) or 100)
     gen = torch.Generator(device=device)
     gen.manual_seed(int(cfg.get("seed", 0) or 0))
     losses = []
     for ep in range(epochs):
         losses.append(engine.run_epoch(gen))
+        if result_dir and (ep % vis_interval == 0 or ep == epochs - 1):
+            write_results(result_dir, engine, model)
         if callback is not None:
             callback(ep, engine)
     return model, engine, geom, torch.stack(losses).cpu().numpy() if losses else np.zeros(0)

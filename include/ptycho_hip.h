@@ -201,6 +201,30 @@ int pty_fp8_roll(float* fp8_state, hipStream_t stream);
 /* ComplexMLP.net over given features [n, n_in] (complex_inr.py:313). */
 int pty_mlp_fwd(const PtyMlp* mlp, const float* feats, int64_t n, float* out, hipStream_t stream);
 
+/* ---- single layers of ComplexMLP.net (complex_inr.py:38-40, 73-78, 308-314) ----
+ * The fused kernels above evaluate the whole network; these run one layer,
+ * for callers that use the modules layer by layer.
+ *   activation 1: ComplexGaborLayer2D.forward, out = exp(1j*omega*(W x + b))
+ *   activation 0: nn.Linear, out = W x + b
+ *   complex_io 0: real x [n, n_in], weight [n_out, n_in], bias [n_out] (the
+ *   first Gabor layer); 1: all complex (interleaved).  The output is complex
+ *   [n, n_out] unless both flags are 0 (real). */
+typedef struct PtyLayer {
+  int32_t n_in, n_out;       /* 1 .. 4096 */
+  int32_t complex_io;
+  int32_t activation;
+  const float* weight;
+  const float* bias;         /* may be NULL (bias=False) */
+  const float* omega;        /* [1]; activation only */
+} PtyLayer;
+int pty_layer_fwd(const PtyLayer* d, const float* x, int64_t n, float* out, hipStream_t stream);
+size_t pty_layer_bwd_workspace(const PtyLayer* d, int64_t n);
+/* Given gout (the output's gradient, PyTorch's complex convention), writes
+ * gx (d x, may be NULL) and overwrites gweight / gbias / gomega (each may be
+ * NULL).  Deterministic (fixed-order sums over point ranges). */
+int pty_layer_bwd(const PtyLayer* d, const float* x, int64_t n, const float* gout, float* gx, float* gweight,
+                  float* gbias, float* gomega, void* work, size_t work_bytes, hipStream_t stream);
+
 /* ---- Fourier upsample (complex_inr.py:252-271) and its adjoint ---- */
 size_t pty_upsample_workspace(int32_t n, int32_t ds);
 /* big [ds*n, ds*n] c64 = ds^2 ifft2(ifftshift(pad(fftshift(fft2(small))))). */

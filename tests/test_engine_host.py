@@ -77,3 +77,18 @@ def test_batch_size_is_checked(pkg):
     tlY, tlX = _windows(5)
     with pytest.raises(ValueError, match="batch_size"):
         pkg.FusedReconstruction(m, torch.zeros(16, 16, 5), tlY, tlX, batch_size=0)
+
+
+def test_changed_coordinates_are_rejected_by_the_fused_step(pkg):
+    m = _model(pkg)
+    assert m.coords_are_grid("object") and m.coords_are_grid("probe")
+    orig = m.xy_coords[5, 0].item()
+    with torch.no_grad():
+        m.xy_coords[5, 0] += 0.01
+    assert not m.coords_are_grid("object") and m.coords_are_grid("probe")
+    tlY, tlX = _windows(5)
+    with pytest.raises(ValueError, match="object coordinates were changed"):
+        pkg.FusedReconstruction(m, torch.zeros(16, 16, 5), tlY, tlX)
+    with torch.no_grad():
+        m.xy_coords[5, 0] = orig
+    assert m.coords_are_grid("object")
