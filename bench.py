@@ -439,7 +439,7 @@ def sweep_mode(args):
     inst = pkg.synthetic.make_instance(seed=0, **shapes)
     M = inst.probe.shape[0]
     frames = inst.stack.shape[2]
-    mask = pkg.metrics.coverage_mask(inst.tlY, inst.tlX, M, M, inst.obj.shape, min_count=2)
+    mask = pkg.metrics.evaluation_mask(inst.tlY, inst.tlX, M, M, inst.obj.shape)
     rng = np.random.default_rng(1)
     batches = [[perm[b:b + batch] for b in range(0, frames, batch)]
                for perm in (rng.permutation(frames) for _ in range(args.epochs))]
@@ -543,7 +543,7 @@ def warmstart_mode(args):
         inst = ws.simulate(900000 + k, jitter_probe=False, device=dev, **shape)
         truth = inst.obj.cpu().numpy()
         ty, tx = inst.tlY.cpu().numpy(), inst.tlX.cpu().numpy()
-        mask = pkg.metrics.coverage_mask(ty, tx, M, M, truth.shape, min_count=2)
+        mask = pkg.metrics.evaluation_mask(ty, tx, M, M, truth.shape)
         stack = inst.intens.permute(1, 2, 0).contiguous()          # reference layout [M, N, frames]
 
         def err(o):
@@ -612,7 +612,7 @@ def nrmse_mode(args):
     inst = pkg.synthetic.make_instance(seed=0, **shapes)
     M = inst.probe.shape[0]
     log2T = 14 if args.workload == "cfg0" else LOG2T
-    mask = pkg.metrics.coverage_mask(inst.tlY, inst.tlX, M, M, inst.obj.shape, min_count=2)
+    mask = pkg.metrics.evaluation_mask(inst.tlY, inst.tlX, M, M, inst.obj.shape)
 
     def make():
         torch.manual_seed(0)

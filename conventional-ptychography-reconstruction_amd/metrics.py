@@ -2,7 +2,7 @@
 NRMSE (below) and the reference's SSIM evaluation (end of the file).
 
 Follows the recipe of scripts/compare_object_ssim.py:42-103 in the reference
-(coverage mask from the scan footprints, integer alignment by magnitude
+(evaluation region from the scan footprints, integer alignment by magnitude
 cross-correlation, global phase), with a complex scale c = <r, g> / <r, r>
 (phase and amplitude) because the energy normalisation fixes the probe
 scale, not the object scale:  NRMSE = ||c r - g|| / ||g|| over the mask.
@@ -17,6 +17,22 @@ def coverage_mask(tlY, tlX, M, N, shape, min_count: int = 1) -> np.ndarray:
     for y, x in zip(np.asarray(tlY), np.asarray(tlX)):
         cov[y:y + M, x:x + N] += 1
     return cov >= min_count
+
+
+def evaluation_mask(tlY, tlX, M, N, shape, frac: float = 0.5) -> np.ndarray:
+    """The region the reference evaluates a reconstruction on: the bounding
+    box of the pixels covered by at least ceil(frac * max) scan footprints
+    (compare_object_ssim.py:64-72, frac 0.5).  Unlike every pixel under some
+    footprint, it leaves out the footprint corners a probe smaller than its
+    window never illuminates, where the measurements do not determine the
+    object (an INR puts arbitrary values there)."""
+    cov = np.zeros(shape, np.int32)
+    for y, x in zip(np.asarray(tlY), np.asarray(tlX)):
+        cov[y:y + M, x:x + N] += 1
+    y0, y1, x0, x1 = high_coverage_bbox(cov, frac)
+    m = np.zeros(shape, bool)
+    m[y0:y1, x0:x1] = True
+    return m
 
 
 def xcorr_shift(recon_mag: np.ndarray, gt_mag: np.ndarray, max_shift: int) -> tuple[int, int]:

@@ -62,3 +62,22 @@ def test_high_coverage_bbox_and_coverage_ssim(pkg):
     out = pkg.metrics.object_ssim(gt, gt, coverage=cov, coverage_frac=0.5)
     assert out["coverage_bbox"] == (5, 25, 3, 28)
     assert out["ssim_amp_cov"] == pytest.approx(1.0, abs=1e-6)
+
+
+def test_evaluation_mask_is_the_high_coverage_box(pkg):
+    """The evaluation region: bbox of pixels under >= ceil(0.5 * max) footprints
+    (compare_object_ssim.py:64-72), here checked against explicit counting."""
+    tlY = np.array([0, 0, 10, 10, 20])
+    tlX = np.array([0, 10, 0, 10, 20])
+    m = pkg.metrics.evaluation_mask(tlY, tlX, 16, 16, (40, 40))
+    cov = np.zeros((40, 40), int)
+    for y, x in zip(tlY, tlX):
+        cov[y:y + 16, x:x + 16] += 1
+    assert cov.max() == 4
+    ys, xs = np.where(cov >= 2)
+    want = np.zeros((40, 40), bool)
+    want[ys.min():ys.max() + 1, xs.min():xs.max() + 1] = True
+    assert np.array_equal(m, want)
+    # single footprint: everything under it
+    m1 = pkg.metrics.evaluation_mask([3], [5], 8, 8, (20, 20))
+    assert m1.sum() == 64 and m1[3:11, 5:13].all()

@@ -40,7 +40,7 @@ def test_warm_start_gains_over_plain_start(pkg, dev):
     inst = ws.simulate(7, jitter_probe=False, device=dev, **shape)
     truth = inst.obj.cpu().numpy()
     ty, tx = inst.tlY.cpu().numpy(), inst.tlX.cpu().numpy()
-    mask = pkg.metrics.coverage_mask(ty, tx, 64, 64, truth.shape, min_count=2)
+    mask = pkg.metrics.evaluation_mask(ty, tx, 64, 64, truth.shape)
     stack = inst.intens.permute(1, 2, 0).contiguous()
 
     def err(o):
