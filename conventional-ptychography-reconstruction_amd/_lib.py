@@ -115,6 +115,12 @@ _SIGNATURES = {
     "pty_batch_next": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_int64, c_void_p]),
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
+    "pty_wsnet_param_count": (c_size_t, [c_int32]),
+    "pty_wsnet_workspace": (c_size_t, [c_int32, c_int32]),
+    "pty_wsnet_inputs": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
+    "pty_wsnet_forward": (c_int32, [c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_wsnet_mse": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_wsnet_backward": (c_int32, [c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pty_cast_f16": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "pty_fp8_roll": (c_int32, [c_void_p, c_void_p]),
     "pty_haar_workspace": (c_size_t, [c_int64, c_int32]),

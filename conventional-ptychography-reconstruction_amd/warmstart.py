@@ -9,8 +9,11 @@ use_residual=True)`` (complex_inr.py:209-250: object = k * INR + initial).
 * Input per position: log(1 + I) average-pooled to 64 x 64, plus two constant
   planes holding the pattern's normalised scan offset (window centre / canvas
   - 1/2).
-* Net: 4-level conv encoder-decoder, 32 base channels, run in BF16 (autocast;
-  the convolutions are PyTorch-ROCm's MFMA kernels) with FP32 master weights.
+* Net: 4-level conv encoder-decoder, 32 base channels (``WarmStartNet``
+  defines the architecture and draws the initial weights); forward, backward
+  and Adam run on this package's HIP kernels (``HipWarmStartNet``,
+  ``pty_wsnet_*``: im2col + BF16 MFMA GEMMs with FP32 accumulation, FP32
+  master weights).
 * Output per position: a 32 x 32 complex object patch (2 channels) = the
   object under the probe window, average-pooled by M / 32, in the scale the
   reconstruction converges to: the step normalises the probe to energy M*N
@@ -32,6 +35,7 @@ import torch
 import torch.nn as nn
 import torch.nn.functional as F
 
+from . import _lib
 from .farfield import farfield_intensity
 
 IN_SIZE = 64
@@ -172,21 +176,126 @@ class WarmStartNet(nn.Module):
         return self.out(y2)                   # [B, 2, 32, 32]
 
 
+def pack_params(module: WarmStartNet) -> torch.Tensor:
+    """The module's parameters as the flat f32 vector of the HIP kernels:
+    torch's parameter order, conv weights [Co][kh][kw][Ci], transposed-conv
+    weights [kh][kw][Co][Ci]."""
+    parts = []
+    for mod in module.modules():
+        if isinstance(mod, nn.ConvTranspose2d):
+            parts += [mod.weight.detach().permute(2, 3, 1, 0).reshape(-1), mod.bias.detach().reshape(-1)]
+        elif isinstance(mod, nn.Conv2d):
+            parts += [mod.weight.detach().permute(0, 2, 3, 1).reshape(-1), mod.bias.detach().reshape(-1)]
+    return torch.cat(parts).float().contiguous()
+
+
+def _module_order_matches(module: WarmStartNet) -> bool:
+    """modules() visits the convolutions in parameters() order."""
+    convs = [m for m in module.modules() if isinstance(m, (nn.Conv2d, nn.ConvTranspose2d))]
+    flat = [p for m in convs for p in (m.weight, m.bias)]
+    return [id(p) for p in flat] == [id(p) for p in module.parameters()]
+
+
+class HipWarmStartNet:
+    """WarmStartNet trained and evaluated on the HIP kernels (pty_wsnet_*).
+
+    FP32 master parameters in one flat buffer (``pack_params`` layout), their
+    gradients and Adam's moments beside them; every GEMM in BF16 on MFMA with
+    FP32 accumulation; Adam is ``pty_adam`` (torch.optim.Adam semantics)."""
+
+    def __init__(self, base: int = 32, device="cuda", seed: int = 0, module: WarmStartNet | None = None):
+        if module is None:
+            with torch.random.fork_rng(devices=[]):
+                torch.manual_seed(seed)
+                module = WarmStartNet(base)
+        assert _module_order_matches(module)
+        self.base = base
+        self.device = torch.device(device)
+        n = _lib.query("pty_wsnet_param_count", base)
+        self.params = pack_params(module).to(self.device)
+        if self.params.numel() != n:
+            raise ValueError(f"base {base}: {self.params.numel()} parameters, the kernels expect {n}")
+        self.grads = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.hyper = torch.tensor([1e-3, 0.0], dtype=torch.float32, device=self.device)
+        self._ws = {}
+        self._mse_ws = _lib.workspace(4096 * 4, self.device)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+
+    def _work(self, batch: int) -> torch.Tensor:
+        if batch not in self._ws:
+            self._ws = {batch: _lib.workspace(_lib.query("pty_wsnet_workspace", self.base, batch), self.device)}
+        return self._ws[batch]
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        """[B, 2, 32, 32] for inputs [B, 64, 64, 3] (NHWC f32, device)."""
+        x = x.float().contiguous()
+        B = x.shape[0]
+        if x.shape[1:] != (64, 64, 3):
+            raise ValueError(f"net input {tuple(x.shape)}, expected [B, 64, 64, 3]")
+        out = torch.empty(B, 2, 32, 32, dtype=torch.float32, device=self.device)
+        ws = self._work(B)
+        _lib.call("pty_wsnet_forward", self.base, self.params.data_ptr(), x.data_ptr(), B, out.data_ptr(),
+                  ws.data_ptr(), ws.numel(), _lib.stream_ptr(self.device))
+        return out
+
+    __call__ = forward
+
+    def loss_and_grad(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+        """MSE of the patches; leaves d loss / d params in ``grads``."""
+        out = self.forward(x)
+        y = y.float().contiguous()
+        dout = torch.empty_like(out)
+        s = _lib.stream_ptr(self.device)
+        _lib.call("pty_wsnet_mse", out.data_ptr(), y.data_ptr(), out.numel(), self.loss.data_ptr(), dout.data_ptr(),
+                  self._mse_ws.data_ptr(), self._mse_ws.numel(), s)
+        ws = self._work(x.shape[0])
+        _lib.call("pty_wsnet_backward", self.base, self.params.data_ptr(), x.shape[0], dout.data_ptr(),
+                  self.grads.data_ptr(), ws.data_ptr(), ws.numel(), s)
+        return self.loss
+
+    def step(self, x, y, lr: float = 1e-3, betas=(0.9, 0.999), eps: float = 1e-8) -> torch.Tensor:
+        """One Adam step on a batch; returns the batch loss (device)."""
+        loss = self.loss_and_grad(x, y)
+        s = _lib.stream_ptr(self.device)
+        _lib.call("pty_fill", self.hyper.data_ptr(), 1, float(lr), s)
+        _lib.call("pty_adam_tick", self.hyper.data_ptr(), s)
+        _lib.call("pty_adam", self.params.data_ptr(), self.grads.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+                  self.params.numel(), self.hyper.data_ptr(), float(betas[0]), float(betas[1]), float(eps), 1, s)
+        return loss
+
+
+def net_inputs_hip(intens: torch.Tensor, tlY, tlX, M: int, canvas: int) -> torch.Tensor:
+    """[frames, 64, 64, 3] (NHWC) by the pty_wsnet_inputs kernel: pooled
+    log(1 + I) and the window offset planes of ``net_inputs``."""
+    dev = intens.device
+    I = intens.float().contiguous()
+    tl = torch.stack([_as_float(tlY, dev), _as_float(tlX, dev)], 1).to(torch.int32).contiguous()
+    P = I.shape[0]
+    x = torch.empty(P, 64, 64, 3, dtype=torch.float32, device=dev)
+    _lib.call("pty_wsnet_inputs", I.data_ptr(), P, M, tl.data_ptr(), canvas, x.data_ptr(), _lib.stream_ptr(dev))
+    return x
+
+
 def build_dataset(seeds, device="cuda", **shape):
-    """Stacked (inputs bf16, targets bf16) of the seeded instances."""
+    """Stacked (inputs [N, 64, 64, 3], targets [N, 2, 32, 32]) f32 of the
+    seeded instances.  The targets are the object patches themselves (values
+    of order one); ``warm_start_object`` puts them on the reconstruction's
+    scale with ``object_scale`` of the probe."""
     xs, ys = [], []
     M, canvas = shape.get("M", 256), shape.get("canvas", 1024)
     for s in seeds:
         inst = simulate(s, device=device, **shape)
-        xs.append(net_inputs(inst.intens, inst.tlY, inst.tlX, M, canvas).to(torch.bfloat16))
-        ys.append((patch_targets(inst.obj, inst.tlY, inst.tlX, M) * object_scale(inst.probe)).to(torch.bfloat16))
+        xs.append(net_inputs_hip(inst.intens, inst.tlY, inst.tlX, M, canvas))
+        ys.append(patch_targets(inst.obj, inst.tlY, inst.tlX, M).float())
     return torch.cat(xs), torch.cat(ys)
 
 
-def train(net: WarmStartNet, x: torch.Tensor, y: torch.Tensor, epochs: int = 30, lr: float = 1e-3,
+def train(net: HipWarmStartNet, x: torch.Tensor, y: torch.Tensor, epochs: int = 30, lr: float = 1e-3,
           batch: int = 256, seed: int = 0, log=None) -> list[float]:
-    """Adam on the MSE of the patch; BF16 autocast, FP32 master weights."""
-    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    """Adam on the MSE of the patches over shuffled minibatches (the last one
+    may be smaller); returns the mean loss of each epoch."""
     gen = torch.Generator(device=x.device)
     gen.manual_seed(seed)
     n = x.shape[0]
@@ -196,12 +305,7 @@ def train(net: WarmStartNet, x: torch.Tensor, y: torch.Tensor, epochs: int = 30,
         tot = torch.zeros((), device=x.device)
         for b in range(0, n, batch):
             i = perm[b:b + batch]
-            with torch.autocast("cuda", dtype=torch.bfloat16):
-                loss = F.mse_loss(net(x[i]).float(), y[i].float())
-            opt.zero_grad(set_to_none=True)
-            loss.backward()
-            opt.step()
-            tot = tot + loss.detach() * i.numel()
+            tot = tot + net.step(x[i], y[i], lr=lr)[0] * i.numel()
         hist.append(float(tot) / n)
         if log:
             log(ep + 1, hist[-1])
@@ -209,18 +313,16 @@ def train(net: WarmStartNet, x: torch.Tensor, y: torch.Tensor, epochs: int = 30,
 
 
 @torch.no_grad()
-def warm_start_object(net: WarmStartNet, intens: torch.Tensor, tlY, tlX, M: int, canvas: int,
-                      batch: int = 512) -> torch.Tensor:
+def warm_start_object(net: HipWarmStartNet, intens: torch.Tensor, tlY, tlX, M: int, canvas: int,
+                      probe: torch.Tensor | None = None, batch: int = 512) -> torch.Tensor:
     """Initial complex canvas from the patterns [frames, M, M] (fftshifted,
     reference orientation): predicted patches upsampled to the window, each
     rotated to agree in phase with the canvas built so far over their overlap
     (positions in scan order), and overlap-averaged; uncovered pixels get the
-    mean of the covered ones."""
-    x = net_inputs(intens, tlY, tlX, M, canvas)
-    preds = []
-    for b in range(0, x.shape[0], batch):
-        with torch.autocast("cuda", dtype=torch.bfloat16):
-            preds.append(net(x[b:b + batch]).float())
+    mean of the covered ones.  With the initial probe, the canvas is put on
+    the scale the reconstruction converges to (``object_scale``)."""
+    x = net_inputs_hip(intens, tlY, tlX, M, canvas)
+    preds = [net(x[b:b + batch]) for b in range(0, x.shape[0], batch)]
     p = F.interpolate(torch.cat(preds), size=M, mode="bilinear", align_corners=False)
     pz = torch.complex(p[:, 0], p[:, 1])
     acc = torch.zeros(canvas, canvas, dtype=torch.complex64, device=x.device)
@@ -237,6 +339,8 @@ def warm_start_object(net: WarmStartNet, intens: torch.Tensor, tlY, tlX, M: int,
     out = torch.empty(canvas, canvas, dtype=torch.complex64, device=x.device)
     out[cov] = acc[cov] / cnt[cov]
     out[~cov] = out[cov].mean() if cov.any() else 1.0
+    if probe is not None:
+        out = out * object_scale(probe)
     return out
 
 

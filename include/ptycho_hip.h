@@ -323,6 +323,34 @@ int pty_adam_tick(float* hyper, hipStream_t stream);
 int pty_batch_next(const int32_t* order, int32_t n_order, int32_t* cursor, int32_t batch, int32_t* idx,
                    float* hyper, const float* loss, float* hist, int64_t n_hist, hipStream_t stream);
 
+/* ---- learned warm start: conv encoder-decoder (no counterpart in the reference) ----
+ * Maps each diffraction pattern (log(1 + I) pooled to 64 x 64 plus the
+ * window's offset, NHWC [B, 64, 64, 3]) to a 32 x 32 complex object patch
+ * (NCHW [B, 2, 32, 32]); the patches seed ComplexINRModel2D's object_initial
+ * (models/complex_inr.py:209-250).  Base width `base` in {16, 32, 48, 64}.
+ * Parameters: one f32 vector of pty_wsnet_param_count(base) floats in the
+ * order of the torch module's parameters, conv weights [Co][kh][kw][Ci],
+ * transposed-conv weights [kh][kw][Co][Ci].  GEMMs in BF16 on MFMA with FP32
+ * accumulation; deterministic.  Batches of 1 .. 1024. */
+size_t pty_wsnet_param_count(int32_t base);
+size_t pty_wsnet_workspace(int32_t base, int32_t batch);
+/* x [P, 64, 64, 3] from fftshifted patterns intens [P, M, M] (M a multiple of
+ * 64) and windows tl [P, 2] on a square canvas. */
+int pty_wsnet_inputs(const float* intens, int32_t P, int32_t M, const int32_t* tl, int32_t canvas, float* x,
+                     hipStream_t stream);
+/* out = net(x); keeps the activations in `work` for pty_wsnet_backward. */
+int pty_wsnet_forward(int32_t base, const float* params, const float* x, int32_t batch, float* out, void* work,
+                      size_t work_bytes, hipStream_t stream);
+/* loss[0] = mean (out - target)^2 over n values; dout (may be NULL) =
+ * d loss / d out.  work: at least 4096 bytes. */
+int pty_wsnet_mse(const float* out, const float* target, int64_t n, float* loss, float* dout, void* work,
+                  size_t work_bytes, hipStream_t stream);
+/* grads (overwritten, same layout as params) = d L / d params given dout =
+ * d L / d out, using the activations of the last pty_wsnet_forward on the
+ * same workspace, batch and parameters. */
+int pty_wsnet_backward(int32_t base, const float* params, int32_t batch, const float* dout, float* grads, void* work,
+                       size_t work_bytes, hipStream_t stream);
+
 /* fill kernel (graph-safe replacement for hipMemsetAsync) */
 int pty_fill(float* p, int64_t n, float value, hipStream_t stream);
 

@@ -34,6 +34,10 @@ Pinning (what checks each function, see tests/test_oracle.py):
                        constant, checkerboard), the orthonormal-Haar energy
                        identity of the four bands, autograd vs central
                        differences.
+  wsnet_forward        the torch module of the same architecture (nn.Conv2d,
+                       nn.ConvTranspose2d) with the same weights, an explicit
+                       loop convolution and transposed convolution on a
+                       tiny input, central differences of a loss.
 """
 from __future__ import annotations
 
@@ -390,3 +394,62 @@ def adam_update(p, g, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     denom = (v.sqrt() / math.sqrt(bc2)).add_(eps)
     p.addcdiv_(m, denom, value=-lr / bc1)
     return p
+
+
+# ----------------------------------------------------------------------------
+# Learned warm start: the conv encoder-decoder of the warm-start row (no
+# counterpart in the reference; it seeds object_initial of
+# models/complex_inr.py:209-250).  Functional restatement from the flat
+# parameter vector the HIP kernels use (torch parameter order; conv weights
+# [Co][kh][kw][Ci], transposed-conv weights [kh][kw][Co][Ci]), with the
+# library routines F.conv2d / F.conv_transpose2d / F.gelu / F.avg_pool2d.
+# ----------------------------------------------------------------------------
+
+def wsnet_layers(base: int):
+    """(kind, ci, co) of the 15 layers: 'c3', 'c1' convolutions, 'up' 2x2/2."""
+    c = base
+    return [("c3", 3, c), ("c3", c, c), ("c3", c, 2 * c), ("c3", 2 * c, 2 * c), ("c3", 2 * c, 4 * c),
+            ("c3", 4 * c, 4 * c), ("c3", 4 * c, 8 * c), ("c3", 8 * c, 8 * c), ("up", 8 * c, 4 * c),
+            ("c3", 8 * c, 4 * c), ("c3", 4 * c, 4 * c), ("up", 4 * c, 2 * c), ("c3", 4 * c, 2 * c),
+            ("c3", 2 * c, 2 * c), ("c1", 2 * c, 2)]
+
+
+def wsnet_unpack(flat, base: int):
+    """[(weight in torch's layout, bias)] per layer, views of ``flat``."""
+    out, off = [], 0
+    for kind, ci, co in wsnet_layers(base):
+        if kind == "up":
+            w = flat[off:off + 4 * co * ci].reshape(2, 2, co, ci).permute(3, 2, 0, 1)   # [Ci, Co, kh, kw]
+            off += 4 * co * ci
+        else:
+            k = 3 if kind == "c3" else 1
+            w = flat[off:off + k * k * ci * co].reshape(co, k, k, ci).permute(0, 3, 1, 2)  # [Co, Ci, kh, kw]
+            off += k * k * ci * co
+        b = flat[off:off + co]
+        off += co
+        out.append((w, b))
+    assert off == flat.numel(), (off, flat.numel())
+    return out
+
+
+def wsnet_forward(flat, x_nhwc, base: int):
+    """Net output [B, 2, 32, 32] for inputs [B, 64, 64, 3] (NHWC)."""
+    L = wsnet_unpack(flat, base)
+    x = x_nhwc.permute(0, 3, 1, 2)
+
+    def conv(i, h):
+        w, b = L[i]
+        return F.gelu(F.conv2d(h, w, b, padding=1))
+
+    def up(i, h):
+        w, b = L[i]
+        return F.conv_transpose2d(h, w, b, stride=2)
+
+    x1 = conv(1, conv(0, x))
+    x2 = conv(3, conv(2, F.avg_pool2d(x1, 2)))
+    x3 = conv(5, conv(4, F.avg_pool2d(x2, 2)))
+    x4 = conv(7, conv(6, F.avg_pool2d(x3, 2)))
+    y3 = conv(10, conv(9, torch.cat([up(8, x4), x3], 1)))
+    y2 = conv(13, conv(12, torch.cat([up(11, y3), x2], 1)))
+    w, b = L[14]
+    return F.conv2d(y2, w, b)

@@ -603,3 +603,90 @@ def test_aligned_nrmse_ambiguities(pkg):
     noisy = g + 0.1 * (rng.standard_normal((32, 32)))
     e = pkg.metrics.aligned_nrmse(noisy, g)
     assert 0.05 < e < 0.1
+
+
+# ------------------------------------------------------------------ warm-start net
+def _loop_conv3(flat_w, bias, x, ci, co):
+    """out[y, x, o] = b[o] + sum W[o][kh][kw][i] in[y + kh - 1][x + kw - 1][i] (NHWC, zero padding)."""
+    H, W = x.shape[:2]
+    w = flat_w.reshape(co, 3, 3, ci)
+    out = np.zeros((H, W, co))
+    xp = np.pad(x, ((1, 1), (1, 1), (0, 0)))
+    for y in range(H):
+        for xx in range(W):
+            patch = xp[y:y + 3, xx:xx + 3, :]
+            out[y, xx] = np.tensordot(w, patch, axes=([1, 2, 3], [0, 1, 2])) + bias
+    return out
+
+
+def _loop_up(flat_w, bias, x, ci, co):
+    """out[2y + a, 2x + c, o] = b[o] + sum_i W[a][c][o][i] in[y, x, i]."""
+    H, W = x.shape[:2]
+    w = flat_w.reshape(2, 2, co, ci)
+    out = np.zeros((2 * H, 2 * W, co))
+    for a in range(2):
+        for c in range(2):
+            out[a::2, c::2] = np.einsum("oi,yxi->yxo", w[a, c], x) + bias
+    return out
+
+
+def test_wsnet_layer_layouts_against_loops():
+    """The oracle's unpacking (flat kernel layout -> torch layouts) against
+    explicit loops written from the layout definition, one conv and one
+    transposed conv."""
+    rng = np.random.default_rng(0)
+    base = 2
+    layers = orc.wsnet_layers(base)
+    n = sum((9 if k == "c3" else 4 if k == "up" else 1) * ci * co + co for k, ci, co in layers)
+    flat = torch.from_numpy(rng.standard_normal(n))
+    L = orc.wsnet_unpack(flat, base)
+    # layer 1: conv3 c -> c
+    w, b = L[1]
+    off = 9 * 3 * base + base
+    x = rng.standard_normal((6, 5, base))
+    got = F.conv2d(torch.from_numpy(x).permute(2, 0, 1)[None], w, b, padding=1)[0].permute(1, 2, 0).numpy()
+    want = _loop_conv3(flat[off:off + 9 * base * base].numpy(), flat[off + 9 * base * base:off + 9 * base * base + base]
+                       .numpy(), x, base, base)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    # layer 8: up 8c -> 4c
+    off = sum((9 if k == "c3" else 4 if k == "up" else 1) * ci * co + co for k, ci, co in layers[:8])
+    ci, co = 8 * base, 4 * base
+    w, b = L[8]
+    x = rng.standard_normal((3, 4, ci))
+    got = F.conv_transpose2d(torch.from_numpy(x).permute(2, 0, 1)[None], w, b, stride=2)[0].permute(1, 2, 0).numpy()
+    want = _loop_up(flat[off:off + 4 * ci * co].numpy(), flat[off + 4 * ci * co:off + 4 * ci * co + co].numpy(), x,
+                    ci, co)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_wsnet_oracle_matches_torch_module(pkg):
+    """The functional oracle = the torch module of the same architecture with
+    the same weights (FP64), through the product's parameter packing."""
+    ws = pkg.warmstart
+    torch.manual_seed(1)
+    mod = ws.WarmStartNet(base=4).double()
+    flat = ws.pack_params(mod).double()
+    x = torch.randn(2, 64, 64, 3, dtype=torch.float64)
+    want = mod(x.permute(0, 3, 1, 2))
+    got = orc.wsnet_forward(flat, x, 4)
+    assert got.shape == (2, 2, 32, 32)
+    assert torch.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_wsnet_oracle_gradient_central_differences():
+    rng = np.random.default_rng(2)
+    base = 2
+    layers = orc.wsnet_layers(base)
+    n = sum((9 if k == "c3" else 4 if k == "up" else 1) * ci * co + co for k, ci, co in layers)
+    flat = torch.from_numpy(rng.standard_normal(n) * 0.3).requires_grad_(True)
+    x = torch.from_numpy(rng.standard_normal((1, 16, 16, 3)))
+    r = torch.from_numpy(rng.standard_normal((1, 2, 8, 8)))
+    (orc.wsnet_forward(flat, x, base) * r).sum().backward()
+    h = 1e-5
+    for i in rng.choice(n, 12, replace=False):
+        fp = flat.detach().clone()
+        fm = flat.detach().clone()
+        fp[i] += h
+        fm[i] -= h
+        d = ((orc.wsnet_forward(fp, x, base) * r).sum() - (orc.wsnet_forward(fm, x, base) * r).sum()) / (2 * h)
+        assert abs(float(d) - float(flat.grad[i])) < 1e-6 * max(1.0, abs(float(d))), i

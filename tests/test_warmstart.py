@@ -1,6 +1,16 @@
-"""Learned warm start: host-side shapes on the CPU; on the GPU, a small net
-trained on simulated instances must start a reconstruction closer to the
-object than the plain initProbe start and stay ahead after a few epochs."""
+"""Learned warm start.
+
+CPU: shapes of the architecture definition and of the patch targets, and the
+parameter packing.  GPU: the HIP net (pty_wsnet_*) against the FP64 oracle
+(forward and every parameter gradient), the input kernel against its torch
+restatement, bitwise determinism of training steps, and the gain: a net
+trained on simulated instances starts a canvas-1024 reconstruction closer to
+the object than the plain initProbe start and ends it closer.
+
+Bars (BF16 GEMM operands, FP32 accumulation, against FP64): net output 1.5e-2
+relative in norm, each layer's weight and bias gradients 5e-2 (products of
+two bf16-rounded operands summed over up to 10^6 pixels; the rounding noise
+does not cancel where the exact gradient does)."""
 import numpy as np
 import pytest
 import torch
@@ -27,28 +37,116 @@ def test_net_and_patch_shapes_cpu(pkg):
     assert ws.object_scale(probe) == pytest.approx(2.0)
 
 
+def test_pack_params_layout_cpu(pkg):
+    ws = pkg.warmstart
+    torch.manual_seed(0)
+    net = ws.WarmStartNet(base=16)
+    flat = ws.pack_params(net)
+    assert flat.numel() == sum(p.numel() for p in net.parameters())
+    conv = net.e1[0]
+    w = flat[:conv.weight.numel()].reshape(16, 3, 3, 3)
+    assert torch.equal(w.permute(0, 3, 1, 2), conv.weight.detach())
+    assert ws._module_order_matches(net)
+
+
+@pytest.mark.gpu
+def test_wsnet_inputs_match_net_inputs(pkg, dev):
+    ws = pkg.warmstart
+    g = torch.Generator(device=dev).manual_seed(3)
+    for M, canvas in ((64, 256), (256, 1024)):
+        I = torch.poisson(torch.rand(7, M, M, generator=g, device=dev) * 50.0, generator=g)
+        ty = torch.randint(0, canvas - M + 1, (7,), generator=g, device=dev)
+        tx = torch.randint(0, canvas - M + 1, (7,), generator=g, device=dev)
+        got = ws.net_inputs_hip(I, ty, tx, M, canvas)
+        want = ws.net_inputs(I, ty, tx, M, canvas).permute(0, 2, 3, 1)
+        assert torch.allclose(got, want, rtol=2e-6, atol=2e-6), (M, (got - want).abs().max().item())
+
+
+def _oracle_grads(flat, x, dout, base):
+    from oracle import ptycho_oracle as orc
+    f = flat.detach().cpu().double().requires_grad_(True)
+    out = orc.wsnet_forward(f, x.detach().cpu().double(), base)
+    (out * dout.detach().cpu().double()).sum().backward()
+    return out.detach(), f.grad
+
+
+@pytest.mark.gpu
+def test_hip_net_forward_backward_against_oracle(pkg, dev):
+    from oracle import ptycho_oracle as orc
+    from tests.helpers import rel_err
+    ws = pkg.warmstart
+    net = ws.HipWarmStartNet(base=32, device=dev, seed=1)
+    g = torch.Generator(device=dev).manual_seed(4)
+    B = 6
+    x = torch.rand(B, 64, 64, 3, generator=g, device=dev)
+    x[..., 0] *= 8.0                       # log(1 + I) of order a few
+    x[..., 1:] -= 0.5                      # offsets in [-0.5, 0.5]
+    out = net(x)
+    dout = torch.randn(out.shape, generator=g, device=dev)
+    ref, gref = _oracle_grads(net.params, x, dout, 32)
+    e_out = rel_err(out, ref)
+    y = torch.zeros_like(out)
+    # the backward of sum(out * dout): MSE gradient of y = out - dout / 2 * n ...
+    # is exercised through pty_wsnet_backward with dout itself
+    ws_buf = net._work(B)
+    pkg._lib.call("pty_wsnet_backward", 32, net.params.data_ptr(), B, dout.contiguous().data_ptr(),
+                  net.grads.data_ptr(), ws_buf.data_ptr(), ws_buf.numel(), pkg._lib.stream_ptr(dev))
+    errs = []
+    off = 0
+    for i, (kind, ci, co) in enumerate(orc.wsnet_layers(32)):
+        nw = (9 if kind == "c3" else 4 if kind == "up" else 1) * ci * co
+        errs.append((i, rel_err(net.grads[off:off + nw], gref[off:off + nw]),
+                     rel_err(net.grads[off + nw:off + nw + co], gref[off + nw:off + nw + co])))
+        off += nw + co
+    print(f"wsnet: output {e_out:.2e}; grads (layer, W, b):", [(i, f"{a:.1e}", f"{b:.1e}") for i, a, b in errs])
+    assert e_out < 1.5e-2, e_out
+    for i, a, b in errs:
+        assert a < 5e-2 and b < 5e-2, (i, a, b)
+    # the MSE kernel: loss and its gradient
+    net.loss_and_grad(x, y)
+    mse = float(net.loss)
+    assert abs(mse - float((out.double() ** 2).mean())) < 1e-5 * mse
+
+
+@pytest.mark.gpu
+def test_hip_net_training_is_deterministic_and_learns(pkg, dev):
+    ws = pkg.warmstart
+    g = torch.Generator(device=dev).manual_seed(5)
+    x = torch.rand(96, 64, 64, 3, generator=g, device=dev)
+    y = torch.randn(96, 2, 32, 32, generator=g, device=dev) * 0.1 + x[:, None, ::2, ::2, 0] * 0.2
+    nets = [ws.HipWarmStartNet(base=16, device=dev, seed=2) for _ in range(2)]
+    hist = [ws.train(n, x, y, epochs=6, batch=32, seed=7) for n in nets]
+    assert hist[0] == hist[1]
+    assert torch.equal(nets[0].params, nets[1].params)
+    assert hist[0][-1] < 0.8 * hist[0][0], hist[0]
+
+
 @pytest.mark.gpu
 def test_warm_start_gains_over_plain_start(pkg, dev):
+    """Canvas-1024 shapes (probe 256, 20 x 20 scan, 1e5 photons): a net
+    trained on 40 simulated instances (16,000 patterns, 6 epochs) seeds the
+    object; against the plain initProbe start on a held-out seed it starts no
+    farther from the object (the plain start is the INR at initialisation,
+    close to a constant canvas, NRMSE ~0.33) and ends 40 epochs closer.""" 
     ws = pkg.warmstart
-    shape = dict(canvas=256, M=64, n_side=10, step=16, jitter=2, photons=1e4)
+    shape = dict(canvas=1024, M=256, n_side=20, step=40, jitter=4, photons=1e5)
     x, y = ws.build_dataset(range(100, 140), device=dev, **shape)
-    torch.manual_seed(0)
-    net = ws.WarmStartNet(base=16).to(dev)
-    hist = ws.train(net, x, y, epochs=6, batch=128)
+    net = ws.HipWarmStartNet(base=32, device=dev, seed=0)
+    hist = ws.train(net, x, y, epochs=6, batch=256)
     assert hist[-1] < hist[0]
-    net.eval()
-    inst = ws.simulate(7, jitter_probe=False, device=dev, **shape)
+    del x, y
+    inst = ws.simulate(900007, jitter_probe=False, device=dev, **shape)
     truth = inst.obj.cpu().numpy()
     ty, tx = inst.tlY.cpu().numpy(), inst.tlX.cpu().numpy()
-    mask = pkg.metrics.evaluation_mask(ty, tx, 64, 64, truth.shape)
+    mask = pkg.metrics.evaluation_mask(ty, tx, 256, 256, truth.shape)
     stack = inst.intens.permute(1, 2, 0).contiguous()
 
     def err(o):
         return pkg.metrics.aligned_nrmse(o.detach().cpu().numpy(), truth, mask, max_shift=2)
 
-    def run(init, epochs=4):
+    def run(init, epochs=40):
         torch.manual_seed(0)
-        model = pkg.ComplexINRModel2D(256, 256, 2, probe_width=64, probe_height=64, log2_hashmap_size=14,
+        model = pkg.ComplexINRModel2D(1024, 1024, 2, probe_width=256, probe_height=256, log2_hashmap_size=18,
                                       hidden_omega_0=1.0, trainable_omega0=True, use_residual=True,
                                       object_initial=init, probe_initial=inst.probe.to(torch.complex64)).to(dev)
         eng = pkg.FusedReconstruction(model, stack, ty, tx, batch_size=10)
@@ -59,11 +157,9 @@ def test_warm_start_gains_over_plain_start(pkg, dev):
             eng.run_epoch(gen)
         return e0, err(eng.current_fields()[0])
 
-    init = ws.warm_start_object(net, inst.intens, ty, tx, 64, 256)
+    init = ws.warm_start_object(net, inst.intens, ty, tx, 256, 1024, probe=inst.probe)
     p0, p1 = run(None)
     w0, w1 = run(init)
     print(f"plain start {p0:.3f} -> {p1:.3f}; warm start {w0:.3f} -> {w1:.3f}")
-    # the gain: after the same number of epochs the warm-started run is
-    # closer to the object, and the warm start is not worse at epoch 0
     assert w0 < p0 + 0.02
-    assert w1 < p1 - 0.02
+    assert w1 < p1 - 0.05
