@@ -57,7 +57,7 @@ WORKLOADS = {
 MLP_FLOP_BWD_PER_POINT = 37376      # dX + dW of the complex MLP per point (recompute not counted)
 MLP_FLOP_FWD_PER_POINT = 18688
 MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}     # MI355X dense
-MLP_BWD_KERNEL = {"fp32": "k_inr_bwd16<2>", "bf16": "k_inr_bwd_lp<pty::FmtBf16, 2>",
+MLP_BWD_KERNEL = {"fp32": "k_inr_bwd16<2>", "bf16": "k_inr_bwd_b16<2>",
                   "fp8": "k_inr_bwd_lp<pty::FmtFp8, 2>"}
 PROFILES = ROOT / "profiles"
 

@@ -202,8 +202,10 @@ extern "C" int pty_inr_fwd(const PtyGrid* g, const PtyMlp* mlp, const float* tab
               g->n_levels * g->n_features);
   PTY_REQUIRE(table && out, PTY_EINVAL, "inr_fwd: null pointer");
   if (pts->n == 0) return PTY_OK;
-  if (mlp->precision != PTY_PREC_FP32)
+  if (mlp->precision == PTY_PREC_FP8)
     launch_inr_fwd_lowp(*g, *mlp, (const float2*)table, *pts, (float2*)out, fwd_blocks(pts->n), stream);
+  else if (mlp->precision == PTY_PREC_BF16)
+    launch_inr_fwd_b16(*g, *mlp, (const float2*)table, *pts, (float2*)out, stream);
   else
     launch_inr_fwd_f32(*g, *mlp, (const float2*)table, *pts, (float2*)out, stream);
   return check_launch("inr_fwd");
@@ -265,9 +267,12 @@ extern "C" int pty_inr_bwd_mlp(const PtyGrid* g, const PtyMlp* mlp, const float*
   PTY_REQUIRE(table && dout, PTY_EINVAL, "inr_bwd: null pointer");
   if (pts->n == 0) return PTY_OK;
   const BwdWork b = bwd_work(g, mlp, pts, work);
-  if (mlp->precision != PTY_PREC_FP32)
+  if (mlp->precision == PTY_PREC_FP8)
     launch_inr_bwd_lowp(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
                         stream);
+  else if (mlp->precision == PTY_PREC_BF16)
+    launch_inr_bwd_b16(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
+                       stream);
   else
     launch_inr_bwd_f32(*g, *mlp, (const float2*)table, b.gfeat, *pts, (const float2*)dout, b.partial, b.blocks,
                        stream);

@@ -1,0 +1,114 @@
+// 16-point wave tiles shared by the FP32 and BF16 fused INR kernels: a
+// wavefront owns 16 points, lane l = (j, g) = (l & 15, l >> 4) holds point j;
+// the 16x16 MFMA C layout gives lane (j, g) rows 4 g + v (v < 4) of column j.
+//   unit16(s, g): the 8 MLP units lane group g holds (two 16-row M blocks);
+//   feat16(c, g): the 8 hash features lane group g interpolates (levels
+//                 4 m + g, m < 4), in the order they enter layer 0;
+// encode16 forms the lane's features, and the feature cache keeps them
+// between a forward and the backward of the same step (8 floats per lane).
+#pragma once
+#include "inr_common.h"
+
+namespace pty {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int unit16(int s, int g) { return 16 * (s >> 2) + 4 * g + (s & 3); }
+__device__ __forceinline__ int feat16(int c, int g) { return 2 * (4 * (c >> 1) + g) + (c & 1); }
+
+// Level constants for lane group g at level 4 m + g: the four candidates are
+// uniform kernel-argument values, the lane selects one.
+__device__ __forceinline__ LevelC level_c4(const PtyGrid& gr, int m, int g) {
+  const int l0 = 4 * m;
+  float sc[4];
+  uint32_t rs[4], sz[4], of[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    sc[q] = __uint_as_float(uniform_u32(__float_as_uint(gr.scale[l0 + q])));
+    rs[q] = uniform_u32(gr.res[l0 + q]);
+    sz[q] = uniform_u32(gr.size[l0 + q]);
+    of[q] = uniform_u32(gr.offset[l0 + q]);
+  }
+  const uint32_t dm = uniform_u32(gr.dense_mask);
+  LevelC c;
+  c.scale = g == 0 ? sc[0] : g == 1 ? sc[1] : g == 2 ? sc[2] : sc[3];
+  c.res = g == 0 ? rs[0] : g == 1 ? rs[1] : g == 2 ? rs[2] : rs[3];
+  c.size = g == 0 ? sz[0] : g == 1 ? sz[1] : g == 2 ? sz[2] : sz[3];
+  c.offset = g == 0 ? of[0] : g == 1 ? of[1] : g == 2 ? of[2] : of[3];
+  c.dense = (dm >> (l0 + g)) & 1u;
+  return c;
+}
+
+// The lane's 8 features feat[c] = feature feat16(c, g): the 16 corner indices
+// are formed first and the 16 gathers issued back to back.
+__device__ __forceinline__ void encode16(const PtyGrid& gr, float x0, float x1, int g,
+                                         const float2* __restrict__ table, float feat[8]) {
+  uint32_t idx[4][4];
+  float fx[4], fy[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int l = 4 * m + g;
+    if (l < gr.n_levels) {
+      const LevelC c = level_c4(gr, m, g);
+      const LevelPos p = level_pos_c(c, x0, x1);
+      corner_indices_c(c, p, idx[m]);
+      fx[m] = p.fx;
+      fy[m] = p.fy;
+    } else {
+      idx[m][0] = idx[m][1] = idx[m][2] = idx[m][3] = 0u;   // in bounds; the result is discarded
+      fx[m] = fy[m] = 0.f;
+    }
+  }
+  float2 t[4][4];
+  if (gr.table_precision == PTY_TABLE_F16) {
+    const __half2* t16 = reinterpret_cast<const __half2*>(table);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[m][c] = __half22float2(t16[idx[m][c]]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[m][c] = table[idx[m][c]];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    float2 acc = make_float2(0.f, 0.f);
+    if (4 * m + g < gr.n_levels) {
+      LevelPos p;
+      p.fx = fx[m];
+      p.fy = fy[m];
+      float w[4];
+      corner_weights(p, w);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        acc.x += w[c] * t[m][c].x;
+        acc.y += w[c] * t[m][c].y;
+      }
+    }
+    feat[2 * m] = acc.x;
+    feat[2 * m + 1] = acc.y;
+  }
+}
+
+// Feature cache: the 8 features of lane `lane` of tile `tile` at
+// features[(tile * 64 + lane) * 8] (32 coalesced bytes per lane).
+__device__ __forceinline__ void store_features16(float* f, int64_t tile, int lane, const float feat[8]) {
+  float4* p = reinterpret_cast<float4*>(f + (tile * 64 + lane) * 8);
+  p[0] = make_float4(feat[0], feat[1], feat[2], feat[3]);
+  p[1] = make_float4(feat[4], feat[5], feat[6], feat[7]);
+}
+__device__ __forceinline__ void load_features16(const float* f, int64_t tile, int lane, float feat[8]) {
+  const float4* p = reinterpret_cast<const float4*>(f + (tile * 64 + lane) * 8);
+  const float4 a = p[0], b = p[1];
+  feat[0] = a.x; feat[1] = a.y; feat[2] = a.z; feat[3] = a.w;
+  feat[4] = b.x; feat[5] = b.y; feat[6] = b.z; feat[7] = b.w;
+}
+
+
+}  // namespace pty

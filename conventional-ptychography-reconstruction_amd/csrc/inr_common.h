@@ -316,7 +316,13 @@ void launch_inr_fwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, 
 void launch_inr_bwd_f32(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
                         const float2* dout, float* partial, int blocks, hipStream_t stream);
 
-// BF16 / FP8 launchers (inr_lowp.hip)
+// BF16 launchers (inr_bf16.hip, 16-point wave tiles)
+void launch_inr_fwd_b16(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
+                        hipStream_t stream);
+void launch_inr_bwd_b16(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,
+                        const float2* dout, float* partial, int blocks, hipStream_t stream);
+
+// FP8 (and the 32-point BF16) launchers (inr_lowp.hip)
 void launch_inr_fwd_lowp(const PtyGrid& g, const PtyMlp& m, const float2* table, const PtyPoints& pts, float2* out,
                          int blocks, hipStream_t stream);
 void launch_inr_bwd_lowp(const PtyGrid& g, const PtyMlp& m, const float2* table, float2* gfeat, const PtyPoints& pts,

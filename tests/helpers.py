@@ -103,3 +103,16 @@ def boundary_coords(table, per_level=40, seed=0):
     x = np.concatenate(xs)
     x = x[(x > 0) & (x < 1)]
     return x
+
+
+def omega_error(g_ours, g_ref, gW_ref, W, gb_ref, b, omega) -> float:
+    """Error of a frequency gradient dL/domega.  By the homogeneity of a
+    Gabor layer in its weights, omega dL/domega = Re<G_W, W> + Re<G_b, b>: a
+    sum of terms that cancel, so the error is measured against the
+    Cauchy-Schwarz scale (|G_W| |W| + |G_b| |b|) / |omega| of that identity
+    rather than against the (possibly tiny) result."""
+    def nrm(t):
+        return torch.as_tensor(t).detach().cpu().to(torch.complex128).abs().pow(2).sum().sqrt().item()
+    scale = (nrm(gW_ref) * nrm(W) + nrm(gb_ref) * nrm(b)) / abs(float(torch.as_tensor(omega).detach().cpu()))
+    d = abs(float(torch.as_tensor(g_ours).detach().cpu().double()) - float(torch.as_tensor(g_ref).detach().cpu().double()))
+    return d / scale if scale > 0 else d

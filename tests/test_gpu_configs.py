@@ -24,7 +24,7 @@ import pytest
 import torch
 
 from oracle import ptycho_oracle as orc
-from tests.helpers import oracle_params, oracle_table, rel_err, step_reference
+from tests.helpers import omega_error, oracle_params, oracle_table, rel_err, step_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -82,9 +82,16 @@ def _step_errors(pkg, dev, inst, model, batch, loss_type, table_fp16, with32=Tru
     errs["prb_table"] = rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"])
     errs32["prb_table"] = rel_err(g32["prb_table"], g["prb_table"])
     for which, mlp in (("obj", model.object_complex_mlp), ("prb", model.probe_complex_mlp)):
-        for k, (a, b, c) in enumerate(zip(mlp.tensors(), g[which + "_mlp"], g32[which + "_mlp"])):
-            errs[f"{which}_mlp{k}"] = rel_err(eng.flat.g(a), b)
-            errs32[f"{which}_mlp{k}"] = rel_err(c, b)
+        ts, gr, gr32 = mlp.tensors(), g[which + "_mlp"], g32[which + "_mlp"]
+        for k, (a, b, c) in enumerate(zip(ts, gr, gr32)):
+            if k < len(ts) - 2 and k % 3 == 2:
+                # frequency of layer k // 3: error on its homogeneity scale
+                errs[f"{which}_mlp{k}"] = omega_error(eng.flat.g(a), b, gr[k - 2], ts[k - 2], gr[k - 1], ts[k - 1],
+                                                      a)
+                errs32[f"{which}_mlp{k}"] = omega_error(c, b, gr[k - 2], ts[k - 2], gr[k - 1], ts[k - 1], a)
+            else:
+                errs[f"{which}_mlp{k}"] = rel_err(eng.flat.g(a), b)
+                errs32[f"{which}_mlp{k}"] = rel_err(c, b)
     errs["k_probe"] = rel_err(eng.flat.g(model.k_probe), g["k_probe"])
     errs32["k_probe"] = rel_err(g32["k_probe"], g["k_probe"])
     return abs(loss - ref_loss) / abs(ref_loss), errs, errs32
@@ -118,13 +125,12 @@ def test_canvas1024_poisson_step_reduced_precision(pkg, dev, inst1024, precision
     gradients 5e-2; FP8: 1.2e-1 / 3e-1): the intensity of a pattern is
     quadratic in the field, so its error doubles, and the Poisson loss
     I - I_meas log I is dominated by sum I, so the loss is held to twice the
-    field bar; the gradients keep the field-level gradient bars, except the
-    scalar gradients of the frequencies omega_k and of the residual scale
-    k_probe in FP8: each is one cancelling sum over all points
-    (d omega = Im sum conj(A) o P + ..., d k = Re sum dprobe conj(mlp)), whose
-    relative error is the pointwise FP8 error amplified by the cancellation
-    (measured up to 0.56), so they are held to 1.0: the right sign and
-    magnitude within a factor of two."""
+    field bar; the gradients keep the field-level gradient bars (the
+    frequencies' on their homogeneity scale, tests.helpers.omega_error),
+    except the residual scale's in FP8: d k_probe = Re sum dprobe conj(mlp) is
+    one cancelling sum over the probe, whose relative error is the pointwise
+    FP8 error amplified by the cancellation (measured 0.56), so it is held to
+    1.0: the right sign and magnitude within a factor of two."""
     inst = inst1024
     model = _model(pkg, dev, 1024, 256, 18, torch.from_numpy(inst.probe.astype(np.complex64)), seed=4,
                    mlp_precision=precision)
@@ -133,11 +139,8 @@ def test_canvas1024_poisson_step_reduced_precision(pkg, dev, inst1024, precision
     print(f"canvas1024 poisson {precision}: loss {el:.2e}", {k: f"{v:.1e}" for k, v in errs.items()})
     loss_bar, grad_bar = (4e-2, 5e-2) if precision == "bf16" else (2.4e-1, 3e-1)
     assert el < loss_bar, el
-    ts = model.object_complex_mlp.tensors()
-    scalar = {k for k in errs if k.startswith(("obj_mlp", "prb_mlp")) and ts[int(k[7:])].numel() == 1}
-    scalar.add("k_probe")
     for k, e in errs.items():
-        bar = 1.0 if (precision == "fp8" and k in scalar) else grad_bar
+        bar = 1.0 if (precision == "fp8" and k == "k_probe") else grad_bar
         assert e < bar, (k, e, bar)
 
 
@@ -180,7 +183,10 @@ def test_canvas3648_inr_field_on_grid_sample(pkg, dev, variant):
     for (w, b, om) in mlp["layers"]:
         theirs += [w, b, om]
     theirs += [mlp["out_weight"], mlp["out_bias"]]
-    errs = [rel_err(a.grad, b.grad) for a, b in zip(model.object_complex_mlp.tensors(), theirs)]
+    ours = model.object_complex_mlp.tensors()
+    errs = [omega_error(a.grad, b.grad, theirs[k - 2].grad, theirs[k - 2], theirs[k - 1].grad, theirs[k - 1], b)
+            if (k < len(ours) - 2 and k % 3 == 2) else rel_err(a.grad, b.grad)
+            for k, (a, b) in enumerate(zip(ours, theirs))]
     print(f"canvas3648 INR sample {variant}: field {err_f:.2e} table {e_t:.2e} mlp {[f'{e:.1e}' for e in errs]}")
     # FP32: the small-grid bars; BF16 + FP16 tables: the BF16 bars
     fbar, gbar = (2e-6, 1e-4) if variant == "fp32" else (2e-2, 5e-2)
@@ -265,8 +271,9 @@ def test_canvas3648_bf16_fp16_step_near_fp32(pkg, dev, inst3648):
     FP32 step at the same parameters and batch, at the tables' initial scale
     (U(-1e-4, 1e-4): the state at the start of every reconstruction).
 
-    Bars: loss 2e-2; gradients 5e-2 relative in norm (the BF16 gradient bar),
-    except the object and probe tables and layer-0 weights, 3e-1.  Those four
+    Bars: loss 2e-2; gradients 5e-2 relative in norm (the BF16 gradient bar;
+    frequencies on their homogeneity scale), except the object and probe
+    tables and layer-0 weights, 3e-1.  Those four
     are sums over all grid points of products with the per-point gradient
     Im(g_z conj z0), which oscillates at the pixel scale (it comes from the
     far-field adjoint), so its smooth part (what a coarse-level table entry
@@ -283,8 +290,15 @@ def test_canvas3648_bf16_fp16_step_near_fp32(pkg, dev, inst3648):
     l2 = e2.step(b, update=False).sum().item()
     el = abs(l2 - l1) / abs(l1)
     errs = {}
-    for (n1, t1), (n2, t2) in zip(m1.named_parameters(), m2.named_parameters()):
-        if t1.requires_grad:
+    named = [(n1, t1, t2) for (n1, t1), (_, t2) in zip(m1.named_parameters(), m2.named_parameters()) if t1.requires_grad]
+    lookup = {n: (t1, t2) for n, t1, t2 in named}
+    for n1, t1, t2 in named:
+        if n1.endswith("omega_0"):
+            base = n1[:-len("omega_0")]
+            W1, _ = lookup[base + "linear.weight"]
+            b1, _ = lookup[base + "linear.bias"]
+            errs[n1] = omega_error(e2.flat.g(t2), e1.flat.g(t1), e1.flat.g(W1), W1, e1.flat.g(b1), b1, t1)
+        else:
             errs[n1] = rel_err(e2.flat.g(t2), e1.flat.g(t1))
     print(f"canvas3648 bf16+fp16 vs fp32: loss {el:.2e}", {k: f"{v:.1e}" for k, v in errs.items()})
     assert el < 2e-2, el
