@@ -207,6 +207,15 @@ def cpu_baseline(inst, seconds=15.0):
 # has no ROCm build) and the reference's own per-position loop and host syncs.
 # Used only by --mode nrmse, to time "wall-s to the plain run's final NRMSE".
 # ----------------------------------------------------------------------------
+def _pos_fma(x, s):
+    """f32 x * s + 0.5 with one rounding, like the kernels' fmaf (tcnn's
+    pos_fract): the product of two f32 values is exact in f64 and, for
+    x * s >= 2^-5, so is the sum; below that the f64 sum rounds at 2^-54,
+    which changes the f32 result only where it lands exactly on an f32
+    rounding midpoint."""
+    return (x.double() * s.double() + 0.5).float()
+
+
 def _plain_hashgrid(coords, params, spec):
     import torch
     x = coords.float()
@@ -214,8 +223,7 @@ def _plain_hashgrid(coords, params, spec):
     tab = params.view(-1, 2)
     for l in range(spec.n_levels):
         s = torch.tensor(spec.scale[l], dtype=torch.float32, device=x.device)
-        pos = x * s
-        pos = pos + 0.5
+        pos = _pos_fma(x, s)
         g = torch.floor(pos)
         f = pos - g
         gi = g.to(torch.int64)
@@ -357,15 +365,14 @@ def _fp64_leaves(P):
 
 
 def _fp64_hashgrid(coords, table, spec):
-    """Hash-grid features in FP64; positions as two rounded f32 operations."""
+    """Hash-grid features in FP64; f32 positions as in the kernels."""
     import torch
     x = coords.float()
     tab = table.view(-1, 2)
     outs = []
     for l in range(spec.n_levels):
         sc = torch.tensor(spec.scale[l], dtype=torch.float32, device=x.device)
-        pos = x * sc
-        pos = pos + 0.5
+        pos = _pos_fma(x, sc)
         g = torch.floor(pos)
         f = (pos - g).double()
         gi = g.to(torch.int64)
@@ -427,8 +434,39 @@ def _fp64_step_loss(model, P, inp_bmn, tlY, tlX, idx, loss_type):
     return per.sum(), o, p
 
 
+_LOSS_TAGS = {"smooth_L1_loss": "smoothL1", "FD_loss": "FD", "Poisson_likelihood_loss": "Poisson"}
+
+
+def _model_leaves(model):
+    """The model's parameters in the order of _fp64_leaves."""
+    out = [model.xy_hash_encoder.params, model.probe_hash_encoder.params, model.k_probe]
+    for mlp in (model.object_complex_mlp, model.probe_complex_mlp):
+        layers = list(mlp.net)
+        for l in layers[:-1]:
+            out += [l.linear.weight, l.linear.bias, l.omega_0]
+        out += [layers[-1].weight, layers[-1].bias]
+    return out
+
+
+def _leaf_groups(model):
+    """Index ranges of _model_leaves grouped as object table, probe table,
+    k_probe, object MLP, probe MLP."""
+    n_o = 3 * (len(model.object_complex_mlp.net) - 1) + 2
+    n_p = 3 * (len(model.probe_complex_mlp.net) - 1) + 2
+    return {"object table": [0], "probe table": [1], "k_probe": [2],
+            "object MLP": list(range(3, 3 + n_o)), "probe MLP": list(range(3 + n_o, 3 + n_o + n_p))}
+
+
 def sweep_mode(args):
-    """Object / probe NRMSE and loss of every precision variant against an FP64 run."""
+    """Every precision variant (MLP fp32 / bf16 / fp8 x table fp32 / fp16)
+    against an FP64 run of the same math on the same batches, two ways.
+    Free-running: each variant trains on its own, and its final loss and
+    fields are compared with the FP64 run's (the trajectories diverge
+    chaotically, so this mixes precision error with chaos).  Along the FP64
+    trajectory: every --sweep-every steps the FP64 run's parameters, batch,
+    loss and gradients are recorded; each variant then evaluates its loss
+    and gradients at exactly those parameters (no update), which isolates the
+    error of one step's arithmetic."""
     import numpy as np
     import torch
     import __graft_entry__
@@ -436,6 +474,7 @@ def sweep_mode(args):
     dev = torch.device("cuda:0")
     shapes, log2T, batch, _, wl_name = WORKLOADS[args.workload]
     loss_type = args.loss or "Poisson_likelihood_loss"
+    wl_name = wl_name.rsplit("_", 1)[0] + "_" + _LOSS_TAGS[loss_type]
     inst = pkg.synthetic.make_instance(seed=0, **shapes)
     M = inst.probe.shape[0]
     frames = inst.stack.shape[2]
@@ -455,22 +494,31 @@ def sweep_mode(args):
         return float(pkg.metrics.aligned_nrmse(np.asarray(a), np.asarray(b), mask if a.shape == mask.shape else None,
                                                max_shift=shift))
 
-    # FP64 reference (Adam in FP64, same batches)
+    # FP64 reference (Adam in FP64, same batches), with trajectory snapshots
     print(f"sweep: FP64 reference, {args.epochs} epochs of {len(batches[0])} steps", file=sys.stderr, flush=True)
     model64 = make()
+    groups = _leaf_groups(model64)
     P = _fp64_params(model64)
-    opt = torch.optim.Adam(_fp64_leaves(P), lr=1e-3)
+    leaves = _fp64_leaves(P)
+    opt = torch.optim.Adam(leaves, lr=1e-3)
     inp = torch.from_numpy(inst.stack).to(dev).permute(2, 0, 1).contiguous().double()
     t0 = time.perf_counter()
     ref_losses = []
+    snaps = []
+    k = 0
     for ep, eb in enumerate(batches):
         tot = 0.0
         for idx in eb:
             opt.zero_grad()
             loss, _, _ = _fp64_step_loss(model64, P, inp, inst.tlY, inst.tlX, list(idx), loss_type)
             loss.backward()
+            if k % args.sweep_every == 0:
+                snaps.append({"step": k, "idx": [int(i) for i in idx], "loss": float(loss),
+                              "params": [t.detach().clone() for t in leaves],
+                              "grads": [t.grad.detach().clone() for t in leaves]})
             opt.step()
             tot += float(loss)
+            k += 1
         ref_losses.append(tot / len(eb))
         if (ep + 1) % 10 == 0:
             print(f"sweep: FP64 epoch {ep + 1} loss {ref_losses[-1]:.6g} ({time.perf_counter() - t0:.0f} s)",
@@ -480,6 +528,10 @@ def sweep_mode(args):
     o64, p64 = o64.cpu().numpy(), p64.cpu().numpy()
     rows = [{"variant": "fp64 reference", "final_epoch_loss": ref_losses[-1],
              "object_nrmse_vs_truth": nrmse(o64, inst.obj, 2), "wall_s": time.perf_counter() - t0}]
+
+    def flat_real(ts):
+        return torch.cat([(torch.view_as_real(t) if t.is_complex() else t).reshape(-1).double() for t in ts])
+
     for mlp_p, tab_p in (("fp32", "fp32"), ("bf16", "fp32"), ("fp8", "fp32"),
                          ("fp32", "fp16"), ("bf16", "fp16"), ("fp8", "fp16")):
         model = make(mlp_p, tab_p)
@@ -497,15 +549,37 @@ def sweep_mode(args):
         o, p = eng.current_fields()
         o, p = o.detach().cpu().numpy(), p.detach().cpu().numpy()
         fl = float(losses[-1]) / len(batches[-1])
-        rows.append({"variant": f"mlp {mlp_p}, table {tab_p}", "final_epoch_loss": fl,
-                     "loss_rel_vs_fp64": abs(fl - ref_losses[-1]) / abs(ref_losses[-1]),
-                     "object_nrmse_vs_fp64": nrmse(o, o64, 0), "probe_nrmse_vs_fp64": nrmse(p, p64, 0),
-                     "object_nrmse_vs_truth": nrmse(o, inst.obj, 2), "wall_s": wall})
-        print(f"sweep: {rows[-1]}", file=sys.stderr, flush=True)
+        row = {"variant": f"mlp {mlp_p}, table {tab_p}", "final_epoch_loss": fl,
+               "loss_rel_vs_fp64": abs(fl - ref_losses[-1]) / abs(ref_losses[-1]),
+               "object_nrmse_vs_fp64": nrmse(o, o64, 0), "probe_nrmse_vs_fp64": nrmse(p, p64, 0),
+               "object_nrmse_vs_truth": nrmse(o, inst.obj, 2), "wall_s": wall}
+        # along the FP64 trajectory: loss and gradients at the recorded parameters
+        mleaves = _model_leaves(model)
+        traj = {"loss_rel": [], **{f"grad_rel {g}": [] for g in groups}}
+        for sn in snaps:
+            with torch.no_grad():
+                for t, v in zip(mleaves, sn["params"]):
+                    t.copy_(v.to(t.dtype))
+            bi = torch.as_tensor(sn["idx"], dtype=torch.int32, device=dev)
+            eng.step(bi, update=False)          # (FP8: records this point's amax for the scales)
+            lv = float(eng.step(bi, update=False).sum())
+            traj["loss_rel"].append(abs(lv - sn["loss"]) / abs(sn["loss"]))
+            for g, ids in groups.items():
+                gv = flat_real([eng.flat.g(mleaves[i]) for i in ids])
+                gr = flat_real([sn["grads"][i] for i in ids])
+                traj[f"grad_rel {g}"].append(float((gv - gr).norm() / gr.norm().clamp_min(1e-300)))
+        row["along_fp64_trajectory"] = {key: {"median": float(np.median(v)), "max": float(np.max(v))}
+                                        for key, v in traj.items()}
+        rows.append(row)
+        print(f"sweep: {row}", file=sys.stderr, flush=True)
     print(json.dumps({"mode": "precision sweep", "workload": wl_name, "loss": loss_type, "epochs": args.epochs,
-                      "steps_per_epoch": len(batches[0]), "rows": rows,
+                      "steps_per_epoch": len(batches[0]), "trajectory_points": len(snaps),
+                      "trajectory_every_steps": args.sweep_every, "rows": rows,
                       "note": "object/probe NRMSE against the FP64 run: global complex scale fitted, no shift; "
-                              "against the truth: shift <= 2 px, phase and scale fitted, coverage mask"}))
+                              "against the truth: shift <= 2 px, phase and scale fitted, coverage mask; "
+                              "along_fp64_trajectory: each variant's batch loss and gradients (relative L2 "
+                              "error per parameter group) at the FP64 run's parameters every "
+                              "trajectory_every_steps steps"}))
 
 
 # ----------------------------------------------------------------------------
@@ -840,6 +914,227 @@ def kernels_mode(args):
     print(json.dumps(out))
 
 
+def counters_mode(args):
+    """The hot kernels once each (after one warm-up round) at the canvas-3648
+    shapes, on inputs made on the GPU, for rocprofv3 --pmc passes: the fused
+    INR forward (hash gather + MLP) and the MLP backward in FP32 and in BF16
+    on the 1824^2 grid (log2T 20), the table-gradient + Adam stage, the
+    Fourier upsample 1824 -> 3648 and its adjoint, the far field (FD, 64
+    patterns of 512^2), the standalone encoder, and a 256 MiB device copy
+    that calibrates the HBM byte counters.  --counters-groups picks a subset;
+    every group synchronises and reports its time, so a slow pass shows where
+    it is."""
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    L = pkg._lib
+    dev = torch.device("cuda:0")
+    s = L.stream_ptr(dev)
+    q = L.query
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+
+    def say(msg):
+        print(f"counters: {msg}", file=sys.stderr, flush=True)
+
+    def inr_group(prec):
+        torch.manual_seed(1)
+        model = pkg.ComplexINRModel2D(3648, 3648, 2, probe_width=512, probe_height=512, log2_hashmap_size=20,
+                                      hidden_omega_0=1.0, trainable_omega0=True, mlp_precision=prec).to(dev)
+        enc, mlp = model.xy_hash_encoder, model.object_complex_mlp
+        with torch.no_grad():
+            enc.params.copy_((torch.rand(enc.params.shape, device=dev, generator=g) - 0.5) * 0.2)
+        spec = enc.spec
+        grid = spec.descriptor("fp32")
+        pts = model.grid_points("object")
+        pd = pts.descriptor(spec)
+        feat = torch.empty(q("pty_points_feature_floats", pts.n), device=dev)
+        pd.features = feat.data_ptr()
+        ts = [t.detach().contiguous() for t in mlp.tensors()]
+        md = pkg.complex_inr.mlp_descriptor(ts, 32, prec)
+        grads = [torch.zeros_like(t) for t in ts]
+        gd = pkg.complex_inr.mlp_descriptor(grads, 32, prec)
+        out = torch.empty(pts.n, dtype=torch.complex64, device=dev)
+        dout = torch.randn(pts.n, dtype=torch.complex64, device=dev, generator=g)
+        ws = L.workspace(q("pty_inr_bwd_workspace", grid, md, pd), dev)
+        m1 = torch.zeros_like(enc.params)
+        v1 = torch.zeros_like(enc.params)
+        hyper = torch.tensor([1e-3, 1.0], device=dev)
+        tab = enc.params.detach()
+
+        def run():
+            L.call("pty_inr_fwd", grid, md, tab.data_ptr(), pd, out.data_ptr(), s)
+            L.call("pty_inr_bwd_mlp", grid, md, tab.data_ptr(), pd, dout.data_ptr(), ws.data_ptr(), ws.numel(), s)
+            L.call("pty_inr_bwd_finish_adam", grid, md, gd, tab.data_ptr(), m1.data_ptr(), v1.data_ptr(),
+                   hyper.data_ptr(), 0.9, 0.999, 1e-8, pd, ws.data_ptr(), ws.numel(), s)
+        # the descriptors hold raw device pointers: keep their owners (the
+        # model's weights, the grid's axes and plan, the feature cache and
+        # the gradient buffers) alive as long as the closure
+        run.owners = (model, pts, feat, ts, grads)
+        return run
+
+    def upsample_group():
+        nsm, ds = 1824, 2
+        small = torch.randn(nsm, nsm, dtype=torch.complex64, device=dev, generator=g)
+        big = torch.empty(nsm * ds, nsm * ds, dtype=torch.complex64, device=dev)
+        wsu = L.workspace(q("pty_upsample_workspace", nsm, ds), dev)
+
+        def run():
+            L.call("pty_upsample_fwd", small.data_ptr(), nsm, ds, big.data_ptr(), wsu.data_ptr(), wsu.numel(), s)
+            L.call("pty_upsample_adj", big.data_ptr(), nsm, ds, small.data_ptr(), wsu.data_ptr(), wsu.numel(), s)
+        return run
+
+    def farfield_group():
+        M, H, B, F = 512, 3648, 64, 128
+        obj = torch.randn(H, H, dtype=torch.complex64, device=dev, generator=g)
+        prb = torch.randn(M, M, dtype=torch.complex64, device=dev, generator=g)
+        stack = torch.rand(F, M, M, device=dev, generator=g) * 10
+        tl = torch.randint(0, H - M, (F, 2), device=dev, dtype=torch.int32, generator=g)
+        bidx = torch.arange(B, device=dev, dtype=torch.int32)
+        loss = torch.empty(B, device=dev)
+        dobj = torch.empty_like(obj)
+        dprb = torch.empty_like(prb)
+        fd = pkg.farfield.far_descriptor(M, M, H, H, B, F, "FD_loss")
+        wsf = L.workspace(q("pty_farfield_workspace", fd), dev)
+
+        def run():
+            L.call("pty_farfield_fwd_bwd", fd, obj.data_ptr(), prb.data_ptr(), stack.data_ptr(), bidx.data_ptr(),
+                   tl.data_ptr(), loss.data_ptr(), dobj.data_ptr(), dprb.data_ptr(), wsf.data_ptr(), wsf.numel(), s)
+        return run
+
+    def hashgrid_group():
+        n = 1824 * 1824
+        spec = pkg.level_table(16, 2, 20, 16, 1.5)
+        ax = (torch.arange(1824, device=dev, dtype=torch.float32) + 0.5) / 1824
+        coords = torch.stack([ax.repeat(1824), ax.repeat_interleave(1824)], 1).contiguous()
+        params = (torch.rand(spec.n_entries * 2, device=dev, generator=g) - 0.5) * 2e-4
+        feats = torch.empty(n, 32, device=dev)
+        desc = spec.descriptor("fp32")
+
+        def run():
+            L.call("pty_hashgrid_fwd", desc, coords.data_ptr(), n, params.data_ptr(), feats.data_ptr(), s)
+        return run
+
+    makers = {"inr_fp32": lambda: inr_group("fp32"), "inr_bf16": lambda: inr_group("bf16"),
+              "upsample": upsample_group, "farfield": farfield_group, "hashgrid": hashgrid_group}
+    names = [n for n in (args.counters_groups or ",".join(makers)).split(",") if n]
+    bad = [n for n in names if n not in makers]
+    if bad:
+        raise SystemExit(f"unknown --counters-groups {bad}; choose from {sorted(makers)}")
+    groups = []
+    for n in names:
+        groups.append((n, makers[n]()))
+        torch.cuda.synchronize()
+        say(f"{n} set up")
+    for it in range(2):                 # warm-up round, then the profiled one
+        for n, run in groups:
+            t0 = time.perf_counter()
+            run()
+            torch.cuda.synchronize()
+            say(f"round {it} {n} {time.perf_counter() - t0:.3f} s")
+    copy_calibration(dev, mib=256, reps=1)
+    torch.cuda.synchronize()
+    say("copy calibration done")
+    print(json.dumps({"mode": "counters", "groups": names, "rounds": 2}))
+
+
+# derived figures of the counter table.  Normalisation (MI355X: 8 XCDs, 256
+# CUs, 1024 SIMDs): GRBM_GUI_ACTIVE is summed over the 8 XCDs, so a kernel's
+# duration in cycles is GRBM_GUI_ACTIVE / 8 (checked against the kernel trace:
+# 2.2-2.4 GHz for every kernel); SQ_VALU_MFMA_BUSY_CYCLES is per SIMD, summed
+# over the 1024 SIMDs; SQ_WAVE_CYCLES, SQ_ACTIVE_INST_VALU and SQ_WAIT_INST_*
+# are wave quad-cycles summed over waves; SQ_LDS_BANK_CONFLICT is stall cycles
+# summed over the 256 CUs; SQ_INSTS_VALU counts MFMA instructions too.
+def _cycles(d):
+    return d["GRBM_GUI_ACTIVE"] / 8
+
+
+_PMC_DERIVED = [
+    ("waves per SIMD", lambda d: d["SQ_WAVE_CYCLES"] * 4 / (1024 * _cycles(d))),
+    ("VALU insts / wave (no MFMA)", lambda d: (d["SQ_INSTS_VALU"] - d["SQ_INSTS_MFMA"]) / d["SQ_WAVES"]),
+    ("MFMA insts / wave", lambda d: d["SQ_INSTS_MFMA"] / d["SQ_WAVES"]),
+    ("transcendental insts / wave", lambda d: d["SQ_INSTS_VALU_TRANS_F32"] / d["SQ_WAVES"]),
+    ("LDS insts / wave", lambda d: d["SQ_INSTS_LDS"] / d["SQ_WAVES"]),
+    ("MFMA busy (% of SIMD cycles)", lambda d: 100 * d["SQ_VALU_MFMA_BUSY_CYCLES"] / (1024 * _cycles(d))),
+    ("VALU active (% of SIMD cycles)", lambda d: 100 * d["SQ_ACTIVE_INST_VALU"] * 4 / (1024 * _cycles(d))),
+    ("LDS bank-conflict stall (% of CU cycles)", lambda d: 100 * d["SQ_LDS_BANK_CONFLICT"] / (256 * _cycles(d))),
+    ("LDS-issue wait (% of wave cycles)", lambda d: 100 * d["SQ_WAIT_INST_LDS"] / d["SQ_WAVE_CYCLES"]),
+    ("any-issue wait (% of wave cycles)", lambda d: 100 * d["SQ_WAIT_INST_ANY"] / d["SQ_WAVE_CYCLES"]),
+]
+
+
+def pmc_table(out_stem, csvs, command):
+    """Per-kernel mean of every counter over its dispatches in rocprofv3
+    --pmc counter_collection CSVs (one CSV per pass), with the derived figures
+    of _PMC_DERIVED; HBM bytes from FETCH_SIZE / WRITE_SIZE (KiB) scaled by
+    what they read for the largest device-copy dispatch (256 MiB read + 256
+    MiB written).  Writes <out_stem>.md and .json."""
+    import csv
+    import re
+    from collections import defaultdict
+    vals = defaultdict(lambda: defaultdict(list))
+    for path in csvs:
+        for r in csv.DictReader(open(path)):
+            name = re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", "").strip()
+            vals[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    mean = {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in vals.items()}
+    true_kib = 256 * 1024
+    factors = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = [v for k, d in vals.items() if "copyBuffer" in k for v in d.get(c, []) if v > true_kib / 4]
+        if cal:
+            factors[c] = true_kib / max(cal)
+    order = ["k_inr_fwd16", "k_inr_bwd16", "k_inr_fwd_b16", "k_inr_bwd_b16", "k_table_hashed", "k_table_dense",
+             "k_dense_pass1", "k_inr_grad_finish", "k_up2_rows", "k_transpose", "k_far_rows_fwd", "k_far_cols",
+             "k_far_fd1", "k_far_fd2", "k_far_rows_bwd", "k_far_dgrad", "k_hashgrid_fwd"]
+    names = [k for o in order for k in sorted(mean) if k.split("::")[-1].startswith(o)]
+    counters = sorted({c for d in mean.values() for c in d})
+    lines = [f"# Hardware counters per kernel ({', '.join(os.path.basename(os.path.dirname(c)) or c for c in csvs)})",
+             "", "Per-dispatch means over the profiled dispatches, one rocprofv3 --pmc pass per counter group, "
+             "run from the repository root on one MI355X:", "", "```", command, "```", ""]
+    if factors:
+        lines += ["HBM bytes: " + ", ".join(f"{c} reports {1 / f:.4f}x a 256 MiB device copy's true bytes, so "
+                                              f"its readings are multiplied by {f:.4f}" for c, f in factors.items()),
+                  ""]
+    lines += ["Derived figures (MI355X: 8 XCDs, 256 CUs, 1024 SIMDs): a kernel's duration in cycles is "
+              "GRBM_GUI_ACTIVE / 8 (the counter is summed over the XCDs; against the kernel trace this gives "
+              "2.2-2.4 GHz for every kernel); MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (1024 x cycles); VALU "
+              "active = 4 SQ_ACTIVE_INST_VALU (quad-cycles) / (1024 x cycles); bank-conflict stall = "
+              "SQ_LDS_BANK_CONFLICT / (256 x cycles); waits are SQ_WAIT_INST_* / SQ_WAVE_CYCLES; waves per "
+              "SIMD = 4 SQ_WAVE_CYCLES / (1024 x cycles); VALU instructions exclude MFMA "
+              "(SQ_INSTS_VALU - SQ_INSTS_MFMA). Read / write MB are the bytes L2 fetched from / wrote to the "
+              "memory fabric (MALL hits count as fetches).", ""]
+    hdr = ["kernel", "cycles"] + [d[0] for d in _PMC_DERIVED] + ["read MB", "write MB"]
+    lines += ["| " + " | ".join(hdr) + " |", "|" + "---|" * len(hdr)]
+    out = {"command": command, "factors": factors, "kernels": {}}
+    for k in names:
+        d = mean[k]
+        row = [k.split("::")[-1], f"{_cycles(d):.3g}" if "GRBM_GUI_ACTIVE" in d else ""]
+        derived = {}
+        for nm, fn in _PMC_DERIVED:
+            try:
+                v = fn(d)
+            except (KeyError, ZeroDivisionError):
+                v = None
+            derived[nm] = v
+            row.append(f"{v:.3g}" if v is not None else "")
+        hb = {}
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            v = d[c] * 1024 * factors[c] / 1e6 if c in d and c in factors else None
+            hb[c] = v
+            row.append(f"{v:.1f}" if v is not None else "")
+        lines.append("| " + " | ".join(row) + " |")
+        out["kernels"][k] = {"counters": d, "derived": derived, "hbm_MB": hb}
+    lines += ["", "Raw per-dispatch means:", "", "| kernel | " + " | ".join(counters) + " |",
+              "|" + "---|" * (len(counters) + 1)]
+    for k in names:
+        lines.append("| " + k.split("::")[-1] + " | " +
+                     " | ".join(f"{mean[k][c]:.4g}" if c in mean[k] else "" for c in counters) + " |")
+    Path(out_stem + ".md").write_text("\n".join(lines) + "\n")
+    Path(out_stem + ".json").write_text(json.dumps(out, indent=1) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -851,7 +1146,8 @@ def main():
     ap.add_argument("--table-precision", choices=["fp32", "fp16"], default="fp32",
                     help="hash-table storage the kernels gather from (f32 master kept for Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart", "kernels"], default="throughput")
+    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart", "kernels", "counters"],
+                    default="throughput")
     ap.add_argument("--ws-train", type=int, default=200, help="warmstart: training instances")
     ap.add_argument("--ws-epochs", type=int, default=8, help="warmstart: net training epochs")
     ap.add_argument("--ws-heldout", type=int, default=3, help="warmstart: held-out seeds")
@@ -860,12 +1156,21 @@ def main():
                     help="throughput: cfg1 (default) | cfg0 | cfg3;  nrmse mode: cfg0 (default) | cfg1 | small")
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--sweep-every", type=int, default=40,
+                    help="sweep mode: steps between points of the FP64 trajectory the variants are evaluated at")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (reference dispatch for HBM byte counters)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 --pmc runs of this bench into profiles/ and exit")
     ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
+    ap.add_argument("--counters-groups", default=None,
+                    help="counters mode: comma list of inr_fp32,inr_bf16,upsample,farfield,hashgrid (default all)")
+    ap.add_argument("--pmc-table", nargs="+", metavar=("OUT_STEM", "CSV"),
+                    help="per-kernel table of rocprofv3 --pmc counter CSVs into OUT_STEM.{md,json} and exit")
     args = ap.parse_args()
+    if args.pmc_table:
+        pmc_table(args.pmc_table[0], args.pmc_table[1:], args.pmc_command)
+        return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.workload or "cfg1", args.precision, args.pmc_command)
         return
@@ -884,6 +1189,9 @@ def main():
     if args.mode == "warmstart":
         args.workload = args.workload or "cfg1"
         warmstart_mode(args)
+        return
+    if args.mode == "counters":
+        counters_mode(args)
         return
     if args.mode == "kernels":
         kernels_mode(args)
