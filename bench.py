@@ -513,7 +513,7 @@ def sweep_mode(args):
             loss, _, _ = _fp64_step_loss(model64, P, inp, inst.tlY, inst.tlX, list(idx), loss_type)
             loss.backward()
             if k % args.sweep_every == 0:
-                snaps.append({"step": k, "idx": [int(i) for i in idx], "loss": float(loss),
+                snaps.append({"step": k, "idx": [int(i) for i in idx], "loss": float(loss.detach()),
                               "params": [t.detach().clone() for t in leaves],
                               "grads": [t.grad.detach().clone() for t in leaves]})
             opt.step()
@@ -568,13 +568,20 @@ def sweep_mode(args):
                 gv = flat_real([eng.flat.g(mleaves[i]) for i in ids])
                 gr = flat_real([sn["grads"][i] for i in ids])
                 traj[f"grad_rel {g}"].append(float((gv - gr).norm() / gr.norm().clamp_min(1e-300)))
-        row["along_fp64_trajectory"] = {key: {"median": float(np.median(v)), "max": float(np.max(v))}
+        row["along_fp64_trajectory"] = {key: {"median": float(np.median(v)), "max": float(np.max(v)),
+                                              "values": [float(f"{x:.3g}") for x in v]}
                                         for key, v in traj.items()}
         rows.append(row)
         print(f"sweep: {row}", file=sys.stderr, flush=True)
+    fp64_norms = {g: [float(f"{float(flat_real([sn['grads'][i] for i in ids]).norm()):.3g}") for sn in snaps]
+                  for g, ids in groups.items()}
     print(json.dumps({"mode": "precision sweep", "workload": wl_name, "loss": loss_type, "epochs": args.epochs,
                       "steps_per_epoch": len(batches[0]), "trajectory_points": len(snaps),
-                      "trajectory_every_steps": args.sweep_every, "rows": rows,
+                      "trajectory_every_steps": args.sweep_every,
+                      "fp64_trajectory": {"step": [sn["step"] for sn in snaps],
+                                          "loss": [float(f"{sn['loss']:.6g}") for sn in snaps],
+                                          "grad_norm": fp64_norms},
+                      "rows": rows,
                       "note": "object/probe NRMSE against the FP64 run: global complex scale fitted, no shift; "
                               "against the truth: shift <= 2 px, phase and scale fitted, coverage mask; "
                               "along_fp64_trajectory: each variant's batch loss and gradients (relative L2 "

@@ -15,12 +15,18 @@ the backward is the hand-written adjoint (``pty_inr_bwd``,
 """
 from __future__ import annotations
 
+import warnings
+
 import torch
 import torch.nn as nn
 
 from . import _lib
 from .hashgrid import HashGridEncoding, table_storage
 from .levels import GridSpec
+
+FP8_WARNING = ("mlp_precision='fp8' is experimental: single steps stay within their error bars, but "
+               "reconstructions do not converge (e4m3 operands perturb the pre-activations that set "
+               "every unit's magnitude); use 'bf16' or 'fp32' for real runs")
 
 
 # ----------------------------------------------------------------------------
@@ -430,6 +436,8 @@ class ComplexINRModel2D(nn.Module):
             self.probe_complex_mlp = None
         if mlp_precision not in _lib.PREC_CODES:
             raise ValueError(f"mlp_precision must be one of {tuple(_lib.PREC_CODES)}")
+        if mlp_precision == "fp8":
+            warnings.warn(FP8_WARNING, stacklevel=2)
         self.mlp_precision = mlp_precision
         self.init_scale_grids(downsample_factor)
         self.has_object_residual = False

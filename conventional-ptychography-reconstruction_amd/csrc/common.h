@@ -246,6 +246,44 @@ __device__ float2* fft_lines(float2* a, float2* b, int n, int log2n, int lines, 
   return a;
 }
 
+// ---------------------------------------------------------------- loop indices
+// (q, r) = (t / d, t % d) along t = t0, t0 + step, ...: one division per
+// thread, then additions (GPU integer division is a long VALU sequence).
+struct SplitIdx {
+  int t, q, r, dq, dr;
+  __device__ __forceinline__ SplitIdx(int t0, int step, int d)
+      : t(t0), q(t0 / d), r(t0 % d), dq(step / d), dr(step % d) {}
+  __device__ __forceinline__ void next(int step, int d) {
+    t += step;
+    q += dq;
+    r += dr;
+    if (r >= d) {
+      r -= d;
+      ++q;
+    }
+  }
+};
+
+// The indices of a Stockham pass loop, (line, j, k) = (t / nb, t % nb,
+// (t % nb) % Ns), along t = t0, t0 + step, ...; nb is a multiple of Ns, so a
+// wrap of j leaves k unchanged.
+struct PassIdx {
+  int t, line, j, k, dl, dj, dk;
+  __device__ __forceinline__ PassIdx(int t0, int step, int nb, int Ns)
+      : t(t0), line(t0 / nb), j(t0 % nb), k((t0 % nb) % Ns), dl(step / nb), dj(step % nb), dk((step % nb) % Ns) {}
+  __device__ __forceinline__ void next(int step, int nb, int Ns) {
+    t += step;
+    line += dl;
+    j += dj;
+    k += dk;
+    if (k >= Ns) k -= Ns;
+    if (j >= nb) {
+      j -= nb;
+      ++line;
+    }
+  }
+};
+
 // ---------------------------------------------------------------- mixed radix
 // Radix plan of a transform length: passes of radix 4 first, then 2, 3, 5 and
 // any remaining prime factors (<= kMaxRadix).  Built on the host.
@@ -269,18 +307,25 @@ template <int R, int DIR>
 __device__ void fft_pass_prime(const float2* __restrict__ a, float2* __restrict__ b, int n, int Ns, int lines,
                                int stride, const float2* tw, int tws, int tid, int nthreads) {
   constexpr int Hh = (R - 1) / 2;
-  float cs[R], sn[R];
-#pragma unroll
-  for (int e = 0; e < R; ++e) sincospif(2.0f * (float)e / (float)R, &sn[e], &cs[e]);
   const int nb = n / R;
+  // cos / sin(2 pi e / R): half of them by sincospi, the rest by symmetry
+  // (reading them from the twiddle table instead lets the compiler hoist R
+  // loads, 235 VGPRs against 168, one wave per SIMD fewer and slower)
+  float cs[R], sn[R];
+  cs[0] = 1.f;
+  sn[0] = 0.f;
+#pragma unroll
+  for (int e = 1; e <= Hh; ++e) {
+    sincospif(2.0f * (float)e / (float)R, &sn[e], &cs[e]);
+    cs[R - e] = cs[e];
+    sn[R - e] = -sn[e];
+  }
   const int total = nb * lines;
   const int twscale = (n / (Ns * R)) * tws;
-  for (int t = tid; t < total; t += nthreads) {
-    const int line = t / nb;
-    const int j = t - line * nb;
-    const float2* src = a + line * stride;
-    float2* dst = b + line * stride;
-    const int k = j % Ns;
+  for (PassIdx x(tid, nthreads, nb, Ns); x.t < total; x.next(nthreads, nb, Ns)) {
+    const int j = x.j, k = x.k;
+    const float2* src = a + x.line * stride;
+    float2* dst = b + x.line * stride;
     const int d = (j - k) * R + k;
     float2 v[R];
 #pragma unroll
@@ -347,12 +392,10 @@ __device__ float2* fft_lines_mixed(float2* a, float2* b, const FftPlan& P, int l
       const int total = nb * lines;
       const int twscale = (n / (Ns * R)) * tws;
       const int wR = (n / R) * tws;          // twiddle stride of the R-point DFT kernel
-      for (int t = tid; t < total; t += nthreads) {
-        const int line = t / nb;
-        const int j = t - line * nb;
-        const float2* src = a + line * stride;
-        float2* dst = b + line * stride;
-        const int k = j % Ns;
+      for (PassIdx x(tid, nthreads, nb, Ns); x.t < total; x.next(nthreads, nb, Ns)) {
+        const int j = x.j, k = x.k;
+        const float2* src = a + x.line * stride;
+        float2* dst = b + x.line * stride;
         const int d = (j - k) * R + k;
         if (R == 8) {
           float2 v[8], y[8];
@@ -441,12 +484,54 @@ __device__ __forceinline__ int embed_freq(int k, int n, int m) {
   return q < 0 ? q + m : q;
 }
 
-// tw[k] = exp(-2 pi i k / n), k < n, accurate sincospi.
-__device__ __forceinline__ void fill_twiddles(float2* tw, int n, int tid, int nthreads) {
-  for (int k = tid; k < n; k += nthreads) {
-    float s, c;
+// Twiddles exp(-2 pi i k / n), k < n.  For n divisible by 8 every entry comes
+// from one sincospi in the first octant, theta0 = 2 pi k0 / n with
+// 0 <= k0 <= n/8, by exact symmetries (with C = cos theta0, S = sin theta0):
+//   k = k0: (C, -S)      n/4 - k0: (S, -C)     n/4 + k0: (-S, -C)
+//   n/2 - k0: (-C, -S)   n/2 + k0: (-C, S)     3n/4 - k0: (-S, C)
+//   3n/4 + k0: (S, C)    n - k0: (C, S)
+// so a table costs n/8 + 1 sincospi and twiddle_value(k, n) gives the same
+// bits as fill_twiddles' entry k.
+__device__ __forceinline__ float2 octant_twiddle(int oct, float C, float S) {
+  switch (oct) {
+    case 0: return make_float2(C, -S);
+    case 1: return make_float2(S, -C);
+    case 2: return make_float2(-S, -C);
+    case 3: return make_float2(-C, -S);
+    case 4: return make_float2(-C, S);
+    case 5: return make_float2(-S, C);
+    case 6: return make_float2(S, C);
+    default: return make_float2(C, S);
+  }
+}
+
+__device__ __forceinline__ float2 twiddle_value(int k, int n) {
+  float s, c;
+  if (n % 8) {
     sincospif(-2.0f * (float)k / (float)n, &s, &c);
-    tw[k] = make_float2(c, s);
+    return make_float2(c, s);
+  }
+  const int q = n / 8, oct = k / q, r = k - oct * q;
+  const int k0 = (oct & 1) ? q - r : r;
+  sincospif(2.0f * (float)k0 / (float)n, &s, &c);
+  return octant_twiddle(oct, c, s);
+}
+
+__device__ __forceinline__ void fill_twiddles(float2* tw, int n, int tid, int nthreads) {
+  if (n % 8) {
+    for (int k = tid; k < n; k += nthreads) tw[k] = twiddle_value(k, n);
+    return;
+  }
+  const int q = n / 8;
+  for (int k0 = tid; k0 <= q; k0 += nthreads) {
+    float S, C;
+    sincospif(2.0f * (float)k0 / (float)n, &S, &C);
+    // octants with k = oct q + k0 (even oct, k0 < q) and (oct + 1) q - k0 (odd oct, k0 > 0)
+#pragma unroll
+    for (int oct = 0; oct < 8; oct += 2) {
+      if (k0 < q) tw[oct * q + k0] = octant_twiddle(oct, C, S);
+      if (k0 > 0) tw[(oct + 2) * q - k0] = octant_twiddle(oct + 1, C, S);
+    }
   }
 }
 

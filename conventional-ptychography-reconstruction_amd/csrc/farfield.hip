@@ -65,8 +65,9 @@ __global__ __launch_bounds__(256) void k_far_rows_fwd(FarArgs a, int lines) {
   fill_twiddles(tw, N, tid, nt);
   const int row0 = blockIdx.x * lines;
   const int total_rows = a.B * a.M;
+  const int lgN = a.lgN;
   for (int t = tid; t < lines * N; t += nt) {
-    const int li = t / N, c = t - li * N;
+    const int li = t >> lgN, c = t & (N - 1);
     const int row = row0 + li;
     float2 v = make_float2(0.f, 0.f);
     if (row < total_rows) {
@@ -80,8 +81,8 @@ __global__ __launch_bounds__(256) void k_far_rows_fwd(FarArgs a, int lines) {
   __syncthreads();
   float2* res = fft_lines<-1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
   for (int t = tid; t < lines * N; t += nt) {
-    const int row = row0 + t / N;
-    if (row < total_rows) a.psi[(int64_t)row * N + (t % N)] = res[t];
+    const int row = row0 + (t >> lgN);
+    if (row < total_rows) a.psi[(int64_t)row * N + (t & (N - 1))] = res[t];
   }
 }
 
@@ -123,18 +124,19 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   fill_twiddles(tw, M, tid, nt);
   const int b = blockIdx.y;
   const int c0 = blockIdx.x * C;
+  const int lgC = __builtin_ctz(C);   // C is a power of two
   int ty, tx, p;
   const bool valid = window_of(a, b, ty, tx, p);
   float2* base = a.psi + (int64_t)b * M * N;
   for (int t = tid; t < M * C; t += nt) {
-    const int r = t / C, c = t - r * C;
+    const int r = t >> lgC, c = t & (C - 1);
     x[c * S + r] = base[(int64_t)r * N + c0 + c];
   }
   __syncthreads();
   if (MODE == 2) {
     float2* res = fft_lines<+1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
     for (int t = tid; t < M * C; t += nt) {
-      const int r = t / C, c = t - r * C;
+      const int r = t >> lgC, c = t & (C - 1);
       base[(int64_t)r * N + c0 + c] = res[c * S + r];
     }
     return;
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   float2* f = fft_lines<-1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
   if (MODE == 1) {
     for (int t = tid; t < M * C; t += nt) {
-      const int r = t / C, c = t - r * C;
+      const int r = t >> lgC, c = t & (C - 1);
       const float2 v = f[c * S + r];
       base[(int64_t)r * N + c0 + c] = v;
       a.inten[(int64_t)b * M * N + (int64_t)r * N + c0 + c] = v.x * v.x + v.y * v.y;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   const float* meas = a.stack + (int64_t)(valid ? p : 0) * M * N;
   KSum acc;
   for (int t = tid; t < M * C; t += nt) {
-    const int r = t / C, c = t - r * C;   // consecutive threads -> consecutive columns
+    const int r = t >> lgC, c = t & (C - 1);   // consecutive threads -> consecutive columns
     float2 v = f[c * S + r];
     float2 gv = make_float2(0.f, 0.f);
     if (valid) {
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   float2* g2 = (f == x) ? y : x;
   float2* res = fft_lines<+1>(f, g2, M, a.lgM, C, S, tw, 1, tid, nt);
   for (int t = tid; t < M * C; t += nt) {
-    const int r = t / C, c = t - r * C;
+    const int r = t >> lgC, c = t & (C - 1);
     base[(int64_t)r * N + c0 + c] = res[c * S + r];
   }
 }
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(256) void k_far_fd1(FarArgs a) {
   const int q0 = blockIdx.x * per, q1 = min(M * N, q0 + per);
   KSum acc;
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-    const int r = q / N, c = q - r * N;
+    const int r = q >> a.lgN, c = q & (N - 1);
     float gxv = 0.f, gyv = 0.f;
     if (valid) {
       const int qx = r * N + ((c + 1) & (N - 1));
@@ -211,8 +213,9 @@ __global__ __launch_bounds__(256) void k_far_fd1(FarArgs a) {
       const float rx = wx - w0, ry = wy - w0;
       const float G = sqrtf(rx * rx + ry * ry + eps);
       acc.add(G);
-      gxv = rx / G * inv_mn;
-      gyv = ry / G * inv_mn;
+      const float s = inv_mn / G;
+      gxv = rx * s;
+      gyv = ry * s;
     }
     a.gx[(int64_t)b * M * N + q] = gxv;
     a.gy[(int64_t)b * M * N + q] = gyv;
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(256) void k_far_fd2(FarArgs a) {
   const int per = (M * N + a.P - 1) / a.P;
   const int q0 = blockIdx.x * per, q1 = min(M * N, q0 + per);
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-    const int r = q / N, c = q - r * N;
+    const int r = q >> a.lgN, c = q & (N - 1);
     const int qxm = r * N + ((c - 1) & (N - 1));
     const int qym = ((r - 1) & (M - 1)) * N + c;
     const float ddw = -a.gx[off + q] - a.gy[off + q] + a.gx[off + qxm] + a.gy[off + qym];
@@ -260,10 +263,11 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   fill_twiddles(tw, N, tid, nt);
   const int r0 = blockIdx.x * lines;
   float2* base = a.psi + (int64_t)b * a.M * N;
-  for (int t = tid; t < lines * N; t += nt) x[t] = base[(int64_t)(r0 + t / N) * N + (t % N)];
+  // the lines of the block are contiguous rows: base + r0 N + t
+  for (int t = tid; t < lines * N; t += nt) x[t] = base[(int64_t)r0 * N + t];
   __syncthreads();
   float2* res = fft_lines<+1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
-  for (int t = tid; t < lines * N; t += nt) base[(int64_t)(r0 + t / N) * N + (t % N)] = res[t];
+  for (int t = tid; t < lines * N; t += nt) base[(int64_t)r0 * N + t] = res[t];
 }
 
 // F4: dprobe = sum_b dpsi_b * conj(patch_b) over valid batch entries (fixed
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
 __device__ __forceinline__ void far_dprobe(const FarArgs& a, int blk, int nblk) {
   const int MN = a.M * a.N;
   for (int q = blk * 256 + (int)threadIdx.x; q < MN; q += nblk * 256) {
-    const int r = q / a.N, c = q - r * a.N;
+    const int r = q >> a.lgN, c = q & (a.N - 1);
     float2 acc = make_float2(0.f, 0.f);
     for (int b = 0; b < a.B; ++b) {
       int ty, tx, p;

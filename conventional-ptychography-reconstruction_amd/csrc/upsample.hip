@@ -46,11 +46,8 @@ __device__ __forceinline__ int crop_freq(int q, int n, int m) {
 }
 
 __global__ void k_twiddle_table(float2* tw, int n) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    float s, c;
-    sincospif(-2.0f * (float)k / (float)n, &s, &c);
-    tw[k] = make_float2(c, s);
-  }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    tw[k] = twiddle_value(k, n);
 }
 
 // out [C, R] = in [R, C]^T (c64), 32 x 32 tiles through LDS; grid
@@ -98,8 +95,8 @@ __global__ __launch_bounds__(256) void k_up_rows(const float2* __restrict__ in, 
   const int r0 = blockIdx.x * lines;
   const int nl = min(lines, rows - r0);
   const int Lin = (MODE == 0) ? n : m;
-  for (int t = tid; t < lines * Lin; t += nt) {
-    const int li = t / Lin, c = t - li * Lin;
+  for (SplitIdx x(tid, nt, Lin); x.t < lines * Lin; x.next(nt, Lin)) {
+    const int li = x.q, c = x.r;
     a[li * m + c] = (li < nl) ? in[(int64_t)(r0 + li) * Lin + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
@@ -107,8 +104,8 @@ __global__ __launch_bounds__(256) void k_up_rows(const float2* __restrict__ in, 
   if (MODE == 0) {
     float2* f = fft_any<-1>(a, b, Pn, lines, m, tw, ds, tid, nt);
     float2* g = (f == a) ? b : a;
-    for (int t = tid; t < lines * m; t += nt) {
-      const int li = t / m, q = t - li * m;
+    for (SplitIdx x(tid, nt, m); x.t < lines * m; x.next(nt, m)) {
+      const int li = x.q, q = x.r;
       const int k = crop_freq(q, n, m);
       g[li * m + q] = (k >= 0) ? f[li * m + k] : make_float2(0.f, 0.f);
     }
@@ -117,16 +114,16 @@ __global__ __launch_bounds__(256) void k_up_rows(const float2* __restrict__ in, 
   } else {
     float2* f = fft_any<-1>(a, b, Pm, lines, m, tw, 1, tid, nt);
     float2* g = (f == a) ? b : a;
-    for (int t = tid; t < lines * n; t += nt) {
-      const int li = t / n, k = t - li * n;
+    for (SplitIdx x(tid, nt, n); x.t < lines * n; x.next(nt, n)) {
+      const int li = x.q, k = x.r;
       g[li * m + k] = f[li * m + embed_freq(k, n, m)];
     }
     __syncthreads();
     res = fft_any<+1>(g, f, Pn, lines, m, tw, ds, tid, nt);
   }
   const int Lout = (MODE == 0) ? m : n;
-  for (int t = tid; t < lines * Lout; t += nt) {
-    const int li = t / Lout, c = t - li * Lout;
+  for (SplitIdx x(tid, nt, Lout); x.t < lines * Lout; x.next(nt, Lout)) {
+    const int li = x.q, c = x.r;
     if (li < nl) out[(int64_t)(r0 + li) * Lout + c] = cscale(res[li * m + c], scale);
   }
 }
@@ -155,28 +152,28 @@ __global__ __launch_bounds__(256) void k_up2_rows(const float2* __restrict__ in,
   float2* b = a + lines * n;
   const int r0 = blockIdx.x * lines;
   const int nl = min(lines, rows - r0);
-  for (int t = tid; t < lines * n; t += nt) {
-    const int li = t / n, j = t - li * n;
+  for (SplitIdx x(tid, nt, n); x.t < lines * n; x.next(nt, n)) {
+    const int li = x.q, j = x.r;
     float2 v = make_float2(0.f, 0.f);
     if (li < nl) v = (MODE == 0) ? in[(int64_t)(r0 + li) * n + j] : in[(int64_t)(r0 + li) * m + 2 * j + 1];
-    a[t] = v;
+    a[x.t] = v;
   }
   __syncthreads();
   float2* f = fft_any<-1>(a, b, Pn, lines, n, tw, 2, tid, nt);
-  for (int t = tid; t < lines * n; t += nt) {
-    const int k = t % n;
+  for (SplitIdx x(tid, nt, n); x.t < lines * n; x.next(nt, n)) {
+    const int k = x.r;
     const int sk = k < h ? k : k - n;          // signed frequency
     const float2 w = tw[sk < 0 ? sk + m : sk];  // exp(-2 pi i s/m) = conj(phi)
-    f[t] = cmul(f[t], (MODE == 0) ? make_float2(w.x, -w.y) : w);
+    f[x.t] = cmul(f[x.t], (MODE == 0) ? make_float2(w.x, -w.y) : w);
   }
   __syncthreads();
   float2* g = (f == a) ? b : a;
   float2* res = fft_any<+1>(f, g, Pn, lines, n, tw, 2, tid, nt);
   const float ns = (float)n * scale;
-  for (int t = tid; t < lines * n; t += nt) {
-    const int li = t / n, j = t - li * n;
+  for (SplitIdx x(tid, nt, n); x.t < lines * n; x.next(nt, n)) {
+    const int li = x.q, j = x.r;
     if (li >= nl) continue;
-    const float2 r = res[t];
+    const float2 r = res[x.t];
     if (MODE == 0) {
       const float2 x = in[(int64_t)(r0 + li) * n + j];
       reinterpret_cast<float4*>(out + (int64_t)(r0 + li) * m)[j] =

@@ -92,3 +92,17 @@ def test_changed_coordinates_are_rejected_by_the_fused_step(pkg):
     with torch.no_grad():
         m.xy_coords[5, 0] = orig
     assert m.coords_are_grid("object")
+
+
+def test_fp8_is_flagged_experimental(pkg):
+    """FP8 single steps meet their bars, but FP8 reconstructions do not converge
+    (the precision sweep along the FP64 trajectory): selecting it warns."""
+    import warnings
+    for prec, warns in (("fp8", True), ("bf16", False), ("fp32", False)):
+        with warnings.catch_warnings(record=True) as rec:
+            warnings.simplefilter("always")
+            torch.manual_seed(0)
+            pkg.ComplexINRModel2D(32, 32, 2, probe_width=16, probe_height=16, log2_hashmap_size=10,
+                                  mlp_precision=prec)
+        hit = [w for w in rec if "experimental" in str(w.message)]
+        assert bool(hit) == warns, (prec, [str(w.message) for w in rec])

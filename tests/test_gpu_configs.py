@@ -273,15 +273,17 @@ def test_canvas3648_bf16_fp16_step_near_fp32(pkg, dev, inst3648):
 
     Bars: loss 2e-2; gradients 5e-2 relative in norm (the BF16 gradient bar;
     frequencies on their homogeneity scale), except the object and probe
-    tables and layer-0 weights, 3e-1.  Those four
+    tables and layer-0 weights, 1.5e-1.  Those four
     are sums over all grid points of products with the per-point gradient
     Im(g_z conj z0), which oscillates at the pixel scale (it comes from the
     far-field adjoint), so its smooth part (what a coarse-level table entry
     or W0 = sum g h^T collects) is small against its pointwise size.  BF16
     rounds each point's operand to 2^-9 relative with white noise, which
-    survives that cancellation: measured 0.13 (object table) and 0.19 (object
-    W0) here, against 5e-3 and 3e-3 at features of order 0.1 (the
-    canvas-1024 reduced-precision test)."""
+    survives that cancellation: measured 0.030 (object table), 0.045 (probe
+    table) and 0.054 (object W0) with the 16-point BF16 kernels (0.13 and
+    0.19 for the object's with the earlier 32-point kernels), against 5e-3
+    and 3e-3 at features of order 0.1 (the canvas-1024 reduced-precision
+    test)."""
     sim, stack = inst3648
     m1, e1 = _engine3648(pkg, dev, sim, stack, seed=6)
     m2, e2 = _engine3648(pkg, dev, sim, stack, seed=6, precision="bf16", table_precision="fp16")
@@ -305,4 +307,4 @@ def test_canvas3648_bf16_fp16_step_near_fp32(pkg, dev, inst3648):
     loose = ("xy_hash_encoder.params", "probe_hash_encoder.params", "object_complex_mlp.net.0.linear.weight",
              "probe_complex_mlp.net.0.linear.weight")
     for k, e in errs.items():
-        assert e < (3e-1 if k in loose else 5e-2), (k, e)
+        assert e < (1.5e-1 if k in loose else 5e-2), (k, e)

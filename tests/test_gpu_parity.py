@@ -16,11 +16,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _model(pkg, dev, canvas, M, log2T=12, residual=True, probe_init=None, seed=0, hidden_layers=2,
-           table_precision="fp32", mlp_precision="fp32"):
+           table_precision="fp32", mlp_precision="fp32", first_omega=10.0, hidden_omega=1.0):
     torch.manual_seed(seed)
     init = probe_init if probe_init is not None else torch.ones(M, M, dtype=torch.complex64)
     return pkg.ComplexINRModel2D(canvas, canvas, 2, probe_width=M, probe_height=M, log2_hashmap_size=log2T,
-                                 hidden_omega_0=1.0, trainable_omega0=True, use_residual=residual,
+                                 first_omega_0=first_omega, hidden_omega_0=hidden_omega, trainable_omega0=True,
+                                 use_residual=residual,
                                  probe_initial=init if residual else None, hidden_layers=hidden_layers,
                                  table_precision=table_precision, mlp_precision=mlp_precision).to(dev)
 
@@ -866,15 +867,19 @@ def test_inr_field_fp8(pkg, dev, hidden_layers):
         assert e < 3e-1, (shape, e)
 
 
-@pytest.mark.parametrize("hidden_layers", [1, 2])
-def test_inr_field_bf16(pkg, dev, hidden_layers):
+@pytest.mark.parametrize("hidden_layers,omegas", [(1, (10.0, 1.0)), (2, (10.0, 1.0)), (2, (13.0, 1.7))])
+def test_inr_field_bf16(pkg, dev, hidden_layers, omegas):
     """BF16 MFMA path (fp32 accumulate): field and gradients against the FP64 oracle.
 
     Tolerance: bf16 operands carry 8 mantissa bits (relative 2^-9 per rounding);
     with features of order 0.1 and omega_0 = 10 the phases move by ~1e-3 rad, so
-    the field is held to 2e-2 and the gradients to 5e-2 relative (norm).
+    the field is held to 2e-2 and the gradients to 5e-2 relative (norm).  The
+    kernels fold each layer's frequency into the bf16 weight operands, so one
+    case runs non-unit frequencies on every layer (first 13, hidden 1.7),
+    where a frequency applied twice or not at all would show.
     """
-    model = _model(pkg, dev, 64, 16, log2T=12, residual=False, hidden_layers=hidden_layers)
+    model = _model(pkg, dev, 64, 16, log2T=12, residual=False, hidden_layers=hidden_layers,
+                   first_omega=omegas[0], hidden_omega=omegas[1])
     model.mlp_precision = "bf16"
     with torch.no_grad():
         model.xy_hash_encoder.params.uniform_(-0.1, 0.1)

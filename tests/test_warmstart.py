@@ -26,6 +26,9 @@ def test_net_and_patch_shapes_cpu(pkg):
     inp = ws.net_inputs(intens, np.arange(5) * 10, np.arange(5) * 20, 128, 512)
     assert inp.shape == (5, 3, 64, 64)
     assert torch.allclose(inp[:, 0], torch.nn.functional.adaptive_avg_pool2d(torch.log1p(intens)[:, None], 64)[:, 0])
+    # the offset planes: window centre / canvas - 1/2
+    assert torch.allclose(inp[:, 1], ((torch.arange(5.) * 10 + 64) / 512 - 0.5)[:, None, None].expand(5, 64, 64))
+    assert torch.allclose(inp[:, 2], ((torch.arange(5.) * 20 + 64) / 512 - 0.5)[:, None, None].expand(5, 64, 64))
     obj = torch.polar(torch.ones(512, 512), torch.zeros(512, 512))
     t = ws.patch_targets(obj, np.arange(5) * 10, np.arange(5) * 20, 128)
     assert t.shape == (5, 2, 32, 32) and torch.allclose(t[:, 0], torch.ones(5, 32, 32))
@@ -53,7 +56,7 @@ def test_pack_params_layout_cpu(pkg):
 def test_wsnet_inputs_match_net_inputs(pkg, dev):
     ws = pkg.warmstart
     g = torch.Generator(device=dev).manual_seed(3)
-    for M, canvas in ((64, 256), (256, 1024)):
+    for M, canvas in ((64, 256), (128, 512), (256, 1024)):
         I = torch.poisson(torch.rand(7, M, M, generator=g, device=dev) * 50.0, generator=g)
         ty = torch.randint(0, canvas - M + 1, (7,), generator=g, device=dev)
         tx = torch.randint(0, canvas - M + 1, (7,), generator=g, device=dev)
