@@ -1052,6 +1052,102 @@ def counters_mode(args):
 # over the 1024 SIMDs; SQ_WAVE_CYCLES, SQ_ACTIVE_INST_VALU and SQ_WAIT_INST_*
 # are wave quad-cycles summed over waves; SQ_LDS_BANK_CONFLICT is stall cycles
 # summed over the 256 CUs; SQ_INSTS_VALU counts MFMA instructions too.
+GS_WORKLOADS = {
+    # name: (n, canvas, parameterisation, max_patch_radius, min_scale)
+    "gs50k_cholesky": (50000, 1024, "cholesky", 16, 1.0),
+    "gs50k_rs": (50000, 1024, "rs", 8, 0.5),
+}
+GS_FLOP_FWD = 15     # per (Gaussian, pixel) evaluation: offsets, quadratic form, exp2, two accumulations
+GS_FLOP_BWD = 30     # the same plus the seven gradient sums
+
+
+def gscp_mode(args):
+    """Gaussian-field rasterizer (pty_gs_*, the reference's gscp CUDA
+    rasterizer) forward + backward through GaussianRasterize2D[Cholesky] on
+    the synthetic scene of the reference's simulate_and_reconstruct.py:170
+    (50,000 object Gaussians, canvas 1024^2 at downsample 1); Cholesky with
+    max_patch_radius 16 / min_scale 1 (config/simulation_conventional_
+    westwest_ng50000.yaml) and RS with the model defaults 8 / 0.5.  A step is
+    one forward (including its count read-back, as the reference's) and one
+    backward.  Throughput is reported as Gaussian-pixel evaluations per
+    second; the CPU baseline is the oracle (numpy) on a sample of tiles."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+    from oracle import gscp_oracle as go
+    pkg = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    name = args.workload or "gs50k_cholesky"
+    n, canvas, param, smax, smin = GS_WORKLOADS[name]
+    rng = np.random.default_rng(0)
+    xy = rng.uniform(0, canvas - 1, size=(n, 2))
+    raw = rng.normal(0.6, 0.45, size=(n, 2))
+    sig = smin + (smax - smin) / (1 + np.exp(-raw)) * 0.5
+    p0 = np.log(sig)
+    p1 = rng.normal(0, 0.1, size=(n, 1)) if param == "rs" else rng.normal(0, 1.0, size=(n,)) * sig[:, 0] * 0.5
+    w = rng.normal(size=(n, 2))
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    xy, p0, p1, w = map(f32, (xy, p0, p1, w))
+    T = [torch.from_numpy(a).to(dev).requires_grad_(True) for a in (xy, p0, p1, w)]
+    fn = pkg.gscp.GaussianRasterize2D if param == "rs" else pkg.gscp.GaussianRasterize2DCholesky
+    dout = torch.randn(2, canvas, canvas, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
+    fwd_ms, bwd_ms = [], []
+
+    def step(record):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        out = fn.apply(*T, canvas, canvas, smax, smin)
+        e[1].record()
+        out.backward(dout)
+        e[2].record()
+        if record:
+            fwd_ms.append(e)
+        return out
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    f_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in fwd_ms]))
+    b_ms = float(np.mean([b.elapsed_time(c) for _, b, c in fwd_ms]))
+    R = pkg.gscp.rasterize_forward(*[t.detach() for t in T], canvas, canvas, smax, smin)[0] if param == "rs" else \
+        pkg.gscp.rasterize_forward_cholesky(*[t.detach() for t in T], canvas, canvas, smax, smin)[0]
+    evals = R * 256
+    ms_step = wall * 1e3 / args.steps
+    # CPU baseline: the oracle's render + backward on a sample of 32 tiles, scaled to every tile
+    pre = go.preprocess_rs if param == "rs" else go.preprocess_cholesky
+    t1 = time.perf_counter()
+    geom = pre(xy, p0, p1, canvas, canvas, smax, smin)
+    lists = go.tile_lists(geom)
+    t_pre = time.perf_counter() - t1
+    tiles = rng.choice(len(lists), size=32, replace=False)
+    keep = set(tiles.tolist())
+    sel = [l if t in keep else [] for t, l in enumerate(lists)]
+    d64 = dout.cpu().numpy().astype(np.float64)
+    t1 = time.perf_counter()
+    go.render(geom, w, sel)
+    go.render_backward(geom, w, d64, sel)
+    t_tiles = time.perf_counter() - t1
+    sel_pairs = sum(len(lists[t]) for t in tiles)
+    cpu_s = t_pre + t_tiles * R / max(sel_pairs, 1)
+    print(json.dumps({
+        "mode": "gaussian-field rasterizer", "workload": name, "n_gaussians": n, "canvas": canvas,
+        "param": param, "max_patch_radius": smax, "min_scale": smin, "steps": args.steps, "warmup": args.warmup,
+        "pairs": R, "tiles_per_gaussian": R / n, "ms_per_step": ms_step, "fwd_ms": f_ms, "bwd_ms": b_ms,
+        "evaluations_per_s": 2 * evals / (ms_step * 1e-3),
+        "fwd_tflops": evals * GS_FLOP_FWD / (f_ms * 1e-3) / 1e12,
+        "bwd_tflops": evals * GS_FLOP_BWD / (b_ms * 1e-3) / 1e12,
+        "flop_per_evaluation": {"fwd": GS_FLOP_FWD, "bwd": GS_FLOP_BWD}, "peak_fp32_tflops": 157.3,
+        "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "steps/s", "cores": 1, "kind": "port",
+                         "sample": f"oracle (numpy FP64) geometry + tile lists of all {n} Gaussians, render + "
+                                   f"backward of 32 tiles ({sel_pairs} pairs) scaled to {R} pairs"},
+    }), flush=True)
+
+
 def _cycles(d):
     return d["GRBM_GUI_ACTIVE"] / 8
 
@@ -1153,14 +1249,15 @@ def main():
     ap.add_argument("--table-precision", choices=["fp32", "fp16"], default="fp32",
                     help="hash-table storage the kernels gather from (f32 master kept for Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart", "kernels", "counters"],
+    ap.add_argument("--mode", choices=["throughput", "nrmse", "sweep", "warmstart", "kernels", "counters", "gscp"],
                     default="throughput")
     ap.add_argument("--ws-train", type=int, default=200, help="warmstart: training instances")
     ap.add_argument("--ws-epochs", type=int, default=8, help="warmstart: net training epochs")
     ap.add_argument("--ws-heldout", type=int, default=3, help="warmstart: held-out seeds")
     ap.add_argument("--loss", default=None, help="sweep mode: loss type (default Poisson_likelihood_loss)")
     ap.add_argument("--workload", default=None,
-                    help="throughput: cfg1 (default) | cfg0 | cfg3;  nrmse mode: cfg0 (default) | cfg1 | small")
+                    help="throughput: cfg1 (default) | cfg0 | cfg3;  nrmse mode: cfg0 (default) | cfg1 | small;  "
+                         "gscp mode: gs50k_cholesky (default) | gs50k_rs")
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--sweep-every", type=int, default=40,
@@ -1202,6 +1299,9 @@ def main():
         return
     if args.mode == "kernels":
         kernels_mode(args)
+        return
+    if args.mode == "gscp":
+        gscp_mode(args)
         return
     args.workload = args.workload or "cfg1"
     shapes, _, batch, loss_type, wl_name = WORKLOADS[args.workload]

@@ -7,12 +7,14 @@ Drop-in surface (same names and semantics as the reference):
   haar_wavelet_sparsity_loss                           utils/loss.py:60-83
   scan_positions_to_pixels                             utils/optics.py:68-99
   load_config / load_ptychography_data / save_model    Recovery.py:44-72, utils/io.py
+  gscp.GaussianRasterize2D[Cholesky], gscp.rasterize_* gscp/cuda/_wrapper.py, gscp/csrc (the
+                                                       Gaussian-field rasterizer)
 The learned object warm start (``warmstart``) is an addition without a
 counterpart in the reference.
 Compute runs in the hand-written HIP kernels of ``libptycho_hip.so`` (C ABI in
 include/ptycho_hip.h); there is no CPU or PyTorch fallback.
 """
-from . import _lib, loss, metrics, recovery, synthetic, warmstart
+from . import _lib, gscp, loss, metrics, recovery, synthetic, warmstart
 from .complex_inr import (ComplexGaborLayer2D, ComplexINRModel2D, ComplexMLP, CoordPoints, GridPoints,
                           InrFieldFn, ResidualFn, UpsampleFn, fourier_upsample, mlp_descriptor, mlp_forward)
 from .engine import FlatParams, FusedReconstruction

@@ -12,7 +12,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libptycho_hip.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 MAX_LEVELS = 16
 MAX_HIDDEN = 4
 FAR_SCRATCH_BYTES = 16 << 20   # PTY_FAR_SCRATCH_BYTES
@@ -63,6 +63,15 @@ class PtyLayer(ctypes.Structure):
         ("weight", c_void_p), ("bias", c_void_p), ("omega", c_void_p),
     ]
 
+
+class PtyGs(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int32), ("width", c_int32), ("height", c_int32), ("max_patch_radius", c_int32),
+        ("min_scale", c_float), ("param", c_int32),
+    ]
+
+
+GS_PARAM_CODES = {"rs": 0, "cholesky": 1}
 
 P = ctypes.POINTER
 _SIGNATURES = {
@@ -115,6 +124,13 @@ _SIGNATURES = {
     "pty_batch_next": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_int64, c_void_p]),
     "pty_fill": (c_int32, [c_void_p, c_int64, c_float, c_void_p]),
+    "pty_gs_geom_bytes": (c_size_t, [P(PtyGs)]),
+    "pty_gs_preprocess": (c_int32, [P(PtyGs), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pty_gs_num_rendered": (c_int32, [P(PtyGs), c_void_p, P(c_int64), c_void_p]),
+    "pty_gs_bin_bytes": (c_size_t, [P(PtyGs), c_int64, c_int32]),
+    "pty_gs_render_fwd": (c_int32, [P(PtyGs), c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pty_gs_render_bwd": (c_int32, [P(PtyGs), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pty_wsnet_param_count": (c_size_t, [c_int32]),
     "pty_wsnet_workspace": (c_size_t, [c_int32, c_int32]),
     "pty_wsnet_inputs": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),

@@ -4,7 +4,8 @@
  * Every entry point takes raw device pointers, plain sizes and a hipStream_t,
  * launches asynchronously on that stream, never allocates device memory and
  * never synchronises (caller-provided workspace, sized by the *_workspace
- * queries).  Return value: 0 on success, a negative PTY_E* code on error,
+ * queries); the one exception is pty_gs_num_rendered, which reads a count
+ * back to the host as the reference's rasterizer does.  Return value: 0 on success, a negative PTY_E* code on error,
  * with a message available from pty_last_error() (thread-local).
  *
  * Complex arrays are interleaved (re, im) float pairs, i.e. the memory of a
@@ -19,6 +20,7 @@
  *   per-position far field + losses    Recovery.py:287-307 (utils/optics.py:102-112)
  *   haar_wavelet_sparsity_loss         utils/loss.py:60-83, Recovery.py:309-313
  *   torch.optim.Adam step              Recovery.py:238-241, 321
+ *   Gaussian-field rasterizer (gscp)   gscp/cuda/_wrapper.py:18-166, gscp/csrc
  */
 #ifndef PTYCHO_HIP_H
 #define PTYCHO_HIP_H
@@ -32,7 +34,7 @@ extern "C" {
 
 typedef struct ihipStream_t* hipStream_t;
 
-#define PTY_ABI_VERSION 6
+#define PTY_ABI_VERSION 7
 #define PTY_MAX_LEVELS 16
 #define PTY_MAX_HIDDEN 4
 #define PTY_FP8_STATE_FLOATS 520
@@ -350,6 +352,53 @@ int pty_wsnet_mse(const float* out, const float* target, int64_t n, float* loss,
  * same workspace, batch and parameters. */
 int pty_wsnet_backward(int32_t base, const float* params, int32_t batch, const float* dout, float* grads, void* work,
                        size_t work_bytes, hipStream_t stream);
+
+/* ---- 2-D Gaussian-field rasterizer (gscp/csrc: model_type gaussian_field) ----
+ * Replaces the reference's CUDA rasterizer behind
+ * gscp/cuda/_wrapper.py:18-166 (GaussianRasterize2D[Cholesky]) and its
+ * bindings gscp/csrc/rasterize_gaussians.cpp:26-254, ext.cpp:69-74:
+ * preprocess (gscp_rasterizer/forward.cu:18-170), tile binning
+ * (rasterizer_impl.cu:45-106, 149-249), tile render (forward.cu:176-258),
+ * render backward (backward.cu:21-133) and preprocess backward
+ * (backward.cu:138-266).  16 x 16 pixel tiles, two channels (real, imag),
+ * legacy circular AABB (USE_SNUGBOX 0).
+ * Inputs: xy [n, 2] pixel positions (clamped to the canvas), weights [n, 2];
+ * param PTY_GS_RS: p0 = scaling [n, 2] (log sigma), p1 = rotation [n];
+ * param PTY_GS_CHOLESKY: p0 = log_L_diag [n, 2], p1 = L_offdiag [n].
+ * out [2, height, width] f32.  Within a tile the Gaussians are summed in
+ * ascending index (the reference's stable sort); the backward sums each
+ * Gaussian's per-tile partials in a fixed order (no atomics): every result is
+ * deterministic. */
+enum { PTY_GS_RS = 0, PTY_GS_CHOLESKY = 1 };
+typedef struct PtyGs {
+  int32_t n;                /* Gaussians, 0 .. 2^24 */
+  int32_t width, height;    /* canvas W, H: 1 .. 16384 */
+  int32_t max_patch_radius; /* upper clamp of the activated scales (pixels) */
+  float min_scale;          /* lower clamp of the activated scales (pixels) */
+  int32_t param;            /* PTY_GS_RS or PTY_GS_CHOLESKY */
+} PtyGs;
+/* Per-Gaussian and per-tile state (the reference's geomBuffer + imgBuffer). */
+size_t pty_gs_geom_bytes(const PtyGs* d);
+/* radii [n] int32 (0: culled) and the geometry state; counts the
+ * (Gaussian, tile) pairs per tile (forward.cu:18-170, rasterizer_impl.cu:186-192). */
+int pty_gs_preprocess(const PtyGs* d, const float* xy, const float* p0, const float* p1, int32_t* radii,
+                      void* geom, hipStream_t stream);
+/* The number of (Gaussian, tile) pairs: copies it to the host and waits for
+ * the stream (the one synchronising call, as rasterizer_impl.cu:194-197). */
+int pty_gs_num_rendered(const PtyGs* d, const void* geom, int64_t* n_rendered, hipStream_t stream);
+/* Per-pair state (the reference's binningBuffer): tile lists, and with
+ * with_grad != 0 the backward's per-pair partial sums. */
+size_t pty_gs_bin_bytes(const PtyGs* d, int64_t n_rendered, int32_t with_grad);
+/* Builds the tile lists in `bin` (per-tile cursors in `geom`) and writes
+ * every pixel of out. */
+int pty_gs_render_fwd(const PtyGs* d, void* geom, int64_t n_rendered, const float* weights, void* bin,
+                      float* out, hipStream_t stream);
+/* dxy [n,2], dp0 [n,2], dp1 [n], dweight [n,2] (overwritten; zero where
+ * radii <= 0) from dout [2, height, width], using the tile lists of the last
+ * pty_gs_render_fwd on the same geom and bin (bin sized with with_grad). */
+int pty_gs_render_bwd(const PtyGs* d, const float* xy, const float* p0, const float* p1, const float* weights,
+                      const int32_t* radii, const void* geom, int64_t n_rendered, void* bin, const float* dout,
+                      float* dxy, float* dp0, float* dp1, float* dweight, hipStream_t stream);
 
 /* fill kernel (graph-safe replacement for hipMemsetAsync) */
 int pty_fill(float* p, int64_t n, float value, hipStream_t stream);
