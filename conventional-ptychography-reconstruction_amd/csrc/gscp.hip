@@ -4,12 +4,15 @@
 //
 // Stages (reference: gscp/csrc/gscp_rasterizer/*):
 //   preprocess   one lane per Gaussian: clamped mean, conic, radius, tile rect,
-//                tile count; per-tile pair counts by integer atomics
-//                (forward.cu:18-170, auxiliary.h:17-30)
+//                tile count; per-tile pair counts from a 2-D difference array
+//                of the rects (four integer atomics per Gaussian) and its
+//                prefix sums (forward.cu:18-170, auxiliary.h:17-30)
 //   scans        exclusive scans of the per-Gaussian and per-tile counts
 //                (rasterizer_impl.cu:186-192)
 //   binning      each Gaussian writes its id into the lists of the tiles its
-//                rect covers, then every tile sorts its list in LDS: ids are
+//                rect covers (slots from per-chunk LDS counters and a column
+//                scan of the chunk counts; global atomics only on canvases
+//                beyond 16384 tiles), then every tile sorts its list: ids are
 //                unique, so the order is the ascending index the reference's
 //                stable radix sort gives (rasterizer_impl.cu:45-106, 212-240)
 //   render       one 256-lane workgroup (4 waves) per 16 x 16 tile, one pixel
@@ -32,14 +35,17 @@ constexpr int kTile = 16;              // 16 x 16 pixel tiles (gscp config.h:13-
 constexpr int kTilePix = kTile * kTile;
 constexpr int kScanBlock = 1024;       // elements per workgroup in the scans
 constexpr int kSortCap = 8192;         // tile lists up to this length sort in LDS
+constexpr int kLdsTiles = 16384;       // canvases up to this many tiles bin through LDS counters
+constexpr int kMaxChunks = 512;        // Gaussian chunks of the LDS binning
 constexpr float kGMin = 1e-7f;         // forward.cu:243
 constexpr float kLog2e = 1.4426950408889634f;
 
 inline size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
 
 struct GeomLayout {
-  size_t mean, conic, rect, goff, touched, gsums, count, start, cursor, tsums, total, bytes;
+  size_t mean, conic, rect, goff, touched, gsums, diff, count, start, cursor, tsums, total, chunk, bytes;
   int tiles_x, tiles_y, tiles, nb_g, nb_t;
+  int chunk_size, n_chunks;   // LDS binning: Gaussians per chunk and chunks (0: global-atomic binning)
 };
 
 GeomLayout geom_layout(const PtyGs* d) {
@@ -57,11 +63,20 @@ GeomLayout geom_layout(const PtyGs* d) {
   L.goff = o;    o = align_up(o + n * sizeof(uint32_t));
   L.touched = o; o = align_up(o + n * sizeof(uint32_t));
   L.gsums = o;   o = align_up(o + (size_t)(L.nb_g + 1) * sizeof(uint32_t));
+  L.diff = o;    o = align_up(o + (size_t)(L.tiles_x + 1) * (L.tiles_y + 1) * sizeof(int));
   L.count = o;   o = align_up(o + (size_t)L.tiles * sizeof(uint32_t));
   L.start = o;   o = align_up(o + (size_t)L.tiles * sizeof(uint32_t));
   L.cursor = o;  o = align_up(o + (size_t)L.tiles * sizeof(uint32_t));
   L.tsums = o;   o = align_up(o + (size_t)(L.nb_t + 1) * sizeof(uint32_t));
   L.total = o;   o = align_up(o + 2 * sizeof(unsigned long long));
+  L.chunk_size = 0;
+  L.n_chunks = 0;
+  if (L.tiles <= kLdsTiles && n > 0) {
+    L.chunk_size = 256;
+    while ((n + L.chunk_size - 1) / L.chunk_size > (size_t)kMaxChunks) L.chunk_size *= 2;
+    L.n_chunks = (int)((n + L.chunk_size - 1) / L.chunk_size);
+  }
+  L.chunk = o;   o = align_up(o + (size_t)L.n_chunks * L.tiles * sizeof(uint32_t));
   L.bytes = o;
   return L;
 }
@@ -137,7 +152,7 @@ __global__ __launch_bounds__(256) void k_gs_preprocess(
     int n, int W, int H, const float* __restrict__ xy, const float* __restrict__ p0, const float* __restrict__ p1,
     float smax, float smin, int gx, int gy, int32_t* __restrict__ radii, float2* __restrict__ mean,
     float4* __restrict__ conic, int4* __restrict__ rect, uint32_t* __restrict__ touched,
-    uint32_t* __restrict__ count) {
+    int* __restrict__ diff) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -161,9 +176,18 @@ __global__ __launch_bounds__(256) void k_gs_preprocess(
   const int4 rc = tile_rect(mx, my, r, gx, gy);
   rect[i] = rc;
   touched[i] = (uint32_t)((rc.w - rc.y) * (rc.z - rc.x));
-  for (int ty = rc.y; ty < rc.w; ++ty)
-    for (int tx = rc.x; tx < rc.z; ++tx) atomicAdd(&count[ty * gx + tx], 1u);
+  // the rect as +1 / -1 at its four corners of a (gy+1) x (gx+1) difference
+  // array; its 2-D prefix sum is the per-tile pair count (four atomics per
+  // Gaussian instead of one per covered tile)
+  const int gx1 = gx + 1;
+  atomicAdd(&diff[rc.y * gx1 + rc.x], 1);
+  atomicAdd(&diff[rc.y * gx1 + rc.z], -1);
+  atomicAdd(&diff[rc.w * gx1 + rc.x], -1);
+  atomicAdd(&diff[rc.w * gx1 + rc.z], 1);
 }
+
+// count[ty][tx] = sum of diff over rows <= ty and columns <= tx.  The
+// prefix kernels are defined after the scan helpers below.
 
 // ------------------------------------------------------------------ scans
 // Exclusive scan of kScanBlock elements per workgroup (256 lanes x 4); the
@@ -239,6 +263,40 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ out, co
   if (i < n) out[i] += sums[i / kScanBlock];
 }
 
+// Inclusive prefix along x of every row of the (rows) x (gx1) difference
+// array, in place; one workgroup per row.
+__global__ __launch_bounds__(256) void k_diff_rows(uint32_t* __restrict__ diff, int gx1) {
+  __shared__ uint32_t s_w[16];
+  uint32_t* row = diff + (size_t)blockIdx.x * gx1;
+  uint32_t carry = 0u;
+  for (int c = 0; c < gx1; c += 256) {
+    const int x = c + threadIdx.x;
+    const uint32_t v = x < gx1 ? row[x] : 0u;
+    uint32_t tot;
+    const uint32_t incl = block_incl_scan(v, s_w, &tot);
+    if (x < gx1) row[x] = carry + incl;
+    carry += tot;
+  }
+}
+
+// Inclusive prefix along y of the row-prefixed difference array: the pair
+// count of every tile; one workgroup per tile column.  Integer sums wrap
+// modulo 2^32, and every count is a non-negative total below 2^32.
+__global__ __launch_bounds__(256) void k_diff_cols(const uint32_t* __restrict__ diff, int gx, int gy,
+                                                   uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_w[16];
+  const int x = blockIdx.x;
+  uint32_t carry = 0u;
+  for (int c = 0; c < gy; c += 256) {
+    const int y = c + threadIdx.x;
+    const uint32_t v = y < gy ? diff[(size_t)y * (gx + 1) + x] : 0u;
+    uint32_t tot;
+    const uint32_t incl = block_incl_scan(v, s_w, &tot);
+    if (y < gy) count[(size_t)y * gx + x] = carry + incl;
+    carry += tot;
+  }
+}
+
 int exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t* sums, int nb, int64_t n,
                    unsigned long long* total, hipStream_t s) {
   if (n > 0) {
@@ -259,12 +317,97 @@ __global__ __launch_bounds__(256) void k_gs_scatter(int n, int gx, const uint32_
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || touched[i] == 0u) return;
   const int4 rc = rect[i];
-  for (int ty = rc.y; ty < rc.w; ++ty)
-    for (int tx = rc.x; tx < rc.z; ++tx) {
-      const int t = ty * gx + tx;
-      const uint32_t slot = atomicAdd(&cursor[t], 1u);
-      list_u[start[t] + slot] = (uint32_t)i;
+  const int w = rc.z - rc.x, cnt = w * (rc.w - rc.y);
+  // eight slot requests in flight before their results are used
+  for (int base = 0; base < cnt; base += 8) {
+    int tt[8];
+    uint32_t slot[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = base + u;
+      tt[u] = -1;
+      if (k < cnt) {
+        tt[u] = (rc.y + k / w) * gx + rc.x + k % w;
+        slot[u] = atomicAdd(&cursor[tt[u]], 1u);
+      }
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (tt[u] >= 0) list_u[start[tt[u]] + slot[u]] = (uint32_t)i;
+  }
+}
+
+// LDS binning (canvases of at most kLdsTiles tiles).  Chunk c holds Gaussians
+// [c * chunk, (c + 1) * chunk); its workgroup counts the chunk's pairs per
+// tile with LDS atomics into hist[c][t] (pass 1), a column scan over the
+// chunks turns the counts into each chunk's offset inside every tile's list
+// (pass 2), and pass 3 hands out slots from LDS counters.  No global atomics;
+// the order inside a tile is settled by the sort that follows.
+__global__ __launch_bounds__(256) void k_gs_chunk_hist(int n, int chunk, int tiles, int gx,
+                                                       const uint32_t* __restrict__ touched,
+                                                       const int4* __restrict__ rect, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kLdsTiles];
+  for (int t = threadIdx.x; t < tiles; t += 256) h[t] = 0u;
+  __syncthreads();
+  const int g0 = blockIdx.x * chunk, g1 = min(n, g0 + chunk);
+  for (int i = g0 + threadIdx.x; i < g1; i += 256) {
+    if (touched[i] == 0u) continue;
+    const int4 rc = rect[i];
+    for (int ty = rc.y; ty < rc.w; ++ty)
+      for (int tx = rc.x; tx < rc.z; ++tx) atomicAdd(&h[ty * gx + tx], 1u);
+  }
+  __syncthreads();
+  uint32_t* out = hist + (size_t)blockIdx.x * tiles;
+  for (int t = threadIdx.x; t < tiles; t += 256) out[t] = h[t];
+}
+
+// hist[c][t] <- start[t] + sum of hist[c'][t] over c' < c: one wavefront per
+// tile, each lane holding up to eight consecutive chunks (n_chunks <= 512),
+// a wave scan of the lanes' sums in between.
+__global__ __launch_bounds__(256) void k_gs_chunk_prefix(int n_chunks, int tiles, const uint32_t* __restrict__ start,
+                                                         uint32_t* __restrict__ hist) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= tiles) return;                      // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const int k = (n_chunks + 63) / 64;
+  uint32_t v[8];
+  uint32_t sum = 0u;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = lane * k + j;
+    v[j] = (j < k && c < n_chunks) ? hist[(size_t)c * tiles + t] : 0u;
+    sum += v[j];
+  }
+  uint32_t run = start[t] + wave_incl_scan(sum) - sum;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = lane * k + j;
+    if (j < k && c < n_chunks) {
+      hist[(size_t)c * tiles + t] = run;
+      run += v[j];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gs_chunk_scatter(int n, int chunk, int tiles, int gx,
+                                                          const uint32_t* __restrict__ touched,
+                                                          const int4* __restrict__ rect,
+                                                          const uint32_t* __restrict__ base,
+                                                          uint32_t* __restrict__ list_u) {
+  __shared__ uint32_t cur[kLdsTiles];
+  for (int t = threadIdx.x; t < tiles; t += 256) cur[t] = 0u;
+  __syncthreads();
+  const uint32_t* off = base + (size_t)blockIdx.x * tiles;
+  const int g0 = blockIdx.x * chunk, g1 = min(n, g0 + chunk);
+  for (int i = g0 + threadIdx.x; i < g1; i += 256) {
+    if (touched[i] == 0u) continue;
+    const int4 rc = rect[i];
+    for (int ty = rc.y; ty < rc.w; ++ty)
+      for (int tx = rc.x; tx < rc.z; ++tx) {
+        const int t = ty * gx + tx;
+        list_u[off[t] + atomicAdd(&cur[t], 1u)] = (uint32_t)i;
+      }
+  }
 }
 
 // Sort each tile's list of (unique) Gaussian ids ascending: bitonic in LDS up
@@ -279,6 +422,18 @@ __global__ __launch_bounds__(256) void k_gs_sort(const uint32_t* __restrict__ st
   if (n == 0u) return;
   if (n == 1u) {
     if (threadIdx.x == 0) list[s] = list_u[s];
+    return;
+  }
+  if (n <= 256u) {
+    // one id per lane; its rank is the number of smaller ids (LDS broadcast reads)
+    key[threadIdx.x] = threadIdx.x < n ? list_u[s + threadIdx.x] : 0xFFFFFFFFu;
+    __syncthreads();
+    if (threadIdx.x < n) {
+      const uint32_t v = key[threadIdx.x];
+      uint32_t r = 0;
+      for (uint32_t j = 0; j < n; ++j) r += key[j] < v ? 1u : 0u;
+      list[s + r] = v;
+    }
     return;
   }
   if (n <= (uint32_t)kSortCap) {
@@ -554,22 +709,28 @@ extern "C" int pty_gs_preprocess(const PtyGs* d, const float* xy, const float* p
   PTY_REQUIRE(d->n == 0 || (xy && p0 && p1 && radii), PTY_EINVAL, "gaussian field: null input");
   const GeomLayout L = geom_layout(d);
   uint32_t* count = at<uint32_t>(geom, L.count);
-  hipLaunchKernelGGL(k_zero_u32, dim3(grid1(L.tiles) < 4096 ? grid1(L.tiles) : 4096), dim3(256), 0, stream,
-                     count, (int64_t)L.tiles);
+  int* diff = at<int>(geom, L.diff);
+  const int64_t n_diff = (int64_t)(L.tiles_x + 1) * (L.tiles_y + 1);
+  hipLaunchKernelGGL(k_zero_u32, dim3(grid1(n_diff) < 4096 ? grid1(n_diff) : 4096), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t*>(diff), n_diff);
   if (d->n > 0) {
     const float smax = (float)d->max_patch_radius, smin = d->min_scale;
     if (d->param == PTY_GS_RS)
       hipLaunchKernelGGL(k_gs_preprocess<PTY_GS_RS>, dim3(grid1(d->n)), dim3(256), 0, stream, d->n, d->width,
                          d->height, xy, p0, p1, smax, smin, L.tiles_x, L.tiles_y, radii, at<float2>(geom, L.mean),
-                         at<float4>(geom, L.conic), at<int4>(geom, L.rect), at<uint32_t>(geom, L.touched), count);
+                         at<float4>(geom, L.conic), at<int4>(geom, L.rect), at<uint32_t>(geom, L.touched), diff);
     else
       hipLaunchKernelGGL(k_gs_preprocess<PTY_GS_CHOLESKY>, dim3(grid1(d->n)), dim3(256), 0, stream, d->n,
                          d->width, d->height, xy, p0, p1, smax, smin, L.tiles_x, L.tiles_y, radii,
                          at<float2>(geom, L.mean), at<float4>(geom, L.conic), at<int4>(geom, L.rect),
-                         at<uint32_t>(geom, L.touched), count);
+                         at<uint32_t>(geom, L.touched), diff);
     rc = check_launch("pty_gs_preprocess");
     if (rc != PTY_OK) return rc;
   }
+  hipLaunchKernelGGL(k_diff_rows, dim3(L.tiles_y + 1), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(diff),
+                     L.tiles_x + 1);
+  hipLaunchKernelGGL(k_diff_cols, dim3(L.tiles_x), dim3(256), 0, stream, reinterpret_cast<const uint32_t*>(diff),
+                     L.tiles_x, L.tiles_y, count);
   unsigned long long* total = at<unsigned long long>(geom, L.total);
   rc = exclusive_scan(at<uint32_t>(geom, L.touched), at<uint32_t>(geom, L.goff), at<uint32_t>(geom, L.gsums),
                       L.nb_g, d->n, total, stream);
@@ -612,10 +773,21 @@ extern "C" int pty_gs_render_fwd(const PtyGs* d, void* geom, int64_t n_rendered,
   uint32_t* list_u = n_rendered ? at<uint32_t>(bin, B.list_u) : nullptr;
   uint32_t* list = n_rendered ? at<uint32_t>(bin, B.list) : nullptr;
   if (n_rendered > 0) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(grid1(L.tiles) < 4096 ? grid1(L.tiles) : 4096), dim3(256), 0, stream,
-                       cursor, (int64_t)L.tiles);
-    hipLaunchKernelGGL(k_gs_scatter, dim3(grid1(d->n)), dim3(256), 0, stream, d->n, L.tiles_x,
-                       at<uint32_t>(geom, L.touched), at<int4>(geom, L.rect), start, cursor, list_u);
+    if (L.n_chunks == 0)
+      hipLaunchKernelGGL(k_zero_u32, dim3(grid1(L.tiles) < 4096 ? grid1(L.tiles) : 4096), dim3(256), 0, stream,
+                         cursor, (int64_t)L.tiles);
+    if (L.n_chunks > 0) {
+      uint32_t* hist = at<uint32_t>(geom, L.chunk);
+      hipLaunchKernelGGL(k_gs_chunk_hist, dim3(L.n_chunks), dim3(256), 0, stream, d->n, L.chunk_size, L.tiles,
+                         L.tiles_x, at<uint32_t>(geom, L.touched), at<int4>(geom, L.rect), hist);
+      hipLaunchKernelGGL(k_gs_chunk_prefix, dim3((L.tiles + 3) / 4), dim3(256), 0, stream, L.n_chunks, L.tiles,
+                         start, hist);
+      hipLaunchKernelGGL(k_gs_chunk_scatter, dim3(L.n_chunks), dim3(256), 0, stream, d->n, L.chunk_size, L.tiles,
+                         L.tiles_x, at<uint32_t>(geom, L.touched), at<int4>(geom, L.rect), hist, list_u);
+    } else {
+      hipLaunchKernelGGL(k_gs_scatter, dim3(grid1(d->n)), dim3(256), 0, stream, d->n, L.tiles_x,
+                         at<uint32_t>(geom, L.touched), at<int4>(geom, L.rect), start, cursor, list_u);
+    }
     hipLaunchKernelGGL(k_gs_sort, dim3(L.tiles), dim3(256), 0, stream, start, count, list_u, list);
   }
   hipLaunchKernelGGL(k_gs_render, dim3(L.tiles), dim3(256), 0, stream, d->width, d->height, L.tiles_x, start,
