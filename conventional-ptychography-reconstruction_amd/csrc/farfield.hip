@@ -11,16 +11,20 @@
 //   dPsi = 2 dI Psi,  dpsi = conj-DFT(dPsi) (= M*N*ifft2),
 //   dprobe += dpsi * conj(patch),  dobject[window] += dpsi * conj(probe_n)
 // Pass structure (LDS Stockham FFTs):
-//   F1 rows  : Psi1[b, r, :] = FFT_N(probe_n[r,:] * patch[r,:])
+//   F1 rows  : Psi1[b, r, :] = FFT_N(probe_n[r,:] * patch[r,:]) (register FFT of
+//              one wavefront for N = 64 .. 512, fft_wave.h; LDS Stockham otherwise)
 //   F2 cols  : FFT_M; smooth-L1 / Poisson: loss, dI, dPsi, IFFT_M in place
-//              FD: store Psi and I, then FD1 (loss, gx, gy), FD2 (dPsi), F2i (IFFT_M)
-//   F3 rows  : per (row block, b): dpsi = IFFT_N(dPsi rows), in place; the
+//              FD: store Psi and I, then one pass per column block: loss,
+//              d(dW) from the circular neighbours (one halo column each
+//              side), dPsi, IFFT_M in place
+//   F3 rows  : dpsi = IFFT_N(dPsi rows), in place (register FFT as F1); the
 //              first row block of each pattern also sums its loss partials
 //   F4       : dprobe = sum_b dpsi_b * conj(patch_b)             (fixed order)
 //   F5       : dobject[px] = sum over windows covering px of dpsi_b * conj(probe_n)
 //              (a per-pixel gather in batch order: no atomics, deterministic)
 //              F4 and F5 share one launch.
 #include "common.h"
+#include "fft_wave.h"
 
 #include <mutex>
 
@@ -35,8 +39,6 @@ struct FarArgs {
   const int* tl;          // [F, 2] (tlY, tlX)
   float2* psi;            // [B, M, N] workspace
   float* inten;           // [B, M, N] (FD / intensity output)
-  float* gx;              // [B, M, N] (FD)
-  float* gy;              // [B, M, N] (FD)
   float* partial;         // [B, P]
   int P;                  // partial sums per pattern
   float* loss_out;        // [B]
@@ -52,6 +54,21 @@ __device__ __forceinline__ bool window_of(const FarArgs& a, int b, int& ty, int&
   ty = a.tl[2 * p];
   tx = a.tl[2 * p + 1];
   return ty >= 0 && tx >= 0 && ty + a.M <= a.H && tx + a.N <= a.W;
+}
+
+// Column block (cb) and pattern (b) of a column-pass workgroup on a 1-D grid
+// of nb * B workgroups.  Workgroups are dispatched round-robin to the 8 XCDs
+// (each with its own L2), so consecutive grid ids land on different XCDs;
+// when nb * B is a multiple of 8 the ids are remapped so that each XCD takes
+// a contiguous run of column blocks of one pattern.  Neighbouring blocks then
+// share their L2 (the other half of every 128-byte row segment, and the halo
+// columns of the FD pass).
+__device__ __forceinline__ void col_block(int nb, int B, int& cb, int& b) {
+  const int total = nb * B;
+  int id = blockIdx.x;
+  if ((total & 7) == 0) id = (id & 7) * (total >> 3) + (id >> 3);
+  b = id / nb;
+  cb = id - b * nb;
 }
 
 // F1: rows.  lines rows of the flattened (b, r) index per workgroup.
@@ -86,6 +103,80 @@ __global__ __launch_bounds__(256) void k_far_rows_fwd(FarArgs a, int lines) {
   }
 }
 
+// F1 for N = 64 E (E in 1, 2, 4, 8): each wavefront transforms L = 8 / E rows
+// per job in registers (fft_wave.h), jobs strided over the grid's
+// wavefronts; the lane's twiddles are loaded once.
+template <int E>
+__global__ __launch_bounds__(256) void k_far_rows_fwd_w(FarArgs a, int jobs) {
+  using F = WaveFft<E>;
+  __shared__ float2 scratch[4][kWaveFftScratch];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float2* s = scratch[wv];
+  F fft;
+  fft.init(lane);
+  const int N = F::N, M = a.M;
+  const int nwaves = gridDim.x * 4;
+  for (int j = blockIdx.x * 4 + wv; j < jobs; j += nwaves) {
+    float2 v[8];
+#pragma unroll
+    for (int l = 0; l < F::L; ++l) {
+      const int row = j * F::L + l;
+      const int b = row >> a.lgM, r = row & (M - 1);
+      int ty = 0, tx = 0, p;
+      const bool valid = window_of(a, b, ty, tx, p);
+      const float2* prow = a.probe + r * N;
+      const float2* orow = a.obj + (int64_t)(ty + r) * a.W + tx;
+#pragma unroll
+      for (int n2 = 0; n2 < E; ++n2) {
+        const int c = lane + 64 * n2;
+        v[l * E + n2] = valid ? cmul(prow[c], orow[c]) : make_float2(0.f, 0.f);
+      }
+    }
+    fft.template run<-1>(v, s, lane);
+    float2* out = a.psi + (int64_t)(j * F::L + F::out_line(lane)) * N + F::out_first(lane);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) out[8 * E * d] = v[d];
+  }
+}
+
+// F3 for N = 64 E: dpsi rows = IFFT_N of the dPsi rows, in place; the job
+// holding a pattern's first row also forms loss_b (as k_far_rows_bwd).
+template <int E>
+__global__ __launch_bounds__(256) void k_far_rows_bwd_w(FarArgs a, int jobs) {
+  using F = WaveFft<E>;
+  __shared__ float2 scratch[4][kWaveFftScratch];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float2* s = scratch[wv];
+  F fft;
+  fft.init(lane);
+  const int N = F::N, M = a.M;
+  const int nwaves = gridDim.x * 4;
+  for (int j = blockIdx.x * 4 + wv; j < jobs; j += nwaves) {
+    const int row0 = j * F::L;            // the job's rows lie in one pattern (L <= M)
+    const int b = row0 >> a.lgM;
+    if ((row0 & (M - 1)) == 0 && a.loss_out) {
+      // the pattern's loss partials, summed across the wavefront in a fixed order
+      float v = 0.f;
+      for (int i = lane; i < a.P; i += 64) v += a.partial[b * a.P + i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) a.loss_out[b] = v / ((float)M * (float)N);
+    }
+    int ty, tx, p;
+    if (!window_of(a, b, ty, tx, p)) continue;   // uniform across the wavefront
+    float2 v[8];
+    const float2* in = a.psi + (int64_t)row0 * N;
+#pragma unroll
+    for (int l = 0; l < F::L; ++l)
+#pragma unroll
+      for (int n2 = 0; n2 < E; ++n2) v[l * E + n2] = in[(int64_t)l * N + lane + 64 * n2];
+    fft.template run<+1>(v, s, lane);
+    float2* out = a.psi + (int64_t)(row0 + F::out_line(lane)) * N + F::out_first(lane);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) out[8 * E * d] = v[d];
+  }
+}
+
 __device__ __forceinline__ float smooth_l1(float d, float& g) {
   const float ad = fabsf(d);
   if (ad < 1.f) { g = d; return 0.5f * d * d; }
@@ -106,12 +197,12 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   return r;
 }
 
-// F2: columns of pattern b = blockIdx.y, C columns per workgroup of
+// F2: columns of pattern b (col_block), C columns per workgroup of
 // kColThreads threads.
 constexpr int kColThreads = 512;
 // MODE 0: pointwise loss (smooth-L1 / Poisson) fused with dPsi and IFFT_M.
-// MODE 1: forward only, store natural-order Psi (in place) and I.
-// MODE 2: inverse only (FD path): IFFT_M of the stored dPsi, in place.
+// MODE 1: forward only, store natural-order Psi (in place) and I (the FD
+//         path, finished by k_far_fd_cols).
 template <int MODE>
 __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   extern __shared__ float2 smem[];
@@ -122,8 +213,9 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   float2* y = x + C * S;
   const int tid = threadIdx.x, nt = blockDim.x;
   fill_twiddles(tw, M, tid, nt);
-  const int b = blockIdx.y;
-  const int c0 = blockIdx.x * C;
+  int cb, b;
+  col_block(N / C, a.B, cb, b);
+  const int c0 = cb * C;
   const int lgC = __builtin_ctz(C);   // C is a power of two
   int ty, tx, p;
   const bool valid = window_of(a, b, ty, tx, p);
@@ -133,14 +225,6 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
     x[c * S + r] = base[(int64_t)r * N + c0 + c];
   }
   __syncthreads();
-  if (MODE == 2) {
-    float2* res = fft_lines<+1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
-    for (int t = tid; t < M * C; t += nt) {
-      const int r = t >> lgC, c = t & (C - 1);
-      base[(int64_t)r * N + c0 + c] = res[c * S + r];
-    }
-    return;
-  }
   float2* f = fft_lines<-1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
   if (MODE == 1) {
     for (int t = tid; t < M * C; t += nt) {
@@ -176,7 +260,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   }
   __syncthreads();
   const float s = block_sum<kColThreads>(acc.value(), red);
-  if (tid == 0) a.partial[b * a.P + blockIdx.x] = s;
+  if (tid == 0) a.partial[b * a.P + cb] = s;
   float2* g2 = (f == x) ? y : x;
   float2* res = fft_lines<+1>(f, g2, M, a.lgM, C, S, tw, 1, tid, nt);
   for (int t = tid; t < M * C; t += nt) {
@@ -185,60 +269,343 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   }
 }
 
-// FD loss (Recovery.py:296-302) on natural-order pixels: the circular
-// neighbours of the shifted array are the circular neighbours here too.
-// FD1: dW = sqrt(m+eps) - sqrt(I+eps); Rx = dW[., x+1] - dW; Ry = dW[y+1, .] - dW;
-//      G = sqrt(Rx^2 + Ry^2 + eps); loss += G; gx = Rx/(G MN); gy = Ry/(G MN).
-__global__ __launch_bounds__(256) void k_far_fd1(FarArgs a) {
-  __shared__ float red[256];
-  const int M = a.M, N = a.N, b = blockIdx.y;
+// FD adjoint at pixel (r, column cc) of a dW block in LDS (column stride S,
+// rows circular with ru = r + 1, rd = r - 1): G, gx, gy at q, at q - x and at
+// q - y, and d(dW)_q = -gx_q - gy_q + gx_{q-x} + gy_{q-y} with
+// g = R / (G M N); G_q is returned for the loss.  1 / G by rsqrt.
+__device__ __forceinline__ float fd_ddw(const float* dW, int S, int cc, int r, int ru, int rd, float inv_mn,
+                                        float& Gq) {
+  const float eps = 1e-8f;
+  const float w = dW[cc * S + r];
+  const float rx = dW[(cc + 1) * S + r] - w, ry = dW[cc * S + ru] - w;
+  const float s0 = rx * rx + ry * ry + eps;
+  const float i0 = rsqrtf(s0);
+  Gq = s0 * i0;
+  const float wl = dW[(cc - 1) * S + r];
+  const float rxl = w - wl, ryl = dW[(cc - 1) * S + ru] - wl;
+  const float il = rsqrtf(rxl * rxl + ryl * ryl + eps);
+  const float wd = dW[cc * S + rd];
+  const float rxd = dW[(cc + 1) * S + rd] - wd, ryd = w - wd;
+  const float id = rsqrtf(rxd * rxd + ryd * ryd + eps);
+  return inv_mn * (-(rx + ry) * i0 + rxl * il + ryd * id);
+}
+
+// Sum over the workgroup (blockDim a multiple of 64) in a fixed order.
+__device__ __forceinline__ float wg_sum(float v, float* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float t = 0.f;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// F2 (MODE 0: smooth-L1 / Poisson loss, dPsi, inverse; MODE 1: store Psi
+// and I) for M = 64 E, as a persistent pipeline over jobs of 8 columns of
+// one pattern (XCD-grouped as col_block): while the wavefronts transform the
+// current job's columns in LDS, the next job's tile (and measurement values)
+// is already being loaded into registers.  Each thread owns the same 8 tile
+// positions in every job (column tid & 7, rows (tid >> 3) + k (64 E / 8)), so
+// the loss phase reads the measurement values it prefetched.  The LDS column
+// stride 584 keeps the tile writes 2-way bank conflicted and leaves room for
+// a wavefront's exchange scratch inside its own column slot.
+constexpr int kColStrideW = 584;
+
+template <int E, int MODE>
+__global__ __launch_bounds__(64 * E) void k_far_cols_wp(FarArgs a, int jobs) {
+  constexpr int C = 8, S = kColStrideW, NT = 64 * E, RSTEP = NT / 8;
+  using Ff = WaveFft<E>;
+  using Fi = WaveFft<E>;
+  __shared__ float2 x[C * S];
+  __shared__ float red[8];
+  const int M = a.M, N = a.N, nb = N / C;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int c = tid & 7, r0 = tid >> 3;
+  const float inv_mn = 1.f / ((float)M * (float)N);
+  Ff fft;
+  fft.init(lane);
+  float2* wscr = x + wv * Ff::L * S;        // the wavefront's first column slot
+  auto job_of = [&](int j, int& cb, int& b) {
+    const int id = (jobs & 7) == 0 ? (j & 7) * (jobs >> 3) + (j >> 3) : j;
+    b = id / nb;
+    cb = id - b * nb;
+  };
+  float2 pf[8];
+  float pm[8];
+  auto prefetch = [&](int j) {
+    if (j >= jobs) return;
+    int cb, b, ty, tx, p;
+    job_of(j, cb, b);
+    const float2* src = a.psi + (int64_t)b * M * N + cb * C + c;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pf[k] = src[(int64_t)(r0 + k * RSTEP) * N];
+    if (MODE == 0) {
+      const bool valid = window_of(a, b, ty, tx, p);
+      const float* ms = a.stack + (int64_t)(valid ? p : 0) * M * N + cb * C + c;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) pm[k] = valid ? ms[(int64_t)(r0 + k * RSTEP) * N] : 0.f;
+    }
+  };
+  prefetch(blockIdx.x);
+  for (int j = blockIdx.x; j < jobs; j += gridDim.x) {
+    int cb, b, ty, tx, p;
+    job_of(j, cb, b);
+    const bool valid = window_of(a, b, ty, tx, p);
+    float mc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      x[c * S + r0 + k * RSTEP] = pf[k];
+      mc[k] = pm[k];
+    }
+    __syncthreads();
+    prefetch(j + gridDim.x);
+    {
+      float2 v[8];
+#pragma unroll
+      for (int l = 0; l < Ff::L; ++l)
+#pragma unroll
+        for (int n2 = 0; n2 < E; ++n2) v[l * E + n2] = x[(wv * Ff::L + l) * S + lane + 64 * n2];
+      fft.template run<-1>(v, wscr, lane);
+      float2* col = x + (wv * Ff::L + Ff::out_line(lane)) * S + Ff::out_first(lane);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) col[8 * E * d] = v[d];
+    }
+    __syncthreads();
+    float2* dst = a.psi + (int64_t)b * M * N + cb * C + c;
+    if (MODE == 1) {
+      float* idst = a.inten + (int64_t)b * M * N + cb * C + c;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float2 v = x[c * S + r0 + k * RSTEP];
+        dst[(int64_t)(r0 + k * RSTEP) * N] = v;
+        idst[(int64_t)(r0 + k * RSTEP) * N] = v.x * v.x + v.y * v.y;
+      }
+      __syncthreads();
+      continue;
+    }
+    KSum acc;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int q = c * S + r0 + k * RSTEP;
+      const float2 v = x[q];
+      float2 gv = make_float2(0.f, 0.f);
+      if (valid) {
+        const float I = v.x * v.x + v.y * v.y;
+        const float m = mc[k];
+        float g;
+        if (a.loss == PTY_LOSS_SMOOTH_L1) {
+          acc.add(smooth_l1(I - m, g));
+        } else {  // Poisson
+          const float e = I + 1e-8f;
+          acc.add(I - m * logf(e));
+          g = 1.f - m / e;
+        }
+        gv = cscale(v, 2.f * g * inv_mn);
+      }
+      x[q] = gv;
+    }
+    __syncthreads();
+    const float sum = wg_sum(acc.value(), red);
+    if (tid == 0) a.partial[b * a.P + cb] = sum;
+    {
+      float2 v[8];
+#pragma unroll
+      for (int l = 0; l < Fi::L; ++l)
+#pragma unroll
+        for (int n2 = 0; n2 < E; ++n2) v[l * E + n2] = x[(wv * Fi::L + l) * S + lane + 64 * n2];
+      fft.template run<+1>(v, wscr, lane);
+      float2* col = x + (wv * Fi::L + Fi::out_line(lane)) * S + Fi::out_first(lane);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) col[8 * E * d] = v[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[(int64_t)(r0 + k * RSTEP) * N] = x[c * S + r0 + k * RSTEP];
+    __syncthreads();
+  }
+}
+
+// The FD loss, its adjoint and the column IFFT for M = 64 E (after the
+// forward column pass stored Psi and I), pipelined like k_far_cols_wp: the
+// next job's I, the measurement (own columns and one halo column each side,
+// circular) and Psi (own columns, kept in registers for dPsi) are loaded
+// while the current job finishes.  Per job: dW = sqrt(m + eps) - sqrt(I + eps)
+// for the C + 2 columns into LDS, then for the own pixels G, d(dW) (fd_ddw),
+// dI, dPsi into the tile, the loss partial, the inverse transforms and the
+// store (in place: the halo columns are read from I, which stays untouched).
+template <int E>
+__global__ __launch_bounds__(64 * E) void k_far_fd_wp(FarArgs a, int jobs) {
+  constexpr int C = 8, S = kColStrideW, NT = 64 * E, RSTEP = NT / 8;
+  using F = WaveFft<E>;
+  __shared__ float2 x[C * S];
+  __shared__ float dW[(C + 2) * (64 * E + 1)];
+  __shared__ float red[8];
+  const int M = a.M, N = a.N, nb = N / C, SW = M + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int c = tid & 7, r0 = tid >> 3;
+  const float eps = 1e-8f;
+  const float inv_mn = 1.f / ((float)M * (float)N);
+  F fft;
+  fft.init(lane);
+  float2* wscr = x + wv * F::L * S;
+  auto job_of = [&](int j, int& cb, int& b) {
+    const int id = (jobs & 7) == 0 ? (j & 7) * (jobs >> 3) + (j >> 3) : j;
+    b = id / nb;
+    cb = id - b * nb;
+  };
+  // prefetched: own-column Psi, I, m; halo I, m (entries tid and tid + NT of
+  // the 2 M halo values: side = t & 1, row = t >> 1)
+  float2 pf[8];
+  float pi[8], pm[8], hi[2], hm[2];
+  auto prefetch = [&](int j) {
+    if (j >= jobs) return;
+    int cb, b, ty, tx, p;
+    job_of(j, cb, b);
+    const bool valid = window_of(a, b, ty, tx, p);
+    const int64_t off = (int64_t)b * M * N;
+    const float* Ib = a.inten + off;
+    const float* ms = a.stack + (int64_t)(valid ? p : 0) * M * N;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int64_t q = (int64_t)(r0 + k * RSTEP) * N + cb * C + c;
+      pf[k] = a.psi[off + q];
+      pi[k] = Ib[q];
+      pm[k] = valid ? ms[q] : 0.f;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = tid + h * NT, side = t & 1, r = t >> 1;
+      const int col = side ? (cb * C + C) & (N - 1) : (cb * C - 1) & (N - 1);
+      const int64_t q = (int64_t)r * N + col;
+      hi[h] = Ib[q];
+      hm[h] = valid ? ms[q] : 0.f;
+    }
+  };
+  prefetch(blockIdx.x);
+  for (int j = blockIdx.x; j < jobs; j += gridDim.x) {
+    int cb, b, ty, tx, p;
+    job_of(j, cb, b);
+    const bool valid = window_of(a, b, ty, tx, p);
+    float2 psi[8];
+    float h2[8];     // 1 / (2 sqrt(I + eps)) of the own pixels
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float si = sqrtf(pi[k] + eps);
+      dW[(c + 1) * SW + r0 + k * RSTEP] = valid ? sqrtf(pm[k] + eps) - si : 0.f;
+      h2[k] = 0.5f / si;
+      psi[k] = pf[k];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = tid + h * NT, side = t & 1, r = t >> 1;
+      dW[(side ? C + 1 : 0) * SW + r] = valid ? sqrtf(hm[h] + eps) - sqrtf(hi[h] + eps) : 0.f;
+    }
+    __syncthreads();
+    prefetch(j + gridDim.x);
+    KSum acc;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int r = r0 + k * RSTEP;
+      float2 g = make_float2(0.f, 0.f);
+      if (valid) {
+        float G;
+        const float ddw = fd_ddw(dW, SW, c + 1, r, (r + 1) & (M - 1), (r - 1) & (M - 1), inv_mn, G);
+        acc.add(G);
+        g = cscale(psi[k], -2.f * ddw * h2[k]);
+      }
+      x[c * S + r] = g;
+    }
+    __syncthreads();
+    const float sum = wg_sum(acc.value(), red);
+    if (tid == 0) a.partial[b * a.P + cb] = sum;
+    {
+      float2 v[8];
+#pragma unroll
+      for (int l = 0; l < F::L; ++l)
+#pragma unroll
+        for (int n2 = 0; n2 < E; ++n2) v[l * E + n2] = x[(wv * F::L + l) * S + lane + 64 * n2];
+      fft.template run<+1>(v, wscr, lane);
+      float2* col = x + (wv * F::L + F::out_line(lane)) * S + F::out_first(lane);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) col[8 * E * d] = v[d];
+    }
+    __syncthreads();
+    float2* dst = a.psi + (int64_t)b * M * N + cb * C + c;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[(int64_t)(r0 + k * RSTEP) * N] = x[c * S + r0 + k * RSTEP];
+    __syncthreads();
+  }
+}
+
+// FD loss and adjoint fused with the column IFFT (Recovery.py:296-302): one
+// workgroup per (C columns, pattern b) after k_far_cols<1> stored Psi and I.
+// Phase A: dW = sqrt(m + eps) - sqrt(I + eps) for the C columns and one halo
+// column on each side (circular), in LDS.  Phase B, own pixels q: G, gx, gy
+// at q, at q - x and at q - y (recomputed from dW, so no gradient arrays),
+// d(dW) = -gx_q - gy_q + gx_{q-x} + gy_{q-y}, dI = -d(dW) / (2 sqrt(I + eps)),
+// dPsi = 2 dI Psi; loss partial = sum of G over own pixels.  Then IFFT_M of
+// the C columns in place.  Psi, I and the measurement are read once and the
+// gradients of the loss never leave the workgroup.
+__global__ __launch_bounds__(kColThreads) void k_far_fd_cols(FarArgs a, int C) {
+  extern __shared__ float2 smem[];
+  __shared__ float red[kColThreads];
+  const int M = a.M, N = a.N, S = M + 1, lgC = __builtin_ctz(C);
+  float2* tw = smem;
+  float2* x = tw + M;
+  float2* y = x + C * S;
+  float* dW = reinterpret_cast<float*>(y);     // (C + 2) x S floats, column cc = c + 1 (halos 0, C + 1)
+  const int tid = threadIdx.x, nt = blockDim.x;
+  fill_twiddles(tw, M, tid, nt);
+  int cb, b;
+  col_block(N / C, a.B, cb, b);
+  const int c0 = cb * C;
   int ty, tx, p;
   const bool valid = window_of(a, b, ty, tx, p);
   const float eps = 1e-8f;
   const float inv_mn = 1.f / ((float)M * (float)N);
-  const float* I = a.inten + (int64_t)b * M * N;
-  const float* meas = a.stack + (int64_t)(valid ? p : 0) * M * N;
-  const int per = (M * N + a.P - 1) / a.P;
-  const int q0 = blockIdx.x * per, q1 = min(M * N, q0 + per);
-  KSum acc;
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-    const int r = q >> a.lgN, c = q & (N - 1);
-    float gxv = 0.f, gyv = 0.f;
-    if (valid) {
-      const int qx = r * N + ((c + 1) & (N - 1));
-      const int qy = ((r + 1) & (M - 1)) * N + c;
-      const float w0 = sqrtf(meas[q] + eps) - sqrtf(I[q] + eps);
-      const float wx = sqrtf(meas[qx] + eps) - sqrtf(I[qx] + eps);
-      const float wy = sqrtf(meas[qy] + eps) - sqrtf(I[qy] + eps);
-      const float rx = wx - w0, ry = wy - w0;
-      const float G = sqrtf(rx * rx + ry * ry + eps);
-      acc.add(G);
-      const float s = inv_mn / G;
-      gxv = rx * s;
-      gyv = ry * s;
-    }
-    a.gx[(int64_t)b * M * N + q] = gxv;
-    a.gy[(int64_t)b * M * N + q] = gyv;
-  }
-  const float s = block_sum<256>(acc.value(), red);
-  if (threadIdx.x == 0) a.partial[b * a.P + blockIdx.x] = s;
-}
-
-// FD2: d(dW)_q = -gx_q - gy_q + gx_{q-x} + gy_{q-y}; dI = -d(dW) / (2 sqrt(I+eps));
-//      dPsi = 2 dI Psi (in place over Psi).
-__global__ __launch_bounds__(256) void k_far_fd2(FarArgs a) {
-  const int M = a.M, N = a.N, b = blockIdx.y;
-  const float eps = 1e-8f;
   const int64_t off = (int64_t)b * M * N;
-  const int per = (M * N + a.P - 1) / a.P;
-  const int q0 = blockIdx.x * per, q1 = min(M * N, q0 + per);
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-    const int r = q >> a.lgN, c = q & (N - 1);
-    const int qxm = r * N + ((c - 1) & (N - 1));
-    const int qym = ((r - 1) & (M - 1)) * N + c;
-    const float ddw = -a.gx[off + q] - a.gy[off + q] + a.gx[off + qxm] + a.gy[off + qym];
-    const float dI = -ddw / (2.f * sqrtf(a.inten[off + q] + eps));
-    a.psi[off + q] = cscale(a.psi[off + q], 2.f * dI);
+  const float* I = a.inten + off;
+  const float* meas = a.stack + (int64_t)(valid ? p : 0) * M * N;
+  float2* base = a.psi + off;
+  // phase A: own columns (coalesced over c), keeping sqrt(I + eps) in x
+  for (int t = tid; t < M * C; t += nt) {
+    const int r = t >> lgC, c = t & (C - 1);
+    const int64_t q = (int64_t)r * N + c0 + c;
+    const float si = sqrtf(I[q] + eps);
+    dW[(c + 1) * S + r] = valid ? sqrtf(meas[q] + eps) - si : 0.f;
+    x[c * S + r] = make_float2(0.5f / si, 0.f);
+  }
+  for (int t = tid; t < 2 * M; t += nt) {
+    const int r = t >> 1, side = t & 1;
+    const int col = side ? (c0 + C) & (N - 1) : (c0 - 1) & (N - 1);
+    const int64_t q = (int64_t)r * N + col;
+    dW[(side ? C + 1 : 0) * S + r] = valid ? sqrtf(meas[q] + eps) - sqrtf(I[q] + eps) : 0.f;
+  }
+  __syncthreads();
+  KSum acc;
+  for (int t = tid; t < M * C; t += nt) {
+    const int r = t >> lgC, c = t & (C - 1);
+    const int cc = c + 1, ru = (r + 1) & (M - 1), rd = (r - 1) & (M - 1);
+    float2 g = make_float2(0.f, 0.f);
+    if (valid) {
+      float G;
+      const float ddw = fd_ddw(dW, S, cc, r, ru, rd, inv_mn, G);
+      acc.add(G);
+      const float dI = -ddw * x[c * S + r].x;      // x holds 1 / (2 sqrt(I + eps))
+      g = cscale(base[(int64_t)r * N + c0 + c], 2.f * dI);
+    }
+    x[c * S + r] = g;
+  }
+  __syncthreads();
+  const float sum = block_sum<kColThreads>(acc.value(), red);
+  if (tid == 0) a.partial[b * a.P + cb] = sum;
+  float2* res = fft_lines<+1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
+  for (int t = tid; t < M * C; t += nt) {
+    const int r = t >> lgC, c = t & (C - 1);
+    base[(int64_t)r * N + c0 + c] = res[c * S + r];
   }
 }
 
@@ -379,7 +746,7 @@ static int check_far(const PtyFar* d) {
 }
 
 struct FarLayout {
-  size_t psi, inten, gx, gy, partial, total;
+  size_t psi, inten, partial, total;
   int P, C;
 };
 
@@ -391,10 +758,7 @@ static FarLayout far_layout(const PtyFar* d) {
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
   L.psi = take(px * sizeof(float2));
-  const bool fd = d->loss_type == PTY_LOSS_FD;
   L.inten = take(px * sizeof(float));
-  L.gx = fd ? take(px * sizeof(float)) : 0;
-  L.gy = fd ? take(px * sizeof(float)) : 0;
   L.partial = take((size_t)d->n_batch * L.P * sizeof(float));
   L.total = off;
   return L;
@@ -411,8 +775,6 @@ static FarArgs far_args(const PtyFar* d, const FarLayout& L, const float* object
   char* w = (char*)work;
   a.psi = (float2*)(w + L.psi);
   a.inten = (float*)(w + L.inten);
-  a.gx = d->loss_type == PTY_LOSS_FD ? (float*)(w + L.gx) : nullptr;
-  a.gy = d->loss_type == PTY_LOSS_FD ? (float*)(w + L.gy) : nullptr;
   a.partial = (float*)(w + L.partial);
   a.P = L.P;
   a.loss_out = loss; a.dobj = (float2*)dobject; a.dprobe = (float2*)dprobe;
@@ -426,17 +788,70 @@ static int far_row_lines(int N, int M) {
   return lines < M ? lines : M;
 }
 
+// Wavefronts of the register-FFT row passes: jobs strided over at most
+// 16 wavefronts per CU.
+static int far_wave_blocks(int jobs) {
+  const int cap = device_cu_count() * 4;   // 4 workgroups of 4 wavefronts per CU
+  const int need = (jobs + 3) / 4;
+  return need < cap ? need : cap;
+}
+
+static bool far_wave_rows(int N) { return N >= 64 && N <= 512; }
+
 static void launch_rows_fwd(const FarArgs& a, hipStream_t s) {
+  if (far_wave_rows(a.N)) {
+    const int E = a.N / 64, jobs = a.B * a.M / (8 / E);
+    const dim3 g(far_wave_blocks(jobs)), blk(256);
+    switch (E) {
+      case 1: hipLaunchKernelGGL(k_far_rows_fwd_w<1>, g, blk, 0, s, a, jobs); break;
+      case 2: hipLaunchKernelGGL(k_far_rows_fwd_w<2>, g, blk, 0, s, a, jobs); break;
+      case 4: hipLaunchKernelGGL(k_far_rows_fwd_w<4>, g, blk, 0, s, a, jobs); break;
+      default: hipLaunchKernelGGL(k_far_rows_fwd_w<8>, g, blk, 0, s, a, jobs); break;
+    }
+    return;
+  }
   const int lines = far_row_lines(a.N, a.M);
   const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
   const int blocks = (a.B * a.M + lines - 1) / lines;
   hipLaunchKernelGGL(k_far_rows_fwd, dim3(blocks), dim3(256), shm, s, a, lines);
 }
 
+// Column passes on the register FFT (M = 64 .. 512, 8 columns per workgroup:
+// C / L = E wavefronts) when the layout's column block is 8 wide.
+static bool far_wave_cols(const FarArgs& a, int C) { return a.M >= 64 && a.M <= 512 && C == 8; }
+
+// MODE 0: pointwise losses; 1: forward store (Psi, I); 3: FD loss + adjoint + inverse
 template <int MODE>
 static void launch_cols(const FarArgs& a, int C, hipStream_t s) {
+  if (far_wave_cols(a, C)) {
+    const int jobs = (a.N / C) * a.B;
+    const int cap = device_cu_count() * 2;
+    const dim3 g(jobs < cap ? jobs : cap);
+    if (MODE == 3) {
+      switch (a.M) {
+        case 64: hipLaunchKernelGGL(k_far_fd_wp<1>, g, dim3(64), 0, s, a, jobs); break;
+        case 128: hipLaunchKernelGGL(k_far_fd_wp<2>, g, dim3(128), 0, s, a, jobs); break;
+        case 256: hipLaunchKernelGGL(k_far_fd_wp<4>, g, dim3(256), 0, s, a, jobs); break;
+        default: hipLaunchKernelGGL(k_far_fd_wp<8>, g, dim3(512), 0, s, a, jobs); break;
+      }
+      return;
+    }
+    constexpr int M1 = MODE == 1 ? 1 : 0;
+    switch (a.M) {
+      case 64: hipLaunchKernelGGL((k_far_cols_wp<1, M1>), g, dim3(64), 0, s, a, jobs); break;
+      case 128: hipLaunchKernelGGL((k_far_cols_wp<2, M1>), g, dim3(128), 0, s, a, jobs); break;
+      case 256: hipLaunchKernelGGL((k_far_cols_wp<4, M1>), g, dim3(256), 0, s, a, jobs); break;
+      default: hipLaunchKernelGGL((k_far_cols_wp<8, M1>), g, dim3(512), 0, s, a, jobs); break;
+    }
+    return;
+  }
+  if (MODE == 3) {
+    const size_t shm = sizeof(float2) * (size_t)(a.M + 2 * C * (a.M + 1));
+    hipLaunchKernelGGL(k_far_fd_cols, dim3((a.N / C) * a.B), dim3(kColThreads), shm, s, a, C);
+    return;
+  }
   const size_t shm = sizeof(float2) * (size_t)(a.M + 2 * C * (a.M + 1));
-  hipLaunchKernelGGL(k_far_cols<MODE>, dim3(a.N / C, a.B), dim3(kColThreads), shm, s, a, C);
+  hipLaunchKernelGGL(k_far_cols<MODE == 1 ? 1 : 0>, dim3((a.N / C) * a.B), dim3(kColThreads), shm, s, a, C);
 }
 
 }  // namespace pty
@@ -459,15 +874,24 @@ static int far_fwd_bwd_launch(const PtyFar* d, const FarLayout& L, const float* 
   launch_rows_fwd(a, stream);
   if (d->loss_type == PTY_LOSS_FD) {
     launch_cols<1>(a, L.C, stream);
-    hipLaunchKernelGGL(k_far_fd1, dim3(a.P, a.B), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(k_far_fd2, dim3(a.P, a.B), dim3(256), 0, stream, a);
-    launch_cols<2>(a, L.C, stream);
+    launch_cols<3>(a, L.C, stream);
   } else {
     launch_cols<0>(a, L.C, stream);
   }
-  const int lines = far_row_lines(a.N, a.M);
-  const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
-  hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), shm, stream, a, lines);
+  if (far_wave_rows(a.N)) {
+    const int E = a.N / 64, jobs = a.B * a.M / (8 / E);
+    const dim3 g(far_wave_blocks(jobs)), blk(256);
+    switch (E) {
+      case 1: hipLaunchKernelGGL(k_far_rows_bwd_w<1>, g, blk, 0, stream, a, jobs); break;
+      case 2: hipLaunchKernelGGL(k_far_rows_bwd_w<2>, g, blk, 0, stream, a, jobs); break;
+      case 4: hipLaunchKernelGGL(k_far_rows_bwd_w<4>, g, blk, 0, stream, a, jobs); break;
+      default: hipLaunchKernelGGL(k_far_rows_bwd_w<8>, g, blk, 0, stream, a, jobs); break;
+    }
+  } else {
+    const int lines = far_row_lines(a.N, a.M);
+    const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
+    hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), shm, stream, a, lines);
+  }
   const int mn = a.M * a.N;
   const int probe_blocks = (mn + 255) / 256 < 1024 ? (mn + 255) / 256 : 1024;
   const int64_t tiles = (int64_t)((a.H + kTile - 1) / kTile) * ((a.W + kTile - 1) / kTile);

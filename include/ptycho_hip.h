@@ -273,8 +273,8 @@ int pty_farfield_fwd_bwd(const PtyFar* d, const float* object, const float* prob
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
  * workspace is a static device array of PTY_FAR_SCRATCH_BYTES per GPU (held
  * until the process ends); a batch whose pty_farfield_workspace exceeds it is
- * rejected with PTY_EWORK (16 MiB: e.g. batch 12 at 256 x 256 with FD, batch 51
- * at 128 x 128).  The library allocates nothing and the kernels need no
+ * rejected with PTY_EWORK (16 MiB: e.g. batch 21 at 256 x 256, batch 85 at
+ * 128 x 128, any loss).  The library allocates nothing and the kernels need no
  * cleared memory.  Every call on one GPU shares that GPU's scratch: calls
  * that overlap on one GPU (different streams or threads) race.  Same kernels
  * and same bits as pty_farfield_fwd_bwd. */

@@ -200,7 +200,7 @@ def test_probe_normalize_fwd_bwd(pkg, dev):
 
 
 # ---------------------------------------------------------------- far field
-@pytest.mark.parametrize("M", [16, 64, 256, 512])
+@pytest.mark.parametrize("M", [16, 64, 128, 256, 512])
 def test_farfield_intensity(pkg, dev, M):
     rng = np.random.default_rng(M)
     H = 2 * M + 24
@@ -218,7 +218,7 @@ def test_farfield_intensity(pkg, dev, M):
         assert err < 1e-5, (b, float(err))
 
 
-@pytest.mark.parametrize("loss_type,M", [(lt, m) for m in (16, 64) for lt in
+@pytest.mark.parametrize("loss_type,M", [(lt, m) for m in (16, 64, 128, 256) for lt in
                                          ("smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss")]
                          + [("FD_loss", 512), ("smooth_L1_loss", 512)])
 def test_farfield_loss_and_grads(pkg, dev, loss_type, M):

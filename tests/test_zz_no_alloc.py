@@ -52,7 +52,7 @@ def _noalloc(pkg, dev, o, p, st, tl, bidx, d):
 def test_noalloc_farfield_matches_workspace_entry_point(pkg, dev, loss_type):
     # a batch that fills most of the scratch, then a smaller one (smaller
     # probe, canvas and batch): the second call must not see the first's data
-    cases = [(256, 30, 12 if loss_type == "FD_loss" else 20, 640, 1), (64, 9, 4, 200, 2)]
+    cases = [(256, 30, 20, 640, 1), (64, 9, 4, 200, 2)]
     for k, (M, F, B, H, seed) in enumerate(cases):
         o, p, st, tl, bidx, d = _case(pkg, dev, M, F, B, H, loss_type, seed)
         assert pkg._lib.query("pty_farfield_workspace", d) <= FAR_SCRATCH
@@ -67,11 +67,11 @@ def test_noalloc_farfield_matches_workspace_entry_point(pkg, dev, loss_type):
 def test_noalloc_rejects_batches_beyond_the_scratch(pkg):
     """Checked on the host before any device work (no GPU needed)."""
     L = pkg._lib
-    d = pkg.farfield.far_descriptor(256, 256, 1024, 1024, 13, 400, "FD_loss")
+    d = pkg.farfield.far_descriptor(256, 256, 1024, 1024, 22, 400, "FD_loss")
     assert L.query("pty_farfield_workspace", d) > FAR_SCRATCH
     with pytest.raises(RuntimeError, match="more than the 16777216 of the static scratch"):
         L.call("pty_farfield_fwd_bwd_noalloc", d, 1, 1, 1, 1, 1, 1, 1, 1, None)
-    d = pkg.farfield.far_descriptor(256, 256, 1024, 1024, 12, 400, "FD_loss")
+    d = pkg.farfield.far_descriptor(256, 256, 1024, 1024, 21, 400, "FD_loss")
     assert L.query("pty_farfield_workspace", d) <= FAR_SCRATCH
     with pytest.raises(RuntimeError, match="null pointer"):
         L.call("pty_farfield_fwd_bwd_noalloc", d, None, 1, 1, 1, 1, 1, 1, 1, None)
