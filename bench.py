@@ -1189,7 +1189,7 @@ def pmc_table(out_stem, csvs, command):
             factors[c] = true_kib / max(cal)
     order = ["k_inr_fwd16", "k_inr_bwd16", "k_inr_fwd_b16", "k_inr_bwd_b16", "k_table_hashed", "k_table_dense",
              "k_dense_pass1", "k_inr_grad_finish", "k_up2_rows", "k_transpose", "k_far_rows_fwd", "k_far_cols",
-             "k_far_fd1", "k_far_fd2", "k_far_rows_bwd", "k_far_dgrad", "k_hashgrid_fwd"]
+             "k_far_fd", "k_far_rows_bwd", "k_far_dgrad", "k_hashgrid_fwd", "k_gs_"]
     names = [k for o in order for k in sorted(mean) if k.split("::")[-1].startswith(o)]
     counters = sorted({c for d in mean.values() for c in d})
     lines = [f"# Hardware counters per kernel ({', '.join(os.path.basename(os.path.dirname(c)) or c for c in csvs)})",

@@ -441,9 +441,11 @@ __global__ __launch_bounds__(64 * E) void k_far_fd_wp(FarArgs a, int jobs) {
   constexpr int C = 8, S = kColStrideW, NT = 64 * E, RSTEP = NT / 8;
   using F = WaveFft<E>;
   __shared__ float2 x[C * S];
-  __shared__ float dW[(C + 2) * (64 * E + 1)];
+  __shared__ float dW[(C + 2) * (64 * E + 8)];
   __shared__ float red[8];
-  const int M = a.M, N = a.N, nb = N / C, SW = M + 1;
+  // dW column stride M + 8 (= 8 mod 64): a wavefront's 8 columns x 8 rows hit
+  // 64 distinct banks
+  const int M = a.M, N = a.N, nb = N / C, SW = M + 8;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c = tid & 7, r0 = tid >> 3;
   const float eps = 1e-8f;
@@ -493,15 +495,18 @@ __global__ __launch_bounds__(64 * E) void k_far_fd_wp(FarArgs a, int jobs) {
     float h2[8];     // 1 / (2 sqrt(I + eps)) of the own pixels
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float si = sqrtf(pi[k] + eps);
-      dW[(c + 1) * SW + r0 + k * RSTEP] = valid ? sqrtf(pm[k] + eps) - si : 0.f;
-      h2[k] = 0.5f / si;
+      // hardware rsqrt / sqrt (about 1 ulp): sqrt(I + eps) = (I + eps) rsqrt(I + eps)
+      const float ie = pi[k] + eps, ri = __builtin_amdgcn_rsqf(ie);
+      dW[(c + 1) * SW + r0 + k * RSTEP] = valid ? __builtin_amdgcn_sqrtf(pm[k] + eps) - ie * ri : 0.f;
+      h2[k] = 0.5f * ri;
       psi[k] = pf[k];
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int t = tid + h * NT, side = t & 1, r = t >> 1;
-      dW[(side ? C + 1 : 0) * SW + r] = valid ? sqrtf(hm[h] + eps) - sqrtf(hi[h] + eps) : 0.f;
+      const float ie = hi[h] + eps;   // the same expression as the owning block's
+      dW[(side ? C + 1 : 0) * SW + r] =
+          valid ? __builtin_amdgcn_sqrtf(hm[h] + eps) - ie * __builtin_amdgcn_rsqf(ie) : 0.f;
     }
     __syncthreads();
     prefetch(j + gridDim.x);
@@ -789,9 +794,9 @@ static int far_row_lines(int N, int M) {
 }
 
 // Wavefronts of the register-FFT row passes: jobs strided over at most
-// 16 wavefronts per CU.
+// 32 wavefronts per CU.
 static int far_wave_blocks(int jobs) {
-  const int cap = device_cu_count() * 4;   // 4 workgroups of 4 wavefronts per CU
+  const int cap = device_cu_count() * 8;   // 8 workgroups of 4 wavefronts per CU
   const int need = (jobs + 3) / 4;
   return need < cap ? need : cap;
 }
@@ -825,7 +830,9 @@ template <int MODE>
 static void launch_cols(const FarArgs& a, int C, hipStream_t s) {
   if (far_wave_cols(a, C)) {
     const int jobs = (a.N / C) * a.B;
-    const int cap = device_cu_count() * 2;
+    // resident workgroups per CU: LDS allows 4 of the pointwise-loss / store
+    // passes (37 KB) and 2 of the FD pass (58 KB)
+    const int cap = device_cu_count() * (MODE == 3 ? 2 : 4);
     const dim3 g(jobs < cap ? jobs : cap);
     if (MODE == 3) {
       switch (a.M) {
