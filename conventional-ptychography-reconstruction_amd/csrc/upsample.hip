@@ -30,6 +30,7 @@
 // Two n-point transforms per line instead of an n- and an m-point one, and
 // line buffers of n instead of m (k_up2_rows).
 #include "common.h"
+#include "fft_wave.h"
 
 namespace pty {
 
@@ -202,6 +203,81 @@ int make_fft_plan(int n, FftPlan& P) {
   return r == 1 ? PTY_OK : PTY_EUNSUP;
 }
 
+// k_up2_rows for n = 64 E (E in 1, 2, 4, 8; canvases 128 .. 1024 at ds = 2):
+// each wavefront takes L = 8 / E rows per job in registers (fft_wave.h),
+// jobs strided over the grid.  Forward transform, the half-sample phase of
+// each frequency, inverse transform; for E < 8 the wavefront's LDS slice
+// brings the transform outputs back to the input order (lane + 64 n2) before
+// the phase-and-inverse and before the store (for E = 8 the two orders are
+// the same).  Output and scales as k_up2_rows.
+template <int E>
+__device__ __forceinline__ void wave_reorder(float2 v[8], float2* s, int lane) {
+  using F = WaveFft<E>;
+  if (E == 8) return;
+  const int line = F::out_line(lane), first = F::out_first(lane);
+#pragma unroll
+  for (int d = 0; d < 8; ++d) s[line * F::N + first + 8 * E * d] = v[d];
+  wave_lds_sync();
+#pragma unroll
+  for (int l = 0; l < F::L; ++l)
+#pragma unroll
+    for (int n2 = 0; n2 < E; ++n2) v[l * E + n2] = s[l * F::N + lane + 64 * n2];
+  wave_lds_sync();
+}
+
+template <int E, int MODE>
+__global__ __launch_bounds__(256) void k_up2_rows_w(const float2* __restrict__ in, float2* __restrict__ out,
+                                                    int rows, float scale) {
+  using F = WaveFft<E>;
+  constexpr int n = F::N, m = 2 * n, h = n / 2;
+  __shared__ float2 scratch[4][kWaveFftScratch];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float2* s = scratch[wv];
+  F fft;
+  fft.init(lane);
+  const float ns = (float)n * scale;
+  const int jobs = rows / F::L;
+  for (int j = blockIdx.x * 4 + wv; j < jobs; j += gridDim.x * 4) {
+    const int r0 = j * F::L;
+    float2 v[8];
+#pragma unroll
+    for (int l = 0; l < F::L; ++l)
+#pragma unroll
+      for (int n2 = 0; n2 < E; ++n2) {
+        const int k = lane + 64 * n2;
+        v[l * E + n2] = (MODE == 0) ? in[(int64_t)(r0 + l) * n + k] : in[(int64_t)(r0 + l) * m + 2 * k + 1];
+      }
+    fft.template run<-1>(v, s, lane);
+    wave_reorder<E>(v, s, lane);
+#pragma unroll
+    for (int l = 0; l < F::L; ++l)
+#pragma unroll
+      for (int n2 = 0; n2 < E; ++n2) {
+        const int k = lane + 64 * n2;
+        const int sk = k < h ? k : k - n;                         // signed frequency
+        const float2 w = tw1024<-1>(((sk + m) & (m - 1)) * (1024 / m));   // exp(-2 pi i s / m)
+        v[l * E + n2] = (MODE == 0) ? cmulc(v[l * E + n2], w) : cmul(v[l * E + n2], w);
+      }
+    fft.template run<+1>(v, s, lane);
+    wave_reorder<E>(v, s, lane);
+#pragma unroll
+    for (int l = 0; l < F::L; ++l)
+#pragma unroll
+      for (int n2 = 0; n2 < E; ++n2) {
+        const int k = lane + 64 * n2;
+        const int64_t r = r0 + l;
+        const float2 res = v[l * E + n2];
+        if (MODE == 0) {
+          const float2 x = in[r * n + k];
+          reinterpret_cast<float4*>(out + r * m)[k] = make_float4(x.x * ns, x.y * ns, res.x * scale, res.y * scale);
+        } else {
+          const float2 e = in[r * m + 2 * k];
+          out[r * n + k] = make_float2(e.x * ns + res.x * scale, e.y * ns + res.y * scale);
+        }
+      }
+  }
+}
+
 struct UpLaunch {
   int lines;     // lines per workgroup of the row stages
   bool lds_tw;   // twiddle table in LDS (else the global table)
@@ -258,6 +334,18 @@ template <int MODE>
 static void launch_up2_rows(const float2* in, float2* out, const FftPlan& Pn, int rows, float scale,
                             const UpLaunch& u, const float2* gtw, hipStream_t s) {
   const int n = Pn.n;
+  if (n >= 64 && n <= 512 && Pn.pow2) {
+    const int E = n / 64, jobs = rows / (8 / E);
+    const int cap = device_cu_count() * 8, need = (jobs + 3) / 4;
+    const dim3 g(need < cap ? need : cap), blk(256);
+    switch (E) {
+      case 1: hipLaunchKernelGGL((k_up2_rows_w<1, MODE>), g, blk, 0, s, in, out, rows, scale); break;
+      case 2: hipLaunchKernelGGL((k_up2_rows_w<2, MODE>), g, blk, 0, s, in, out, rows, scale); break;
+      case 4: hipLaunchKernelGGL((k_up2_rows_w<4, MODE>), g, blk, 0, s, in, out, rows, scale); break;
+      default: hipLaunchKernelGGL((k_up2_rows_w<8, MODE>), g, blk, 0, s, in, out, rows, scale); break;
+    }
+    return;
+  }
   const size_t shm = sizeof(float2) * (size_t)((u.lds_tw ? 2 * n : 0) + 2 * u.lines * n);
   const int blocks = (rows + u.lines - 1) / u.lines;
   // 128 threads for long lines (n = 1824: 243 us against 255 us with 256)

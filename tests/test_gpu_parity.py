@@ -158,10 +158,11 @@ def test_mlp_forward_from_features(pkg, dev):
 
 
 # ---------------------------------------------------------------- upsample
-@pytest.mark.parametrize("n,ds", [(16, 2), (64, 2), (128, 2), (512, 2), (5, 2), (7, 3), (57, 2), (114, 2),
-                                  (513, 2), (912, 2), (1824, 2)])
+@pytest.mark.parametrize("n,ds", [(16, 2), (32, 2), (64, 2), (128, 2), (256, 2), (512, 2), (1024, 2), (64, 3),
+                                  (5, 2), (7, 3), (57, 2), (114, 2), (513, 2), (912, 2), (1824, 2)])
 def test_upsample_fwd_adj(pkg, dev, n, ds):
-    """Power-of-two and mixed-radix sizes (57 = 3*19, 1824 = 2^5*3*19), odd n (DC on m-1)."""
+    """Power-of-two (64 .. 512 at ds 2 on the wavefront register FFT, the rest on the LDS
+    Stockham kernels) and mixed-radix sizes (57 = 3*19, 1824 = 2^5*3*19), odd n (DC on m-1)."""
     torch.manual_seed(n)
     m = n * ds
     x = torch.randn(n, n, dtype=torch.complex64, device=dev).requires_grad_(True)
