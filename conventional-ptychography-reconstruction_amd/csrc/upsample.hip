@@ -278,6 +278,90 @@ __global__ __launch_bounds__(256) void k_up2_rows_w(const float2* __restrict__ i
   }
 }
 
+// The same line transform along the columns of a row-major array, for
+// n = 64 E at ds = 2, so that the upsample needs no transposes: a workgroup of
+// 8 / L wavefronts takes 8 columns (coalesced 64-byte row segments staged in
+// LDS, column stride 584), each wavefront transforms its L columns in
+// registers, and the workgroup writes the rows back.
+//   MODE 0 (forward, columns of `in` [n x n]): out [2n x n], out[2j][c] =
+//          n scale x_j, out[2j+1][c] = scale IFFT(phi FFT(x))_j
+//   MODE 1 (adjoint, columns of `in` [2n x n]): out [n x n], out[j][c] =
+//          n scale in[2j][c] + scale IFFT(conj(phi) FFT(in[2 . + 1][c]))_j
+constexpr int kUpColStride = 584;
+
+template <int E, int MODE>
+__global__ __launch_bounds__(64 * E) void k_up2_cols_w(const float2* __restrict__ in, float2* __restrict__ out,
+                                                       float scale) {
+  using F = WaveFft<E>;
+  constexpr int n = F::N, m = 2 * n, h = n / 2, S = kUpColStride, NT = 64 * E;
+  __shared__ float2 xt[8 * S];   // the columns to transform
+  __shared__ float2 rt[8 * S];   // MODE 0: results; MODE 1: the even rows
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int c0 = blockIdx.x * 8, c = tid & 7;
+  const float ns = (float)n * scale;
+  for (int t = tid; t < 8 * n; t += NT) {
+    const int j = t >> 3;
+    if (MODE == 0) {
+      xt[c * S + j] = in[(int64_t)j * n + c0 + c];
+    } else {
+      xt[c * S + j] = in[(int64_t)(2 * j + 1) * n + c0 + c];
+      rt[c * S + j] = in[(int64_t)(2 * j) * n + c0 + c];
+    }
+  }
+  __syncthreads();
+  F fft;
+  fft.init(lane);
+  float2 v[8];
+#pragma unroll
+  for (int l = 0; l < F::L; ++l)
+#pragma unroll
+    for (int n2 = 0; n2 < E; ++n2) v[l * E + n2] = xt[(wv * F::L + l) * S + lane + 64 * n2];
+  float2* scr = (MODE == 0 ? rt : xt) + wv * F::L * S;   // a slot no other wavefront reads now
+  fft.template run<-1>(v, scr, lane);
+  wave_reorder<E>(v, scr, lane);
+#pragma unroll
+  for (int l = 0; l < F::L; ++l)
+#pragma unroll
+    for (int n2 = 0; n2 < E; ++n2) {
+      const int k = lane + 64 * n2;
+      const int sk = k < h ? k : k - n;
+      const float2 w = tw1024<-1>(((sk + m) & (m - 1)) * (1024 / m));
+      v[l * E + n2] = (MODE == 0) ? cmulc(v[l * E + n2], w) : cmul(v[l * E + n2], w);
+    }
+  fft.template run<+1>(v, scr, lane);
+  wave_reorder<E>(v, scr, lane);
+  float2* dst = (MODE == 0 ? rt : xt) + wv * F::L * S;
+#pragma unroll
+  for (int l = 0; l < F::L; ++l)
+#pragma unroll
+    for (int n2 = 0; n2 < E; ++n2) dst[l * S + lane + 64 * n2] = v[l * E + n2];
+  __syncthreads();
+  for (int t = tid; t < 8 * n; t += NT) {
+    const int j = t >> 3;
+    const float2 x = xt[c * S + j], r = rt[c * S + j];
+    if (MODE == 0) {
+      out[(int64_t)(2 * j) * n + c0 + c] = make_float2(x.x * ns, x.y * ns);
+      out[(int64_t)(2 * j + 1) * n + c0 + c] = make_float2(r.x * scale, r.y * scale);
+    } else {
+      out[(int64_t)j * n + c0 + c] = make_float2(r.x * ns + x.x * scale, r.y * ns + x.y * scale);
+    }
+  }
+}
+
+template <int MODE>
+static void launch_up2_cols(const float2* in, float2* out, int n, float scale, hipStream_t s) {
+  const dim3 g(n / 8);
+  switch (n) {
+    case 64: hipLaunchKernelGGL((k_up2_cols_w<1, MODE>), g, dim3(64), 0, s, in, out, scale); break;
+    case 128: hipLaunchKernelGGL((k_up2_cols_w<2, MODE>), g, dim3(128), 0, s, in, out, scale); break;
+    case 256: hipLaunchKernelGGL((k_up2_cols_w<4, MODE>), g, dim3(256), 0, s, in, out, scale); break;
+    default: hipLaunchKernelGGL((k_up2_cols_w<8, MODE>), g, dim3(512), 0, s, in, out, scale); break;
+  }
+}
+
+// power-of-two n = 64 .. 512 at ds = 2: register-FFT row and column stages
+static bool up_wave(int n, int ds) { return ds == 2 && n >= 64 && n <= 512 && is_pow2(n); }
+
 struct UpLaunch {
   int lines;     // lines per workgroup of the row stages
   bool lds_tw;   // twiddle table in LDS (else the global table)
@@ -407,6 +491,12 @@ extern "C" int pty_upsample_fwd(const float* small, int32_t n, int32_t ds, float
   PTY_REQUIRE(work && work_bytes >= W.total, PTY_EWORK, "upsample_fwd: workspace too small");
   const int m = n * ds;
   char* w = (char*)work;
+  if (up_wave(n, ds)) {
+    float2* Y = (float2*)(w + W.y);     // U along columns   [m, n]
+    launch_up2_cols<0>((const float2*)small, Y, n, 1.f, stream);
+    up_stage<0>(Y, (float2*)big, Pn, Pm, m, 1.f / ((float)n * (float)n), nullptr, stream);
+    return check_launch("upsample_fwd");
+  }
   float2* sT = (float2*)(w + W.t_nn);   // small^T        [n, n]
   float2* YT = (float2*)(w + W.yt);     // U along columns, transposed [n, m]
   float2* Y = (float2*)(w + W.y);       // [m, n]
@@ -429,6 +519,12 @@ extern "C" int pty_upsample_adj(const float* gbig, int32_t n, int32_t ds, float*
   PTY_REQUIRE(work && work_bytes >= W.total, PTY_EWORK, "upsample_adj: workspace too small");
   const int m = n * ds;
   char* w = (char*)work;
+  if (up_wave(n, ds)) {
+    float2* Wr = (float2*)(w + W.y);    // V along rows      [m, n]
+    up_stage<1>((const float2*)gbig, Wr, Pn, Pm, m, 1.f, nullptr, stream);
+    launch_up2_cols<1>(Wr, (float2*)gsmall, n, 1.f / ((float)n * (float)n), stream);
+    return check_launch("upsample_adj");
+  }
   float2* Wr = (float2*)(w + W.y);      // V along rows            [m, n]
   float2* WT = (float2*)(w + W.yt);     // its transpose           [n, m]
   float2* gT = (float2*)(w + W.t_nn);   // gsmall^T                [n, n]
