@@ -244,3 +244,24 @@ def test_full_size_properties(pkg, dev):
     b = run_gpu(pkg, dev, "cholesky", xy, p0, p1, (w + w2).astype(np.float32), H, W, smax, smin)["out"]
     np.testing.assert_allclose(b, gpu["out"] + a, rtol=0, atol=2e-5 * np.abs(b).max())
     print(f"full size: pairs {gpu['R']} ({gpu['R'] / n:.2f} tiles per Gaussian)")
+
+
+def test_large_canvas_uses_the_global_scatter(pkg, dev):
+    """A canvas of more than 16384 tiles (2112^2 = 132^2 tiles) bins through
+    global slot counters instead of the per-chunk LDS counters; the canvas
+    and the gradients still match the oracle."""
+    n, W, H, smax, smin = 3000, 2112, 2112, 16, 1.0
+    xy, p0, p1, w = scene(n, W, H, 31, "cholesky", smin, smax)
+    rng = np.random.default_rng(32)
+    dout = rng.normal(size=(2, H, W)).astype(np.float32)
+    gpu = run_gpu(pkg, dev, "cholesky", xy, p0, p1, w, H, W, smax, smin, dout)
+    ref = go.rasterize("cholesky", xy, p0, p1, w, H, W, smax, smin, dout=dout.astype(np.float64),
+                       radii=gpu["radii"])
+    assert go.tile_grid(W, H)[0] * go.tile_grid(W, H)[1] > 16384
+    assert gpu["R"] == ref["num_rendered"]
+    mag = go.render(ref["geom"], np.abs(w), ref["lists"])
+    check_canvas(gpu["out"], ref["out"], np.abs(mag).sum(0, keepdims=True) * np.ones((2, 1, 1)))
+    check_grad("dweight", gpu["dweight"], ref["dweight"])
+    check_grad("dxy", gpu["dxy"], ref["dxy"])
+    check_grad("dlog_L_diag", gpu["dp0"], ref["dp0"])
+    check_grad("dL_offdiag", gpu["dp1"], ref["dp1"].reshape(-1))
