@@ -298,6 +298,9 @@ struct FftPlan {
   int radix[kMaxPasses];
 };
 
+// Host: the radix plan of n (PTY_EUNSUP when n has a prime factor above kMaxRadix).
+int make_fft_plan(int n, FftPlan& P);
+
 // One Stockham pass of prime radix R (5 <= R <= 31) over `lines` lines: each
 // thread loads its R inputs into registers, applies the stage twiddles and
 // forms the R-point DFT from the conjugate pairs a_r = v_r + v_{R-r},

@@ -10,9 +10,12 @@
 // Adjoint of sum_b loss_b:
 //   dPsi = 2 dI Psi,  dpsi = conj-DFT(dPsi) (= M*N*ifft2),
 //   dprobe += dpsi * conj(patch),  dobject[window] += dpsi * conj(probe_n)
-// Pass structure (LDS Stockham FFTs):
+// Any M x N whose prime factors are at most 64 (mixed-radix LDS Stockham
+// FFTs, common.h); powers of two take the register / radix-8 paths.
+// Pass structure:
 //   F1 rows  : Psi1[b, r, :] = FFT_N(probe_n[r,:] * patch[r,:]) (register FFT of
-//              one wavefront for N = 64 .. 512, fft_wave.h; LDS Stockham otherwise)
+//              one wavefront for M, N powers of two and N = 64 .. 512, fft_wave.h;
+//              LDS Stockham otherwise)
 //   F2 cols  : FFT_M; smooth-L1 / Poisson: loss, dI, dPsi, IFFT_M in place
 //              FD: store Psi and I, then one pass per column block: loss,
 //              d(dW) from the circular neighbours (one halo column each
@@ -32,6 +35,7 @@ namespace pty {
 
 struct FarArgs {
   int M, N, lgM, lgN, H, W, B, F, loss;
+  FftPlan pm, pn;         // radix plans of M and N
   const float2* obj;
   const float2* probe;
   const float* stack;     // [F, M, N] natural order
@@ -82,9 +86,8 @@ __global__ __launch_bounds__(256) void k_far_rows_fwd(FarArgs a, int lines) {
   fill_twiddles(tw, N, tid, nt);
   const int row0 = blockIdx.x * lines;
   const int total_rows = a.B * a.M;
-  const int lgN = a.lgN;
   for (int t = tid; t < lines * N; t += nt) {
-    const int li = t >> lgN, c = t & (N - 1);
+    const int li = t / N, c = t - li * N;
     const int row = row0 + li;
     float2 v = make_float2(0.f, 0.f);
     if (row < total_rows) {
@@ -96,10 +99,9 @@ __global__ __launch_bounds__(256) void k_far_rows_fwd(FarArgs a, int lines) {
     x[t] = v;
   }
   __syncthreads();
-  float2* res = fft_lines<-1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
+  float2* res = fft_any<-1>(x, y, a.pn, lines, N, tw, 1, tid, nt);
   for (int t = tid; t < lines * N; t += nt) {
-    const int row = row0 + (t >> lgN);
-    if (row < total_rows) a.psi[(int64_t)row * N + (t & (N - 1))] = res[t];
+    if (t < (total_rows - row0) * N) a.psi[(int64_t)row0 * N + t] = res[t];
   }
 }
 
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
     x[c * S + r] = base[(int64_t)r * N + c0 + c];
   }
   __syncthreads();
-  float2* f = fft_lines<-1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
+  float2* f = fft_any<-1>(x, y, a.pm, C, S, tw, 1, tid, nt);
   if (MODE == 1) {
     for (int t = tid; t < M * C; t += nt) {
       const int r = t >> lgC, c = t & (C - 1);
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_cols(FarArgs a, int C) {
   const float s = block_sum<kColThreads>(acc.value(), red);
   if (tid == 0) a.partial[b * a.P + cb] = s;
   float2* g2 = (f == x) ? y : x;
-  float2* res = fft_lines<+1>(f, g2, M, a.lgM, C, S, tw, 1, tid, nt);
+  float2* res = fft_any<+1>(f, g2, a.pm, C, S, tw, 1, tid, nt);
   for (int t = tid; t < M * C; t += nt) {
     const int r = t >> lgC, c = t & (C - 1);
     base[(int64_t)r * N + c0 + c] = res[c * S + r];
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(64 * E) void k_far_fd_wp(FarArgs a, int jobs) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int t = tid + h * NT, side = t & 1, r = t >> 1;
-      const int col = side ? (cb * C + C) & (N - 1) : (cb * C - 1) & (N - 1);
+      const int col = side ? (cb * C + C == N ? 0 : cb * C + C) : (cb == 0 ? N - 1 : cb * C - 1);
       const int64_t q = (int64_t)r * N + col;
       hi[h] = Ib[q];
       hm[h] = valid ? ms[q] : 0.f;
@@ -561,7 +563,8 @@ __global__ __launch_bounds__(kColThreads) void k_far_fd_cols(FarArgs a, int C) {
   float2* tw = smem;
   float2* x = tw + M;
   float2* y = x + C * S;
-  float* dW = reinterpret_cast<float*>(y);     // (C + 2) x S floats, column cc = c + 1 (halos 0, C + 1)
+  float* dW = reinterpret_cast<float*>(y);     // (C + 2) x S floats, column cc = c + 1 (halos 0, C + 1);
+                                               // the launch sizes y for both uses
   const int tid = threadIdx.x, nt = blockDim.x;
   fill_twiddles(tw, M, tid, nt);
   int cb, b;
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_fd_cols(FarArgs a, int C) {
   }
   for (int t = tid; t < 2 * M; t += nt) {
     const int r = t >> 1, side = t & 1;
-    const int col = side ? (c0 + C) & (N - 1) : (c0 - 1) & (N - 1);
+    const int col = side ? (c0 + C == N ? 0 : c0 + C) : (c0 == 0 ? N - 1 : c0 - 1);
     const int64_t q = (int64_t)r * N + col;
     dW[(side ? C + 1 : 0) * S + r] = valid ? sqrtf(meas[q] + eps) - sqrtf(I[q] + eps) : 0.f;
   }
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_fd_cols(FarArgs a, int C) {
   KSum acc;
   for (int t = tid; t < M * C; t += nt) {
     const int r = t >> lgC, c = t & (C - 1);
-    const int cc = c + 1, ru = (r + 1) & (M - 1), rd = (r - 1) & (M - 1);
+    const int cc = c + 1, ru = r + 1 == M ? 0 : r + 1, rd = r == 0 ? M - 1 : r - 1;
     float2 g = make_float2(0.f, 0.f);
     if (valid) {
       float G;
@@ -607,7 +610,7 @@ __global__ __launch_bounds__(kColThreads) void k_far_fd_cols(FarArgs a, int C) {
   __syncthreads();
   const float sum = block_sum<kColThreads>(acc.value(), red);
   if (tid == 0) a.partial[b * a.P + cb] = sum;
-  float2* res = fft_lines<+1>(x, y, M, a.lgM, C, S, tw, 1, tid, nt);
+  float2* res = fft_any<+1>(x, y, a.pm, C, S, tw, 1, tid, nt);
   for (int t = tid; t < M * C; t += nt) {
     const int r = t >> lgC, c = t & (C - 1);
     base[(int64_t)r * N + c0 + c] = res[c * S + r];
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   // the lines of the block are contiguous rows: base + r0 N + t
   for (int t = tid; t < lines * N; t += nt) x[t] = base[(int64_t)r0 * N + t];
   __syncthreads();
-  float2* res = fft_lines<+1>(x, y, N, a.lgN, lines, N, tw, 1, tid, nt);
+  float2* res = fft_any<+1>(x, y, a.pn, lines, N, tw, 1, tid, nt);
   for (int t = tid; t < lines * N; t += nt) base[(int64_t)r0 * N + t] = res[t];
 }
 
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
 __device__ __forceinline__ void far_dprobe(const FarArgs& a, int blk, int nblk) {
   const int MN = a.M * a.N;
   for (int q = blk * 256 + (int)threadIdx.x; q < MN; q += nblk * 256) {
-    const int r = q >> a.lgN, c = q & (a.N - 1);
+    const int r = q / a.N, c = q - r * a.N;
     float2 acc = make_float2(0.f, 0.f);
     for (int b = 0; b < a.B; ++b) {
       int ty, tx, p;
@@ -731,19 +734,43 @@ __global__ __launch_bounds__(256) void k_far_dgrad(FarArgs a, int n_probe_blocks
   else far_dobject(a, blockIdx.x - n_probe_blocks);
 }
 
+// Dynamic LDS of the LDS-FFT column passes (k_far_cols, k_far_fd_cols):
+// twiddles, then two C x (M + 1) buffers; the FD pass keeps (C + 2) x (M + 1)
+// floats of dW in the second, which needs more room only at C = 1.
+static size_t far_col_lds(int M, int C, bool fd) {
+  const size_t S = (size_t)M + 1;
+  const size_t ybuf = (fd && C == 1) ? (3 * S + 1) / 2 : (size_t)C * S;
+  return ((size_t)M + (size_t)C * S + ybuf) * sizeof(float2);
+}
+// Dynamic LDS of the LDS-FFT row passes (k_far_rows_fwd, k_far_rows_bwd).
+static size_t far_row_lds(int N, int lines) { return sizeof(float2) * (size_t)(N + 2 * lines * N); }
+constexpr size_t kFarMaxLds = 144 * 1024;   // of gfx950's 160 KB per workgroup
+
 // Columns per workgroup of the column pass: up to 8 (measured best for
-// probes of 256 and 512), within 80 KB of LDS (two workgroups per CU).
+// probes of 256 and 512), a power of two dividing N, within 80 KB of LDS
+// (two workgroups per CU) where one column allows it.
 static int far_col_tile(int M, int N) {
   int C = 8;
-  while (C > 1 && (size_t)(M + 2 * C * (M + 1)) * sizeof(float2) > 81920) C >>= 1;
+  while (C > 1 && far_col_lds(M, C, true) > 81920) C >>= 1;
   while (C > 1 && N % C) C >>= 1;
   return C;
 }
 
+// Row passes: two rows per workgroup below N = 1024 (measured: 2 beats 4 at
+// N = 256 and beats 4 or 1 at N = 512), one from 1024 on; rows per workgroup
+// divide M.
+static int far_row_lines(int N, int M) { return (N >= 1024 || M % 2) ? 1 : 2; }
+
 static int check_far(const PtyFar* d) {
   PTY_REQUIRE(d, PTY_EINVAL, "farfield: descriptor is NULL");
-  PTY_REQUIRE(is_pow2(d->M) && is_pow2(d->N) && d->M >= 16 && d->N >= 16 && d->M <= 1024 && d->N <= 1024,
-              PTY_EUNSUP, "farfield: probe %dx%d must be powers of two in [16, 1024]", d->M, d->N);
+  PTY_REQUIRE(d->M >= 8 && d->N >= 8 && d->M <= 4096 && d->N <= 4096, PTY_EUNSUP,
+              "farfield: probe %dx%d outside [8, 4096]", d->M, d->N);
+  FftPlan pm, pn;
+  PTY_REQUIRE(make_fft_plan(d->M, pm) == PTY_OK && make_fft_plan(d->N, pn) == PTY_OK, PTY_EUNSUP,
+              "farfield: probe %dx%d has a prime factor above %d", d->M, d->N, kMaxRadix);
+  PTY_REQUIRE(far_col_lds(d->M, far_col_tile(d->M, d->N), true) <= kFarMaxLds &&
+                  far_row_lds(d->N, far_row_lines(d->N, d->M)) <= kFarMaxLds,
+              PTY_EUNSUP, "farfield: probe %dx%d needs more LDS than a workgroup has", d->M, d->N);
   PTY_REQUIRE(d->canvas_h >= d->M && d->canvas_w >= d->N, PTY_EINVAL, "farfield: canvas smaller than probe");
   PTY_REQUIRE(d->n_batch >= 1 && d->n_frames >= 1, PTY_EINVAL, "farfield: empty batch or stack");
   PTY_REQUIRE(d->loss_type >= 0 && d->loss_type <= 2, PTY_EINVAL, "farfield: bad loss_type %d", d->loss_type);
@@ -774,6 +801,8 @@ static FarArgs far_args(const PtyFar* d, const FarLayout& L, const float* object
                         float* dprobe, void* work) {
   FarArgs a;
   a.M = d->M; a.N = d->N; a.lgM = ilog2(d->M); a.lgN = ilog2(d->N);
+  make_fft_plan(d->M, a.pm);   // checked by check_far
+  make_fft_plan(d->N, a.pn);
   a.H = d->canvas_h; a.W = d->canvas_w; a.B = d->n_batch; a.F = d->n_frames; a.loss = d->loss_type;
   a.obj = (const float2*)object; a.probe = (const float2*)probe_n; a.stack = stack;
   a.bidx = bidx; a.tl = tl;
@@ -786,13 +815,6 @@ static FarArgs far_args(const PtyFar* d, const FarLayout& L, const float* object
   return a;
 }
 
-// Row passes: two rows per workgroup below N = 1024 (measured: 2 beats 4 at
-// N = 256 and beats 4 or 1 at N = 512), one from 1024 on.
-static int far_row_lines(int N, int M) {
-  const int lines = N >= 1024 ? 1 : 2;
-  return lines < M ? lines : M;
-}
-
 // Wavefronts of the register-FFT row passes: jobs strided over at most
 // 32 wavefronts per CU.
 static int far_wave_blocks(int jobs) {
@@ -801,10 +823,11 @@ static int far_wave_blocks(int jobs) {
   return need < cap ? need : cap;
 }
 
-static bool far_wave_rows(int N) { return N >= 64 && N <= 512; }
+// Row passes on the register FFT: M and N powers of two, N = 64 .. 512.
+static bool far_wave_rows(const FarArgs& a) { return is_pow2(a.M) && is_pow2(a.N) && a.N >= 64 && a.N <= 512; }
 
 static void launch_rows_fwd(const FarArgs& a, hipStream_t s) {
-  if (far_wave_rows(a.N)) {
+  if (far_wave_rows(a)) {
     const int E = a.N / 64, jobs = a.B * a.M / (8 / E);
     const dim3 g(far_wave_blocks(jobs)), blk(256);
     switch (E) {
@@ -816,14 +839,15 @@ static void launch_rows_fwd(const FarArgs& a, hipStream_t s) {
     return;
   }
   const int lines = far_row_lines(a.N, a.M);
-  const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
+  const size_t shm = far_row_lds(a.N, lines);
   const int blocks = (a.B * a.M + lines - 1) / lines;
   hipLaunchKernelGGL(k_far_rows_fwd, dim3(blocks), dim3(256), shm, s, a, lines);
 }
 
-// Column passes on the register FFT (M = 64 .. 512, 8 columns per workgroup:
-// C / L = E wavefronts) when the layout's column block is 8 wide.
-static bool far_wave_cols(const FarArgs& a, int C) { return a.M >= 64 && a.M <= 512 && C == 8; }
+// Column passes on the register FFT (M a power of two in 64 .. 512, 8
+// columns per workgroup: C / L = E wavefronts) when the layout's column
+// block is 8 wide.
+static bool far_wave_cols(const FarArgs& a, int C) { return is_pow2(a.M) && a.M >= 64 && a.M <= 512 && C == 8; }
 
 // MODE 0: pointwise losses; 1: forward store (Psi, I); 3: FD loss + adjoint + inverse
 template <int MODE>
@@ -853,11 +877,10 @@ static void launch_cols(const FarArgs& a, int C, hipStream_t s) {
     return;
   }
   if (MODE == 3) {
-    const size_t shm = sizeof(float2) * (size_t)(a.M + 2 * C * (a.M + 1));
-    hipLaunchKernelGGL(k_far_fd_cols, dim3((a.N / C) * a.B), dim3(kColThreads), shm, s, a, C);
+    hipLaunchKernelGGL(k_far_fd_cols, dim3((a.N / C) * a.B), dim3(kColThreads), far_col_lds(a.M, C, true), s, a, C);
     return;
   }
-  const size_t shm = sizeof(float2) * (size_t)(a.M + 2 * C * (a.M + 1));
+  const size_t shm = far_col_lds(a.M, C, false);
   hipLaunchKernelGGL(k_far_cols<MODE == 1 ? 1 : 0>, dim3((a.N / C) * a.B), dim3(kColThreads), shm, s, a, C);
 }
 
@@ -885,7 +908,7 @@ static int far_fwd_bwd_launch(const PtyFar* d, const FarLayout& L, const float* 
   } else {
     launch_cols<0>(a, L.C, stream);
   }
-  if (far_wave_rows(a.N)) {
+  if (far_wave_rows(a)) {
     const int E = a.N / 64, jobs = a.B * a.M / (8 / E);
     const dim3 g(far_wave_blocks(jobs)), blk(256);
     switch (E) {
@@ -896,8 +919,8 @@ static int far_fwd_bwd_launch(const PtyFar* d, const FarLayout& L, const float* 
     }
   } else {
     const int lines = far_row_lines(a.N, a.M);
-    const size_t shm = sizeof(float2) * (size_t)(a.N + 2 * lines * a.N);
-    hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), shm, stream, a, lines);
+    hipLaunchKernelGGL(k_far_rows_bwd, dim3(a.M / lines, a.B), dim3(256), far_row_lds(a.N, lines), stream, a,
+                       lines);
   }
   const int mn = a.M * a.N;
   const int probe_blocks = (mn + 255) / 256 < 1024 ? (mn + 255) / 256 : 1024;
@@ -973,7 +996,8 @@ __global__ void k_far_shift_out(FarArgs a, float* out) {
     const int b = (int)(t / ((int64_t)M * N));
     const int q = (int)(t - (int64_t)b * M * N);
     const int k0 = q / N, k1 = q - k0 * N;
-    const int y = (k0 + M / 2) & (M - 1), x = (k1 + N / 2) & (N - 1);
+    const int y = k0 + M / 2 < M ? k0 + M / 2 : k0 + M / 2 - M;
+    const int x = k1 + N / 2 < N ? k1 + N / 2 : k1 + N / 2 - N;
     out[(int64_t)b * M * N + (int64_t)y * N + x] = a.inten[t];
   }
 }

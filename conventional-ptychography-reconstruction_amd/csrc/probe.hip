@@ -14,6 +14,9 @@ namespace pty {
 
 constexpr int kProbeBlocks = 256;
 
+// i mod n for i in [0, 2n) (a rolled index: i < n plus a shift in [0, n))
+__device__ __forceinline__ int wrap(int i, int n) { return i < n ? i : i - n; }
+
 __global__ __launch_bounds__(256) void k_probe_stats(const float2* __restrict__ p, int M, int N,
                                                      double* __restrict__ part) {
   __shared__ double red[3][256];
@@ -94,7 +97,7 @@ __global__ __launch_bounds__(256) void k_probe_apply(const float2* __restrict__ 
   const int total = M * N;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = t / N, j = t - i * N;
-    const int si = (i + pn.cp0) & (M - 1), sj = (j + pn.cp1) & (N - 1);
+    const int si = wrap(i + pn.cp0, M), sj = wrap(j + pn.cp1, N);
     out[t] = cscale(p[si * N + sj], pn.s);
   }
 }
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(256) void k_probe_bwd_dot(const float2* __restrict_
   const int total = M * N;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = t / N, j = t - i * N;
-    const float2 q = p[((i + cp0) & (M - 1)) * N + ((j + cp1) & (N - 1))];
+    const float2 q = p[wrap(i + cp0, M) * N + wrap(j + cp1, N)];
     const float2 g = gn[t];
     acc += (double)(g.x * q.x + g.y * q.y);
   }
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(256) void k_probe_bwd_apply(const float2* __restric
   const int total = M * N;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = t / N, j = t - i * N;
-    const int src = ((i + cp0) & (M - 1)) * N + ((j + cp1) & (N - 1));
+    const int src = wrap(i + cp0, M) * N + wrap(j + cp1, N);
     const float2 q = p[src];
     const float2 g = gn[t];
     float2 r;
@@ -156,8 +159,7 @@ static int probe_blocks(int M, int N) {
 }
 
 static int check_probe(int M, int N) {
-  PTY_REQUIRE(is_pow2(M) && is_pow2(N) && M >= 2 && N >= 2 && M <= 4096 && N <= 4096, PTY_EUNSUP,
-              "probe: %dx%d must be powers of two in [2, 4096]", M, N);
+  PTY_REQUIRE(M >= 2 && N >= 2 && M <= 4096 && N <= 4096, PTY_EUNSUP, "probe: %dx%d outside [2, 4096]", M, N);
   return PTY_OK;
 }
 

@@ -134,7 +134,8 @@ typedef struct PtyPoints {
 
 /* Far-field batch descriptor (Recovery.py:287-307). */
 typedef struct PtyFar {
-  int32_t M, N;              /* pattern / probe size; powers of two, 16..1024 */
+  int32_t M, N;              /* pattern / probe size: 8..4096, prime factors <= 64 (any
+                              * shape the LDS holds; powers of two take the fastest path) */
   int32_t canvas_h, canvas_w;
   int32_t n_batch;           /* positions in this batch */
   int32_t n_frames;          /* patterns in the measured stack */

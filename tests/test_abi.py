@@ -38,6 +38,18 @@ def test_errors_are_reported_without_a_gpu(pkg):
         pkg._lib.call("pty_upsample_fwd", 1, 67, 2, 1, 1, 1 << 30, None)   # 67 is prime > 64
     far = pkg.farfield.far_descriptor(64, 64, 256, 256, 10, 100, "FD_loss")
     assert pkg._lib.query("pty_farfield_workspace", far) > 0
+    # any probe shape whose prime factors are at most 64 and whose LDS tiles fit
+    for M, N in ((45, 63), (100, 128), (57, 114), (384, 384), (2048, 2048), (8, 4096)):
+        far = pkg.farfield.far_descriptor(M, N, 4096, 4096, 4, 10, "FD_loss")
+        assert pkg._lib.query("pty_farfield_workspace", far) >= 4 * M * N * 12, (M, N)
+    for M, N, what in ((134, 134, "prime factor"), (64, 67, "prime factor"), (4, 64, "outside"),
+                       (64, 5000, "outside")):
+        far = pkg.farfield.far_descriptor(M, N, 8192, 8192, 4, 10, "FD_loss")
+        assert pkg._lib.query("pty_farfield_workspace", far) == 0, (M, N)
+        with pytest.raises(RuntimeError, match=what):
+            pkg._lib.call("pty_farfield_fwd_bwd", far, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1 << 30, None)
+    with pytest.raises(RuntimeError, match="outside"):
+        pkg._lib.call("pty_probe_normalize_fwd", 1, 1, 64, 1, 1, 1, 1 << 20, None)
     with pytest.raises(ValueError):
         pkg.farfield.far_descriptor(64, 64, 256, 256, 10, 100, "l2")
     # batch_next: a batch longer than the order, an empty batch, a missing cursor

@@ -251,6 +251,109 @@ def test_farfield_loss_and_grads(pkg, dev, loss_type, M):
     assert rel_err(p.grad, p64.grad) < 1e-4
 
 
+# Probe shapes other than powers of two (the reference takes the shape of the
+# measured patterns): odd sizes (one-column blocks, fftshift by M // 2), prime
+# factors 3, 5, 7, 19, non-square windows, a power-of-two M with a
+# non-power-of-two N (register-FFT column passes over N = 200 columns) and
+# the reverse; 1000 x 1000 (2^3 5^3) has column passes of 72 KB of LDS.
+ANY_SHAPES = [(45, 63), (100, 128), (64, 200), (57, 114), (96, 40), (384, 384), (1000, 1000)]
+
+
+@pytest.mark.parametrize("M,N", ANY_SHAPES)
+def test_farfield_intensity_any_shape(pkg, dev, M, N):
+    rng = np.random.default_rng(M * 1000 + N)
+    H, W = 2 * M + 11, 2 * N + 5
+    obj = (rng.standard_normal((H, W)) + 1j * rng.standard_normal((H, W))).astype(np.complex64)
+    probe = (rng.standard_normal((M, N)) + 1j * rng.standard_normal((M, N))).astype(np.complex64)
+    tlY = rng.integers(0, H - M + 1, 5)
+    tlX = rng.integers(0, W - N + 1, 5)
+    tl = pkg.positions_tensor(tlY, tlX, dev)
+    order = [2, 0, 4, 4]
+    bidx = torch.tensor(order, dtype=torch.int32, device=dev)
+    I = pkg.farfield_intensity(torch.from_numpy(obj).to(dev), torch.from_numpy(probe).to(dev), bidx, tl, 5)
+    assert I.shape == (4, M, N)
+    for b, i in enumerate(order):
+        ref = orc.farfield_intensity(torch.from_numpy(obj).to(torch.complex128),
+                                     torch.from_numpy(probe).to(torch.complex128), int(tlY[i]), int(tlX[i]))
+        err = (I[b].cpu().double() - ref).abs().max() / ref.abs().max()
+        assert err < 1e-5, (b, float(err))
+
+
+@pytest.mark.parametrize("loss_type", ["smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss"])
+@pytest.mark.parametrize("M,N", ANY_SHAPES)
+def test_farfield_loss_and_grads_any_shape(pkg, dev, loss_type, M, N):
+    rng = np.random.default_rng(M * 7 + N + len(loss_type))
+    H, W = 2 * M + 9, 2 * N + 3
+    F = 6
+    obj = (0.5 + 0.5 * rng.random((H, W))) * np.exp(1j * rng.uniform(-1, 1, (H, W)))
+    probe = (rng.standard_normal((M, N)) + 1j * rng.standard_normal((M, N))) * 0.3
+    tlY = rng.integers(0, H - M + 1, F)
+    tlX = rng.integers(0, W - N + 1, F)
+    stack = rng.poisson(5.0, (M, N, F)).astype(np.float32)
+    batch = [3, 0, 5]
+    o = torch.from_numpy(obj.astype(np.complex64)).to(dev).requires_grad_(True)
+    p = torch.from_numpy(probe.astype(np.complex64)).to(dev).requires_grad_(True)
+    st = pkg.prepare_stack(torch.from_numpy(stack).to(dev))
+    tl = pkg.positions_tensor(tlY, tlX, dev)
+    L, per = pkg.farfield_loss(o, p, st, torch.tensor(batch, dtype=torch.int32, device=dev), tl, loss_type,
+                               return_per_pattern=True)
+    L.backward()
+    o64 = torch.from_numpy(obj.astype(np.complex64)).to(torch.complex128).requires_grad_(True)
+    p64 = torch.from_numpy(probe.astype(np.complex64)).to(torch.complex128).requires_grad_(True)
+    Lr = orc.batch_loss(o64, p64, torch.from_numpy(stack).double(), batch, tlY, tlX, loss_type)
+    Lr.backward()
+    assert abs(L.item() - Lr.item()) < 2e-5 * abs(Lr.item())
+    for b, i in enumerate(batch):
+        li = orc.batch_loss(o64.detach(), p64.detach(), torch.from_numpy(stack).double(), [i], tlY, tlX, loss_type)
+        assert abs(per[b].item() - li.item()) < 2e-5 * abs(li.item())
+    assert rel_err(o.grad, o64.grad) < 1e-4
+    assert rel_err(p.grad, p64.grad) < 1e-4
+
+
+@pytest.mark.parametrize("M,N", [(45, 63), (100, 100), (57, 114)])
+def test_probe_normalize_any_shape(pkg, dev, M, N):
+    rng = np.random.default_rng(M + N)
+    yy, xx = np.mgrid[0:M, 0:N]
+    # the centre of mass far from the middle: rolls of both signs
+    amp = np.exp(-((yy - 0.8 * M) ** 2 + (xx - 0.2 * N) ** 2) / (0.05 * M * N))
+    p0 = (amp * np.exp(1j * rng.uniform(-1, 1, (M, N)))).astype(np.complex64)
+    p = torch.from_numpy(p0).to(dev).requires_grad_(True)
+    pn, stats = pkg.normalize_probe(p, return_stats=True)
+    pr = torch.from_numpy(p0).to(torch.complex128).requires_grad_(True)
+    ref = orc.probe_center_normalize(pr)
+    cp = orc.probe_com_shift(pr)
+    st = stats.cpu().numpy()
+    assert int(st[0]) == int(cp[0]) % M and int(st[1]) == int(cp[1]) % N and st[4] == 1.0
+    assert int(cp[0]) != 0 and int(cp[1]) != 0
+    assert rel_err(pn, ref) < 1e-6
+    g = torch.randn(M, N, dtype=torch.complex64, device=dev)
+    (torch.real(torch.conj(g) * pn)).sum().backward()
+    (torch.real(torch.conj(g.cpu().to(torch.complex128)) * ref)).sum().backward()
+    assert rel_err(p.grad, pr.grad) < 1e-5
+
+
+@pytest.mark.parametrize("loss_type", ["smooth_L1_loss", "FD_loss"])
+def test_fused_step_non_power_of_two_probe(pkg, dev, loss_type):
+    """The whole fused step with a 40 x 40 probe (2^3 * 5) on the small canvas."""
+    inst = pkg.synthetic.make_named("small", seed=2, M=40)
+    model = _model(pkg, dev, inst.canvas, 40, log2T=12, residual=True,
+                   probe_init=torch.from_numpy(inst.probe.astype(np.complex64)))
+    params = oracle_params(model)
+    eng = pkg.FusedReconstruction(model, torch.from_numpy(inst.stack).to(dev), inst.tlY, inst.tlX,
+                                  loss_type=loss_type, batch_size=4)
+    batch = [1, 8, 20, 33]
+    loss = eng.step(torch.tensor(batch, dtype=torch.int32, device=dev), update=False).sum().item()
+    ref_loss, g = step_reference(model, params, inst, batch, loss_type)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss), (loss, ref_loss)
+    assert rel_err(eng.flat.g(model.xy_hash_encoder.params), g["obj_table"]) < 2e-3
+    for a, b in zip(model.object_complex_mlp.tensors(), g["obj_mlp"]):
+        assert rel_err(eng.flat.g(a), b) < 2e-3, ("obj", a.shape, rel_err(eng.flat.g(a), b))
+    _, g32 = step_reference(model, oracle_params(model, torch.float32), inst, batch, loss_type,
+                            dtype=torch.float32)
+    bar = max(2e-3, 4 * rel_err(g32["prb_table"], g["prb_table"]))
+    assert rel_err(eng.flat.g(model.probe_hash_encoder.params), g["prb_table"]) < bar
+
+
 # ---------------------------------------------------------------- fused step
 @pytest.mark.parametrize("loss_type", ["smooth_L1_loss", "FD_loss", "Poisson_likelihood_loss"])
 def test_fused_step_matches_oracle(pkg, dev, loss_type):
