@@ -30,6 +30,7 @@
 // Two n-point transforms per line instead of an n- and an m-point one, and
 // line buffers of n instead of m (k_up2_rows).
 #include "common.h"
+#include "fft_reg.h"
 #include "fft_wave.h"
 
 namespace pty {
@@ -278,6 +279,110 @@ __global__ __launch_bounds__(256) void k_up2_rows_w(const float2* __restrict__ i
   }
 }
 
+// k_up2_rows for n = P Q, P a power of two (16 or 32), Q = 57 (n = 912,
+// 1824: the INR grids of canvases 1824 and 3648), on register DFTs.  One
+// wavefront per workgroup takes L = 64 / P lines per job (n = Q n1 + n2 =
+// m1 + Q m2, k = k1 + P k2):
+//   pass A : lane n2 < Q of each line loads x[Q n1 + n2] (n1 < P; coalesced
+//            across the lanes), P-point DFT over n1, times W_n^{n2 k1}, into
+//            the line's LDS slot at Q k1 + n2;
+//   pass B : lane (line, k1) takes the Q values at Q k1 .. Q k1 + Q - 1:
+//            Q-point DFT (X[k1 + P k2] for k2 < Q), the half-sample phase of
+//            each frequency, inverse Q-point DFT, times W_n^{-m1 k1}, back to
+//            the same Q slots;
+//   pass A': lane m1 < Q reads Q k1 + m1 (k1 < P), inverse P-point DFT: the
+//            shifted line at m1 + Q m2 (m2 < P), stored with the even samples
+//            (coalesced across the lanes).
+// The exchanges are wavefront-synchronous (no workgroup barrier); LDS is
+// L n float2 (29 KB).  Twiddles exp(-2 pi i k / m), k < m = 2n, come from a
+// table built at compile time (double precision, rounded once).
+// Output and scales as k_up2_rows.
+__constant__ constexpr CTwPairs<3648> kTwUp3648 = make_ctw_pairs<3648>();
+__constant__ constexpr CTwPairs<1824> kTwUp1824 = make_ctw_pairs<1824>();
+
+template <int P, int Q, int MODE>
+__global__ __launch_bounds__(64) void k_up2_rows_r(const float2* __restrict__ in, float2* __restrict__ out,
+                                                   int rows, float scale) {
+  constexpr int n = P * Q, m = 2 * n, h = n / 2, L = 64 / P;
+  static_assert(P <= 64 && 64 % P == 0 && Q <= 64, "k_up2_rows_r: P divides 64, Q at most 64");
+  static_assert(m == 3648 || m == 1824, "k_up2_rows_r: twiddle tables for n = 912, 1824");
+  const float2* tw = reinterpret_cast<const float2*>(m == 3648 ? kTwUp3648.v : kTwUp1824.v);
+  __shared__ float2 buf[L * n];
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * L;
+  const float ns = (float)n * scale;
+  // pass A
+  for (int li = 0; li < L; ++li) {
+    const int64_t row = r0 + li;
+    if (lane < Q) {
+      float2 v[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) {
+        const int j = Q * n1 + lane;
+        v[n1] = row < rows ? (MODE == 0 ? in[row * n + j] : in[row * m + 2 * j + 1]) : make_float2(0.f, 0.f);
+      }
+      dft_reg<P, -1>(v);
+      float2* dst = buf + li * n + lane;
+      int e = 0;   // lane k1 mod n
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        dst[Q * k1] = k1 ? cmul(v[k1], tw[2 * e]) : v[k1];
+        e += lane;
+        if (e >= n) e -= n;
+      }
+    }
+  }
+  wave_lds_sync();
+  // pass B
+  {
+    const int k1 = lane % P;
+    float2* slot = buf + (lane / P) * n + Q * k1;
+    float2 x[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) x[i] = slot[i];
+    dft_reg<Q, -1>(x);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) {
+      const int k = k1 + P * k2;
+      const int sk = k < h ? k : k - n;                    // signed frequency
+      const float2 w = tw[sk < 0 ? sk + m : sk];           // exp(-2 pi i s / m) = conj(phi)
+      x[k2] = (MODE == 0) ? cmulc(x[k2], w) : cmul(x[k2], w);
+    }
+    dft_reg<Q, +1>(x);
+    int e = 0;   // m1 k1 mod n
+#pragma unroll
+    for (int m1 = 0; m1 < Q; ++m1) {
+      slot[m1] = m1 ? cmulc(x[m1], tw[2 * e]) : x[m1];
+      e += k1;
+      if (e >= n) e -= n;
+    }
+  }
+  wave_lds_sync();
+  // pass A'
+  for (int li = 0; li < L; ++li) {
+    const int64_t row = r0 + li;
+    if (lane < Q && row < rows) {
+      float2 v[P];
+      const float2* src = buf + li * n + lane;
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) v[k1] = src[Q * k1];
+      dft_reg<P, +1>(v);
+#pragma unroll
+      for (int m2 = 0; m2 < P; ++m2) {
+        const int j = lane + Q * m2;
+        const float2 r = v[m2];
+        if (MODE == 0) {
+          const float2 x = in[row * n + j];
+          reinterpret_cast<float4*>(out + row * m)[j] = make_float4(x.x * ns, x.y * ns, r.x * scale, r.y * scale);
+        } else {
+          const float2 ev = in[row * m + 2 * j];
+          out[row * n + j] = make_float2(ev.x * ns + r.x * scale, ev.y * ns + r.y * scale);
+        }
+      }
+    }
+  }
+}
+
 // The same line transform along the columns of a row-major array, for
 // n = 64 E at ds = 2, so that the upsample needs no transposes: a workgroup of
 // 8 / L wavefronts takes 8 columns (coalesced 64-byte row segments staged in
@@ -361,6 +466,8 @@ static void launch_up2_cols(const float2* in, float2* out, int n, float scale, h
 
 // power-of-two n = 64 .. 512 at ds = 2: register-FFT row and column stages
 static bool up_wave(int n, int ds) { return ds == 2 && n >= 64 && n <= 512 && is_pow2(n); }
+// n = 912, 1824 at ds = 2: register-DFT row stages (k_up2_rows_r, compile-time twiddles)
+static bool up_reg(int n) { return n == 912 || n == 1824; }
 
 struct UpLaunch {
   int lines;     // lines per workgroup of the row stages
@@ -430,6 +537,13 @@ static void launch_up2_rows(const float2* in, float2* out, const FftPlan& Pn, in
     }
     return;
   }
+  if (up_reg(n)) {
+    if (n == 1824)
+      hipLaunchKernelGGL((k_up2_rows_r<32, 57, MODE>), dim3((rows + 1) / 2), dim3(64), 0, s, in, out, rows, scale);
+    else
+      hipLaunchKernelGGL((k_up2_rows_r<16, 57, MODE>), dim3((rows + 3) / 4), dim3(64), 0, s, in, out, rows, scale);
+    return;
+  }
   const size_t shm = sizeof(float2) * (size_t)((u.lds_tw ? 2 * n : 0) + 2 * u.lines * n);
   const int blocks = (rows + u.lines - 1) / u.lines;
   // 128 threads for long lines (n = 1824: 243 us against 255 us with 256)
@@ -449,7 +563,9 @@ static void up_stage(const float2* in, float2* out, const FftPlan& Pn, const Fft
     launch_up_rows<MODE>(in, out, Pn, Pm, rows, scale, up_launch(n, m), gtw, s);
 }
 
+// true when no stage of (n, m) reads the global twiddle table
 static bool up_lds_twiddles(int n, int m) {
+  if (m == 2 * n && up_reg(n)) return true;   // no global table needed
   return (m == 2 * n && n % 2 == 0) ? up2_launch(n).lds_tw : up_launch(n, m).lds_tw;
 }
 
