@@ -28,6 +28,8 @@
 //                 4 terms per point and level); each step then gathers them.
 #include "hashgrid.h"
 
+#include <algorithm>
+
 namespace pty {
 
 constexpr uint32_t kPrime = 2654435761u;
@@ -304,14 +306,18 @@ __global__ __launch_bounds__(256) void k_dense_pass1(PtyGrid g, PlanLayout L, Pt
 }
 
 // Row form of pass 1: one workgroup per (dense level l, row i).  The row's
-// feature gradients and x fractions are staged in LDS with coalesced loads,
-// then each cell's S0 / S1 is summed from LDS in point order (the same sums
-// as cell_sums) and A[l][i][cx] = S0(cx) + S1(cx - 1) is written.  LDS:
-// n0 * 12 B + 2 (res + 1) * 8 B.
+// feature gradients and x fractions and the level's cell bounds are staged in
+// LDS (each thread issues all of its loads of a batch before its first LDS
+// store, so the staging is one round trip to memory, not one per element),
+// then each thread sums its cells' S0 / S1 from LDS in point order (the same
+// sums as cell_sums; cells c = tid + 256 k, k < KC) and A[l][i][cx] =
+// S0(cx) + S1(cx - 1) is written, S1 passed through LDS.  LDS:
+// max(n0, res + 1) * 8 B + n0 * 4 B + (res + 2) * 4 B.
+template <int KC>
 __global__ __launch_bounds__(256) void k_dense_pass1_rows(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                           const float2* __restrict__ gfeat, float2* __restrict__ A) {
   extern __shared__ float2 srow[];
-  const int n0 = L.n0;
+  const int n0 = L.n0, tid = threadIdx.x;
   int b = blockIdx.x, l = -1;
   for (int q = 0; q < g.n_levels; ++q) {
     if (!((g.dense_mask >> q) & 1u)) continue;
@@ -321,39 +327,95 @@ __global__ __launch_bounds__(256) void k_dense_pass1_rows(PtyGrid g, PlanLayout 
   if (l < 0) return;   // uniform across the workgroup
   const int i = b;
   const int res1 = (int)g.res[l] + 1;
-  float2* sg = srow;                                  // [n0] feature gradients of the row
-  float* sf = reinterpret_cast<float*>(sg + n0);      // [n0] x fractions
-  float2* s0c = sg + n0 + (n0 + 1) / 2;               // [res1] S0 per cell
-  float2* s1c = s0c + res1;                           // [res1] S1 per cell
+  float2* sg = srow;                                          // [n0] row gradients, later [res1] S1
+  float* sf = reinterpret_cast<float*>(srow + max(n0, res1));  // [n0] x fractions
+  int* sst = reinterpret_cast<int*>(sf + n0);                  // [res1 + 1] first point of each cell
   const float2* gl = gfeat + (int64_t)l * pts.n + (int64_t)i * n0;
   const float* fxl = P.fx + (int64_t)l * n0;
-  for (int t = threadIdx.x; t < n0; t += blockDim.x) {
-    sg[t] = gl[t];
-    sf[t] = fxl[t];
+  const int* st = P.s0 + L.off_s[l];
+  constexpr int kB = 8;   // row elements per thread and batch
+  // the first batch and the cell bounds in one round trip (clamped loads,
+  // guarded stores), then any further batches
+  {
+    float2 vg[kB];
+    float vf[kB];
+    int vs[KC + 1];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int t = min(u * 256 + tid, n0 - 1);
+      vg[u] = gl[t];
+      vf[u] = fxl[t];
+    }
+#pragma unroll
+    for (int u = 0; u <= KC; ++u) vs[u] = st[min(u * 256 + tid, res1)];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int t = u * 256 + tid;
+      if (t < n0) {
+        sg[t] = vg[u];
+        sf[t] = vf[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u <= KC; ++u) {
+      const int t = u * 256 + tid;
+      if (t <= res1) sst[t] = vs[u];
+    }
+  }
+  for (int t0 = kB * 256; t0 < n0; t0 += kB * 256) {
+    float2 vg[kB];
+    float vf[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int t = min(t0 + u * 256 + tid, n0 - 1);
+      vg[u] = gl[t];
+      vf[u] = fxl[t];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int t = t0 + u * 256 + tid;
+      if (t < n0) {
+        sg[t] = vg[u];
+        sf[t] = vf[u];
+      }
+    }
   }
   __syncthreads();
-  const int* st = P.s0 + L.off_s[l];
-  for (int c = threadIdx.x; c < res1; c += blockDim.x) {
+  float2 s0r[KC], s1r[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int c = tid + 256 * k;
     float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
-    const int j1 = st[c + 1];
+    if (c < res1) {
+      const int j1 = sst[c + 1];
 #pragma unroll 4
-    for (int j = st[c]; j < j1; ++j) {
-      const float f = sf[j];
-      const float2 v = sg[j];
-      s0.x += (1.f - f) * v.x;
-      s0.y += (1.f - f) * v.y;
-      s1.x += f * v.x;
-      s1.y += f * v.y;
+      for (int j = sst[c]; j < j1; ++j) {
+        const float f = sf[j];
+        const float2 v = sg[j];
+        s0.x += (1.f - f) * v.x;
+        s0.y += (1.f - f) * v.y;
+        s1.x += f * v.x;
+        s1.y += f * v.y;
+      }
     }
-    s0c[c] = s0;
-    s1c[c] = s1;
+    s0r[k] = s0;
+    s1r[k] = s1;
+  }
+  __syncthreads();   // every thread is done reading the row
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int c = tid + 256 * k;
+    if (c < res1) sg[c] = s1r[k];
   }
   __syncthreads();
   float2* Ar = A + L.off_a[l] + (int64_t)i * res1;
-  for (int c = threadIdx.x; c < res1; c += blockDim.x) {
-    const float2 s0 = s0c[c];
-    const float2 left = c > 0 ? s1c[c - 1] : make_float2(0.f, 0.f);
-    Ar[c] = make_float2(s0.x + left.x, s0.y + left.y);
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int c = tid + 256 * k;
+    if (c < res1) {
+      const float2 left = c > 0 ? sg[c - 1] : make_float2(0.f, 0.f);
+      Ar[c] = make_float2(s0r[k].x + left.x, s0r[k].y + left.y);
+    }
   }
 }
 
@@ -724,13 +786,21 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
         ++n_dense;
         if ((int)g.res[l] + 1 > res1_max) res1_max = (int)g.res[l] + 1;
       }
-    const size_t shm = sizeof(float2) * ((size_t)L.n0 + (L.n0 + 1) / 2 + 2 * (size_t)res1_max);
-    if (shm <= 65536)
-      hipLaunchKernelGGL(k_dense_pass1_rows, dim3((unsigned)(n_dense * L.n1)), dim3(256), shm, stream, g, L, pts, P,
-                         gfeat, A);
-    else
+    const size_t shm = sizeof(float2) * (size_t)std::max(L.n0, res1_max) + sizeof(float) * (size_t)L.n0 +
+                       sizeof(int) * ((size_t)res1_max + 1);
+    const int kc = (res1_max + 255) / 256;
+    const dim3 grid((unsigned)(n_dense * L.n1)), blk(256);
+    if (shm <= 65536 && kc <= 4) {
+      switch (kc) {
+        case 1: hipLaunchKernelGGL(k_dense_pass1_rows<1>, grid, blk, shm, stream, g, L, pts, P, gfeat, A); break;
+        case 2: hipLaunchKernelGGL(k_dense_pass1_rows<2>, grid, blk, shm, stream, g, L, pts, P, gfeat, A); break;
+        case 3: hipLaunchKernelGGL(k_dense_pass1_rows<3>, grid, blk, shm, stream, g, L, pts, P, gfeat, A); break;
+        default: hipLaunchKernelGGL(k_dense_pass1_rows<4>, grid, blk, shm, stream, g, L, pts, P, gfeat, A); break;
+      }
+    } else {
       hipLaunchKernelGGL(k_dense_pass1, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, g, L, pts, P,
                          gfeat, A);
+    }
   }
   uint32_t n_dense_entries = 0, hashed_chunks = 0, n_split_entries = 0;
   bool coarse = true;
