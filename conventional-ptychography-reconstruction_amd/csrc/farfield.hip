@@ -645,19 +645,79 @@ __global__ __launch_bounds__(256) void k_far_rows_bwd(FarArgs a, int lines) {
   for (int t = tid; t < lines * N; t += nt) base[(int64_t)r0 * N + t] = res[t];
 }
 
+// F4 + F5 share a list of batch windows in LDS: wave 0 lists, in batch order,
+// the valid entries of batch[c0, c1) (and, for a canvas tile, only those that
+// intersect it) by a ballot scan; returns the count.
+constexpr int kMaxWinLds = 1024;
+constexpr int kTile = 32;
+constexpr int kWinBatch = 8;   // dprobe: windows whose loads are in flight together
+constexpr int kWinBatchObj = 4;   // dobject: the same per pixel (four pixels per thread)
+
+__device__ __forceinline__ int list_windows(const FarArgs& a, int c0, int c1, bool tile_test, int Y0, int X0,
+                                            int3* list, int* n_list) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int count = 0;
+    for (int b0 = c0; b0 < c1; b0 += 64) {
+      const int b = b0 + lane;
+      int ty = 0, tx = 0, p;
+      bool hit = false;
+      if (b < c1 && window_of(a, b, ty, tx, p))
+        hit = !tile_test || (ty < Y0 + kTile && ty + a.M > Y0 && tx < X0 + kTile && tx + a.N > X0);
+      const uint64_t mask = __ballot(hit);
+      const int pos = count + __popcll(mask & ((1ull << lane) - 1ull));
+      if (hit) list[pos] = make_int3(b, ty, tx);
+      count += __popcll(mask);
+    }
+    if (lane == 0) *n_list = count;
+  }
+  __syncthreads();
+  const int n = *n_list;
+  return n;
+}
+
 // F4: dprobe = sum_b dpsi_b * conj(patch_b) over valid batch entries (fixed
-// order); workgroup blk of nblk strides over the probe pixels.
-__device__ __forceinline__ void far_dprobe(const FarArgs& a, int blk, int nblk) {
+// order); workgroup blk of nblk strides over the probe pixels.  The windows
+// come from the LDS list and the loads of kWinBatch of them are issued
+// together (one round trip per batch instead of three per entry).
+__device__ __forceinline__ void far_dprobe(const FarArgs& a, int blk, int nblk, int3* list, int* n_list) {
   const int MN = a.M * a.N;
-  for (int q = blk * 256 + (int)threadIdx.x; q < MN; q += nblk * 256) {
+  const int stride = nblk * 256;
+  const int q0 = blk * 256 + (int)threadIdx.x;
+  if (a.B > kMaxWinLds) {   // uniform: batches longer than the list, one entry at a time
+    for (int q = q0; q < MN; q += stride) {
+      const int r = q / a.N, c = q - r * a.N;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int b = 0; b < a.B; ++b) {
+        int ty, tx, p;
+        if (!window_of(a, b, ty, tx, p)) continue;
+        const float2 v = cmulc(a.psi[(int64_t)b * MN + q], a.obj[(int64_t)(ty + r) * a.W + tx + c]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      a.dprobe[q] = acc;
+    }
+    return;
+  }
+  const int nl = list_windows(a, 0, a.B, false, 0, 0, list, n_list);
+  for (int q = q0; q < MN; q += stride) {
     const int r = q / a.N, c = q - r * a.N;
     float2 acc = make_float2(0.f, 0.f);
-    for (int b = 0; b < a.B; ++b) {
-      int ty, tx, p;
-      if (!window_of(a, b, ty, tx, p)) continue;
-      const float2 v = cmulc(a.psi[(int64_t)b * MN + q], a.obj[(int64_t)(ty + r) * a.W + tx + c]);
-      acc.x += v.x;
-      acc.y += v.y;
+    for (int k0 = 0; k0 < nl; k0 += kWinBatch) {
+      float2 pv[kWinBatch], ov[kWinBatch];
+#pragma unroll
+      for (int u = 0; u < kWinBatch; ++u) {
+        const int3 w = list[min(k0 + u, nl - 1)];
+        pv[u] = a.psi[(int64_t)w.x * MN + q];
+        ov[u] = a.obj[(int64_t)(w.y + r) * a.W + w.z + c];
+      }
+#pragma unroll
+      for (int u = 0; u < kWinBatch; ++u)
+        if (k0 + u < nl) {
+          const float2 v = cmulc(pv[u], ov[u]);
+          acc.x += v.x;
+          acc.y += v.y;
+        }
     }
     a.dprobe[q] = acc;
   }
@@ -665,15 +725,10 @@ __device__ __forceinline__ void far_dprobe(const FarArgs& a, int blk, int nblk) 
 
 // F5: dobject (overwritten, whole canvas) = sum over the windows covering each
 // pixel of dpsi_b * conj(probe_n), in batch order: a gather, no atomics.
-// Workgroup = one kTile x kTile canvas tile: wave 0 first lists, in batch
-// order, the batch windows that intersect the tile (a ballot scan), so each
-// pixel only tests the few windows near it instead of all B.
-constexpr int kMaxWinLds = 1024;
-constexpr int kTile = 32;
-
-__device__ __forceinline__ void far_dobject(const FarArgs& a, int tile) {
-  __shared__ int3 list[kMaxWinLds];
-  __shared__ int n_list;
+// Workgroup = one kTile x kTile canvas tile: each pixel only tests the few
+// listed windows that intersect the tile; a thread's loads for its kPer
+// pixels and kWinBatch windows are issued together.
+__device__ __forceinline__ void far_dobject(const FarArgs& a, int tile, int3* list, int* n_list) {
   const int tiles_x = (a.W + kTile - 1) / kTile;
   const int Y0 = (tile / tiles_x) * kTile, X0 = (tile % tiles_x) * kTile;
   const int MN = a.M * a.N;
@@ -684,39 +739,34 @@ __device__ __forceinline__ void far_dobject(const FarArgs& a, int tile) {
   // batch entries in chunks of kMaxWinLds (one chunk for every practical batch)
   for (int c0 = 0; c0 < a.B; c0 += kMaxWinLds) {
     const int c1 = min(a.B, c0 + kMaxWinLds);
-    if (threadIdx.x < 64) {
-      const int lane = threadIdx.x;
-      int count = 0;
-      for (int b0 = c0; b0 < c1; b0 += 64) {
-        const int b = b0 + lane;
-        int ty = 0, tx = 0, p;
-        bool hit = false;
-        if (b < c1 && window_of(a, b, ty, tx, p))
-          hit = ty < Y0 + kTile && ty + a.M > Y0 && tx < X0 + kTile && tx + a.N > X0;
-        const uint64_t mask = __ballot(hit);
-        const int pos = count + __popcll(mask & ((1ull << lane) - 1ull));
-        if (hit) list[pos] = make_int3(b, ty, tx);
-        count += __popcll(mask);
-      }
-      if (lane == 0) n_list = count;
-    }
-    __syncthreads();
-    const int nl = n_list;
+    const int nl = list_windows(a, c0, c1, true, Y0, X0, list, n_list);
+    for (int k0 = 0; k0 < nl; k0 += kWinBatchObj) {
+      float2 pv[kPer][kWinBatchObj], qv[kPer][kWinBatchObj];
+      bool in[kPer][kWinBatchObj];
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int t = threadIdx.x + 256 * i;
-      const int yy = Y0 + t / kTile, xx = X0 + t % kTile;
-      for (int k = 0; k < nl; ++k) {
-        const int3 w = list[k];
-        const int r = yy - w.y, c = xx - w.z;
-        if ((unsigned)r < (unsigned)a.M && (unsigned)c < (unsigned)a.N) {
-          const float2 v = cmulc(a.psi[(int64_t)w.x * MN + r * a.N + c], a.probe[r * a.N + c]);
-          acc[i].x += v.x;
-          acc[i].y += v.y;
+      for (int i = 0; i < kPer; ++i) {
+        const int t = threadIdx.x + 256 * i;
+        const int yy = Y0 + t / kTile, xx = X0 + t % kTile;
+#pragma unroll
+        for (int u = 0; u < kWinBatchObj; ++u) {
+          const int3 w = list[min(k0 + u, nl - 1)];
+          const int r = yy - w.y, c = xx - w.z;
+          in[i][u] = k0 + u < nl && (unsigned)r < (unsigned)a.M && (unsigned)c < (unsigned)a.N;
+          pv[i][u] = in[i][u] ? a.psi[(int64_t)w.x * MN + r * a.N + c] : make_float2(0.f, 0.f);
+          qv[i][u] = in[i][u] ? a.probe[r * a.N + c] : make_float2(0.f, 0.f);
         }
       }
+#pragma unroll
+      for (int i = 0; i < kPer; ++i)
+#pragma unroll
+        for (int u = 0; u < kWinBatchObj; ++u)
+          if (in[i][u]) {
+            const float2 v = cmulc(pv[i][u], qv[i][u]);
+            acc[i].x += v.x;
+            acc[i].y += v.y;
+          }
     }
-    __syncthreads();
+    __syncthreads();   // the next chunk's list overwrites this one
   }
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
@@ -730,8 +780,10 @@ __device__ __forceinline__ void far_dobject(const FarArgs& a, int tile) {
 // rest one canvas tile of dobject each (the two read the same dpsi and write
 // disjoint outputs).
 __global__ __launch_bounds__(256) void k_far_dgrad(FarArgs a, int n_probe_blocks) {
-  if ((int)blockIdx.x < n_probe_blocks) far_dprobe(a, blockIdx.x, n_probe_blocks);
-  else far_dobject(a, blockIdx.x - n_probe_blocks);
+  __shared__ int3 list[kMaxWinLds];
+  __shared__ int n_list;
+  if ((int)blockIdx.x < n_probe_blocks) far_dprobe(a, blockIdx.x, n_probe_blocks, list, &n_list);
+  else far_dobject(a, blockIdx.x - n_probe_blocks, list, &n_list);
 }
 
 // Dynamic LDS of the LDS-FFT column passes (k_far_cols, k_far_fd_cols):

@@ -404,13 +404,25 @@ __global__ __launch_bounds__(64 * E) void k_up2_cols_w(const float2* __restrict_
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c0 = blockIdx.x * 8, c = tid & 7;
   const float ns = (float)n * scale;
-  for (int t = tid; t < 8 * n; t += NT) {
-    const int j = t >> 3;
-    if (MODE == 0) {
-      xt[c * S + j] = in[(int64_t)j * n + c0 + c];
-    } else {
-      xt[c * S + j] = in[(int64_t)(2 * j + 1) * n + c0 + c];
-      rt[c * S + j] = in[(int64_t)(2 * j) * n + c0 + c];
+  {
+    // all of a thread's loads first, then its LDS stores (one round trip)
+    constexpr int IT = 8 * n / NT;
+    float2 va[IT], vb[IT];
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      const int j = (tid + u * NT) >> 3;
+      if (MODE == 0) {
+        va[u] = in[(int64_t)j * n + c0 + c];
+      } else {
+        va[u] = in[(int64_t)(2 * j + 1) * n + c0 + c];
+        vb[u] = in[(int64_t)(2 * j) * n + c0 + c];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      const int j = (tid + u * NT) >> 3;
+      xt[c * S + j] = va[u];
+      if (MODE == 1) rt[c * S + j] = vb[u];
     }
   }
   __syncthreads();
