@@ -420,26 +420,49 @@ __global__ __launch_bounds__(256) void k_dense_pass1_rows(PtyGrid g, PlanLayout 
 }
 
 // G(cx, cy) from A (dense levels), or the part of it over the rows i with
-// i - st[cell] = s mod S (lane s of an entry split over S lanes)
+// i - st[cell] = s mod S (lane s of an entry split over S lanes): the rows of
+// cell cy (weight 1 - fy) and of cell cy - 1 (weight fy), each summed in row
+// order and the two sums added.  The three cell bounds are loaded together,
+// then the rows of both cells in predicated batches of kRowBatch, so a
+// typical entry (a few rows per cell) waits on memory twice, not once per row.
+constexpr int kRowBatch = 4;
+
 __device__ __forceinline__ float2 corner_from_A(const PlanLayout& L, const PlanPtrs& P, int l, int res,
                                                 const float2* __restrict__ Al, int cx, int cy, int s, int S) {
   const int* st = P.s1 + L.off_s[l];
   const float* fyl = P.fy + (int64_t)l * L.n1;
-  float ax = 0.f, ay = 0.f;
+  const int64_t stride = res + 1;
+  // cy in [0, res]: cell cy is rows [b0, b1), cell cy - 1 rows [a0, b0) (empty at cy = 0)
+  const int b0 = st[cy], b1 = st[cy + 1];
+  const int sa = st[max(cy - 1, 0)];
+  const int a0 = cy > 0 ? sa : b0;
+  float2 sum0 = make_float2(0.f, 0.f), sum1 = make_float2(0.f, 0.f);
+  for (int i = b0 + s, j = a0 + s; i < b1 || j < b0; i += kRowBatch * S, j += kRowBatch * S) {
+    float2 v0[kRowBatch], v1[kRowBatch];
+    float f0[kRowBatch], f1[kRowBatch];
 #pragma unroll
-  for (int dy = 0; dy < 2; ++dy) {
-    const int cell = cy - dy;
-    if (cell < 0 || cell > res) continue;
-    const int i1 = st[cell + 1];
-#pragma unroll 8
-    for (int i = st[cell] + s; i < i1; i += S) {
-      const float w = dy ? fyl[i] : 1.f - fyl[i];
-      const float2 v = Al[(int64_t)i * (res + 1) + cx];
-      ax += w * v.x;
-      ay += w * v.y;
+    for (int u = 0; u < kRowBatch; ++u) {
+      const int ii = i + u * S, jj = j + u * S;
+      const int ic = ii < b1 ? ii : 0, jc = jj < b0 ? jj : 0;   // row 0 stands in for a missing row
+      v0[u] = Al[ic * stride + cx];
+      f0[u] = fyl[ic];
+      v1[u] = Al[jc * stride + cx];
+      f1[u] = fyl[jc];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowBatch; ++u) {
+      if (i + u * S < b1) {
+        const float w = 1.f - f0[u];
+        sum0.x += w * v0[u].x;
+        sum0.y += w * v0[u].y;
+      }
+      if (j + u * S < b0) {
+        sum1.x += f1[u] * v1[u].x;
+        sum1.y += f1[u] * v1[u].y;
+      }
     }
   }
-  return make_float2(ax, ay);
+  return make_float2(sum0.x + sum1.x, sum0.y + sum1.y);
 }
 
 // Dense levels: one thread per table entry (entries of hashed levels are
