@@ -49,14 +49,38 @@ struct ProbeNorm {
   bool roll, scale;
 };
 
+// Sum of the nb (<= kProbeBlocks) partials part[0..nb) by the first
+// wavefront: lane l adds partials l, l + 64, ... (its loads all in flight
+// together), then a fixed xor butterfly; the total is returned on every lane
+// of that wavefront.  The order is fixed, so the sum is deterministic.
+__device__ __forceinline__ double wave_sum_partials(const double* part, int nb) {
+  const int lane = threadIdx.x & 63;
+  double v[kProbeBlocks / 64];
+#pragma unroll
+  for (int u = 0; u < kProbeBlocks / 64; ++u) {
+    const int b = lane + 64 * u;
+    v[u] = b < nb ? part[b] : 0.0;
+  }
+  double acc = v[0];
+#pragma unroll
+  for (int u = 1; u < kProbeBlocks / 64; ++u) acc += v[u];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  return acc;
+}
+
 // Every block reduces the partials itself (a few hundred doubles) so no
 // extra launch is needed.
 __device__ ProbeNorm probe_norm_from(const double* part, int nb, int M, int N) {
   __shared__ double tot[3];
-  if (threadIdx.x < 3) {
-    double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += part[threadIdx.x * nb + b];
-    tot[threadIdx.x] = acc;
+  if (threadIdx.x < 64) {
+    const double e = wave_sum_partials(part, nb), sy = wave_sum_partials(part + nb, nb),
+                 sx = wave_sum_partials(part + 2 * nb, nb);
+    if (threadIdx.x == 0) {
+      tot[0] = e;
+      tot[1] = sy;
+      tot[2] = sx;
+    }
   }
   __syncthreads();
   ProbeNorm r;
@@ -130,10 +154,9 @@ __global__ __launch_bounds__(256) void k_probe_bwd_apply(const float2* __restric
                                                          const double* __restrict__ part, int nb,
                                                          float2* __restrict__ gp) {
   __shared__ double dot;
-  if (threadIdx.x == 0) {
-    double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += part[b];
-    dot = acc;
+  if (threadIdx.x < 64) {
+    const double acc = wave_sum_partials(part, nb);
+    if (threadIdx.x == 0) dot = acc;
   }
   __syncthreads();
   const int cp0 = (int)stats[0], cp1 = (int)stats[1];
