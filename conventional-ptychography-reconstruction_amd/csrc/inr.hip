@@ -84,6 +84,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(PtyMlp m, const float* __r
 //   wo / bo : += t
 constexpr int kFinishCols = 32;
 constexpr int kFinishGroups = 8;
+constexpr int kFinishBatch = 32;
 
 template <int H>
 __global__ __launch_bounds__(256) void k_inr_grad_finish(const float* __restrict__ partial, int nblocks, PtyMlp m,
@@ -95,8 +96,18 @@ __global__ __launch_bounds__(256) void k_inr_grad_finish(const float* __restrict
   const int t = blockIdx.x * kFinishCols + c;
   float acc = 0.f;
   if (t < GL::total) {
-#pragma unroll 8
-    for (int b = rg; b < nblocks; b += kFinishGroups) acc += partial[(int64_t)b * GL::total + t];
+    // batches of kFinishBatch partials with their loads in flight together,
+    // added in workgroup order (a missing one adds 0)
+    for (int b0 = rg; b0 < nblocks; b0 += kFinishBatch * kFinishGroups) {
+      float v[kFinishBatch];
+#pragma unroll
+      for (int u = 0; u < kFinishBatch; ++u) {
+        const int b = b0 + u * kFinishGroups;
+        v[u] = b < nblocks ? partial[(int64_t)b * GL::total + t] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < kFinishBatch; ++u) acc += v[u];
+    }
   }
   sums[rg][c] = acc;
   __syncthreads();

@@ -17,18 +17,31 @@ constexpr int kProbeBlocks = 256;
 // i mod n for i in [0, 2n) (a rolled index: i < n plus a shift in [0, n))
 __device__ __forceinline__ int wrap(int i, int n) { return i < n ? i : i - n; }
 
+// The pixel loops are grid-strided with kPix pixels per thread and batch:
+// the batch's loads are issued together, then its pixels are processed in
+// order (the grids give at most kPix pixels per thread up to 512 x 512).
+constexpr int kPix = 4;
+
 __global__ __launch_bounds__(256) void k_probe_stats(const float2* __restrict__ p, int M, int N,
                                                      double* __restrict__ part) {
   __shared__ double red[3][256];
   double e = 0.0, sy = 0.0, sx = 0.0;
-  const int total = M * N;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const float2 v = p[t];
-    const double a = (double)(v.x * v.x + v.y * v.y);
-    const int i = t / N, j = t - i * N;
-    e += a;
-    sy += a * i;
-    sx += a * j;
+  const int total = M * N, stride = gridDim.x * blockDim.x;
+  for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < total; t0 += kPix * stride) {
+    float2 v[kPix];
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) v[u] = p[min(t0 + u * stride, total - 1)];
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) {
+      const int t = t0 + u * stride;
+      if (t < total) {
+        const double a = (double)(v[u].x * v[u].x + v[u].y * v[u].y);
+        const int i = t / N, j = t - i * N;
+        e += a;
+        sy += a * i;
+        sx += a * j;
+      }
+    }
   }
   red[0][threadIdx.x] = e;
   red[1][threadIdx.x] = sy;
@@ -118,11 +131,20 @@ __global__ __launch_bounds__(256) void k_probe_apply(const float2* __restrict__ 
     stats[6] = 0.f;
     stats[7] = 0.f;
   }
-  const int total = M * N;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = t / N, j = t - i * N;
-    const int si = wrap(i + pn.cp0, M), sj = wrap(j + pn.cp1, N);
-    out[t] = cscale(p[si * N + sj], pn.s);
+  const int total = M * N, stride = gridDim.x * blockDim.x;
+  for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < total; t0 += kPix * stride) {
+    float2 v[kPix];
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) {
+      const int t = min(t0 + u * stride, total - 1);
+      const int i = t / N, j = t - i * N;
+      v[u] = p[wrap(i + pn.cp0, M) * N + wrap(j + pn.cp1, N)];
+    }
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) {
+      const int t = t0 + u * stride;
+      if (t < total) out[t] = cscale(v[u], pn.s);
+    }
   }
 }
 
@@ -133,12 +155,19 @@ __global__ __launch_bounds__(256) void k_probe_bwd_dot(const float2* __restrict_
   __shared__ double red[256];
   const int cp0 = (int)stats[0], cp1 = (int)stats[1];
   double acc = 0.0;
-  const int total = M * N;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = t / N, j = t - i * N;
-    const float2 q = p[wrap(i + cp0, M) * N + wrap(j + cp1, N)];
-    const float2 g = gn[t];
-    acc += (double)(g.x * q.x + g.y * q.y);
+  const int total = M * N, stride = gridDim.x * blockDim.x;
+  for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < total; t0 += kPix * stride) {
+    float2 q[kPix], g[kPix];
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) {
+      const int t = min(t0 + u * stride, total - 1);
+      const int i = t / N, j = t - i * N;
+      q[u] = p[wrap(i + cp0, M) * N + wrap(j + cp1, N)];
+      g[u] = gn[t];
+    }
+#pragma unroll
+    for (int u = 0; u < kPix; ++u)
+      if (t0 + u * stride < total) acc += (double)(g[u].x * q[u].x + g[u].y * q[u].y);
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -163,16 +192,26 @@ __global__ __launch_bounds__(256) void k_probe_bwd_apply(const float2* __restric
   const bool scale = stats[5] != 0.f;
   const float s = stats[2], E = stats[3];
   const float coef = scale ? (float)(dot * (double)s / (double)E) : 0.f;
-  const int total = M * N;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = t / N, j = t - i * N;
-    const int src = wrap(i + cp0, M) * N + wrap(j + cp1, N);
-    const float2 q = p[src];
-    const float2 g = gn[t];
-    float2 r;
-    if (scale) r = make_float2(s * g.x - coef * q.x, s * g.y - coef * q.y);
-    else r = g;
-    gp[src] = r;
+  const int total = M * N, stride = gridDim.x * blockDim.x;
+  for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < total; t0 += kPix * stride) {
+    float2 q[kPix], g[kPix];
+    int src[kPix];
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) {
+      const int t = min(t0 + u * stride, total - 1);
+      const int i = t / N, j = t - i * N;
+      src[u] = wrap(i + cp0, M) * N + wrap(j + cp1, N);
+      q[u] = p[src[u]];
+      g[u] = gn[t];
+    }
+#pragma unroll
+    for (int u = 0; u < kPix; ++u) {
+      if (t0 + u * stride >= total) continue;
+      float2 r;
+      if (scale) r = make_float2(s * g[u].x - coef * q[u].x, s * g[u].y - coef * q[u].y);
+      else r = g[u];
+      gp[src[u]] = r;
+    }
   }
 }
 
