@@ -495,15 +495,18 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
     const int res = (int)g.res[l];
     const uint32_t size = g.size[l];
     const float2* Al = A + L.off_a[l];
-    const int64_t top = (int64_t)res * res + res;  // largest cx + cy*res
+    // 32-bit index math: dense levels have res^2 <= T <= 2^28, so every
+    // target below is under 2^29 (64-bit divisions cost several times more)
+    const uint32_t ures = (uint32_t)res;
+    const uint32_t top = ures * ures + ures;  // largest cx + cy*res
     for (int k = 0; k < 2; ++k) {
-      const int64_t target = (int64_t)e + (int64_t)k * size;
+      const uint32_t target = e + (uint32_t)k * size;
       if (target > top) break;
       // cy with cx = target - cy*res in [0, res]
-      const int64_t cy_lo = target > res ? (target - res + res - 1) / res : 0;
-      const int64_t cy_hi = target / res;
-      for (int64_t cy = cy_lo; cy <= cy_hi && cy <= res; ++cy) {
-        const float2 v = corner_from_A(L, P, l, res, Al, (int)(target - cy * res), (int)cy, s, S);
+      const uint32_t cy_lo = target > ures ? (target - 1u) / ures : 0u;
+      const uint32_t cy_hi = target / ures;
+      for (uint32_t cy = cy_lo; cy <= cy_hi && cy <= ures; ++cy) {
+        const float2 v = corner_from_A(L, P, l, res, Al, (int)(target - cy * ures), (int)cy, s, S);
         acc.x += v.x;
         acc.y += v.y;
       }
