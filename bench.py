@@ -28,7 +28,12 @@ recompute, k_inr_bwd16 in FP32, launched once for the object grid and once for t
 probe grid per step; its average launch duration from the events of region
 (1)), its HBM traffic from the committed rocprofv3 PMC summary, and the CPU
 baseline (the FP32 oracle restatement on the host cores, a bounded sample of
-steps).
+steps).  With the CPU baseline the line also carries `nrmse_leg`, the
+wall-clock half of the metric measured after the timed regions on the
+canvas-256 shapes (the plain PyTorch loop takes about 10 s per epoch at
+canvas 1024): the plain loop for --nrmse-epochs epochs, then the fused engine
+from the same initial model until it reaches the plain run's final object
+NRMSE (within 1 %).
 """
 from __future__ import annotations
 
