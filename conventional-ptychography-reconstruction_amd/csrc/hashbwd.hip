@@ -485,12 +485,21 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
   const int S = split ? kDenseSplit : 1;
   const bool active = t < n_dense_entries;
   const bool owner = active && s == 0;
+  // Adam's step coefficients are uniform: formed once per workgroup
+  __shared__ AdamCoef s_coef;
+  if (ad.p && threadIdx.x == 0) s_coef = adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps);
   EntryAdam st{};
   if (ad.p && owner) st = table_adam_load(ad, t);
+  // level: the wave's first entry's by a uniform search, then one step for
+  // the lanes past a level boundary (a wave spans at most 64 entries, every
+  // dense level holds more)
+  const uint32_t t_first = __builtin_amdgcn_readfirstlane(min(t, n_dense_entries - 1));
+  int l0 = 0;
+  while (l0 + 1 < g.n_levels && t_first >= g.offset[l0 + 1]) ++l0;
+  const uint32_t next_off = l0 + 1 < g.n_levels ? g.offset[l0 + 1] : 0xFFFFFFFFu;
   float2 acc = make_float2(0.f, 0.f);
   if (active) {
-    int l = 0;
-    while (l + 1 < g.n_levels && t >= g.offset[l + 1]) ++l;
+    const int l = t >= next_off ? l0 + 1 : l0;
     const uint32_t e = t - g.offset[l];
     const int res = (int)g.res[l];
     const uint32_t size = g.size[l];
@@ -520,10 +529,11 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
     sum.x += __shfl_xor(sum.x, d);
     sum.y += __shfl_xor(sum.y, d);
   }
+  __syncthreads();   // s_coef
   if (!owner) return;
   if (split) acc = sum;
   if (ad.p) {
-    table_adam_store(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), t, st, acc);
+    table_adam_store(ad, s_coef, t, st, acc);
   } else if (acc.x != 0.f || acc.y != 0.f) {
     float2 d = dtable[t];
     d.x += acc.x;
@@ -667,9 +677,11 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
                                                       float2* __restrict__ dtable, TableAdam ad) {
   __shared__ int s_off[257];
   __shared__ float2 s_val[kHashWin];
+  __shared__ AdamCoef s_coef;   // Adam's step coefficients, formed once per workgroup
   int l;
   uint32_t chunk;
   if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;   // uniform across the workgroup
+  if (ad.p && threadIdx.x == 0) s_coef = adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps);
   const int tid = threadIdx.x;
   const uint32_t e0 = chunk * 256;
   const int ne = (int)min(256u, g.size[l] - e0);
@@ -722,7 +734,7 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
     __syncthreads();
   }
   if (tid < ne && ad.p) {
-    table_adam_store(ad, adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps), g.offset[l] + e0 + tid, st, make_float2(ax, ay));
+    table_adam_store(ad, s_coef, g.offset[l] + e0 + tid, st, make_float2(ax, ay));
   } else if (tid < ne && (ax != 0.f || ay != 0.f)) {
     const uint32_t t = g.offset[l] + e0 + tid;
     float2 d = dtable[t];
