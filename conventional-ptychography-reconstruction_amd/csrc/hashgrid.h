@@ -101,12 +101,16 @@ __device__ __forceinline__ LevelPos level_pos_c(const LevelC& c, float x0, float
 }
 
 // level_index with the level's constants in registers; both index forms are
-// computed and selected (the full modulo only runs for inputs outside [0, 1]).
+// computed and selected.  For inputs in [0, 1] the dense index is below
+// 2 size, so one conditional subtraction (no branch) is the modulo; the full
+// modulo runs only when some lane of the wavefront has an input outside
+// [0, 1] (a wave-uniform branch: no per-lane exec-mask nesting in the common
+// path).  (dl - size) mod size = dl mod size, so the two steps compose.
 __device__ __forceinline__ uint32_t level_index_c(const LevelC& c, uint32_t cx, uint32_t cy) {
   uint32_t dl = cx + cy * c.res;
-  if (dl >= c.size) {
-    if (dl < 2u * c.size) dl -= c.size;
-    else dl %= c.size;
+  dl = dl >= c.size ? dl - c.size : dl;
+  if (__builtin_amdgcn_ballot_w64(dl >= c.size)) {
+    if (dl >= c.size) dl %= c.size;
   }
   const uint32_t hl = (cx ^ (cy * 2654435761u)) & (c.size - 1u);
   return c.offset + (c.dense ? dl : hl);

@@ -20,40 +20,43 @@ __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
 __device__ __forceinline__ int unit16(int s, int g) { return 16 * (s >> 2) + 4 * g + (s & 3); }
 __device__ __forceinline__ int feat16(int c, int g) { return 2 * (4 * (c >> 1) + g) + (c & 1); }
 
-// Level constants for lane group g at level 4 m + g: the four candidates are
-// uniform kernel-argument values, the lane selects one.
-__device__ __forceinline__ LevelC level_c4(const PtyGrid& gr, int m, int g) {
-  const int l0 = 4 * m;
-  float sc[4];
-  uint32_t rs[4], sz[4], of[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    sc[q] = __uint_as_float(uniform_u32(__float_as_uint(gr.scale[l0 + q])));
-    rs[q] = uniform_u32(gr.res[l0 + q]);
-    sz[q] = uniform_u32(gr.size[l0 + q]);
-    of[q] = uniform_u32(gr.offset[l0 + q]);
-  }
-  const uint32_t dm = uniform_u32(gr.dense_mask);
+// The encoder's per-level constants (scale bits, res, size, offset) staged in
+// LDS once per workgroup: lane group g reads level 4 m + g's entry (four
+// distinct addresses per read, a broadcast).  Kept in LDS rather than as
+// uniform kernel-argument values, which the compiler had to hold in 64 SGPRs
+// (spilling some into VGPR lanes inside the tile loop) and select per lane.
+constexpr int kLevelTab = 16;
+
+__device__ __forceinline__ void load_level_tab(const PtyGrid& gr, uint4* tab) {
+  const int l = threadIdx.x;
+  if (l < kLevelTab)
+    tab[l] = l < gr.n_levels ? make_uint4(__float_as_uint(gr.scale[l]), gr.res[l], gr.size[l], gr.offset[l])
+                             : make_uint4(0u, 0u, 0u, 0u);
+}
+
+__device__ __forceinline__ LevelC level_c_tab(const uint4* tab, uint32_t dense_mask, int l) {
+  const uint4 q = tab[l];
   LevelC c;
-  c.scale = g == 0 ? sc[0] : g == 1 ? sc[1] : g == 2 ? sc[2] : sc[3];
-  c.res = g == 0 ? rs[0] : g == 1 ? rs[1] : g == 2 ? rs[2] : rs[3];
-  c.size = g == 0 ? sz[0] : g == 1 ? sz[1] : g == 2 ? sz[2] : sz[3];
-  c.offset = g == 0 ? of[0] : g == 1 ? of[1] : g == 2 ? of[2] : of[3];
-  c.dense = (dm >> (l0 + g)) & 1u;
+  c.scale = __uint_as_float(q.x);
+  c.res = q.y;
+  c.size = q.z;
+  c.offset = q.w;
+  c.dense = (dense_mask >> l) & 1u;
   return c;
 }
 
 // The lane's 8 features feat[c] = feature feat16(c, g): the 16 corner indices
 // are formed first and the 16 gathers issued back to back.
-__device__ __forceinline__ void encode16(const PtyGrid& gr, float x0, float x1, int g,
+__device__ __forceinline__ void encode16(const PtyGrid& gr, const uint4* lvl, float x0, float x1, int g,
                                          const float2* __restrict__ table, float feat[8]) {
   uint32_t idx[4][4];
   float fx[4], fy[4];
+  const uint32_t dm = __builtin_amdgcn_readfirstlane(gr.dense_mask);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const int l = 4 * m + g;
     if (l < gr.n_levels) {
-      const LevelC c = level_c4(gr, m, g);
+      const LevelC c = level_c_tab(lvl, dm, l);
       const LevelPos p = level_pos_c(c, x0, x1);
       corner_indices_c(c, p, idx[m]);
       fx[m] = p.fx;

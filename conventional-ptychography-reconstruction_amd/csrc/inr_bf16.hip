@@ -234,7 +234,9 @@ __global__ __launch_bounds__(64 * kB16FwdWaves) void k_inr_fwd_b16(PtyGrid gr, P
                                                                   const float2* __restrict__ table, PtyPoints pts,
                                                                   float2* __restrict__ out) {
   __shared__ B16Fwd<H> o;
+  __shared__ uint4 s_lvl[kLevelTab];
   build_fwd16<H>(o, m, threadIdx.x, blockDim.x);
+  load_level_tab(gr, s_lvl);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(64 * kB16FwdWaves) void k_inr_fwd_b16(PtyGrid gr, P
     float x0 = 0.5f, x1 = 0.5f;
     if (valid) point_coords(pts, p, x0, x1);
     float feat[8];
-    encode16(gr, x0, x1, g, table, feat);
+    encode16(gr, s_lvl, x0, x1, g, table, feat);
     if (pts.features) store_features16(pts.features, tile, lane, feat);
     b16x8 zbr[H + 1], zbi[H + 1];
     float zr[8], zi[8];
@@ -335,8 +337,10 @@ __global__ __launch_bounds__(64 * kB16Waves) void k_inr_bwd_b16(PtyGrid gr, PtyM
   __shared__ B16Fwd<H> o;
   __shared__ B16Bwd<H> ob;
   __shared__ B16Stage<H> st[kB16Waves];
+  __shared__ uint4 s_lvl[kLevelTab];
   build_fwd16<H>(o, m, threadIdx.x, blockDim.x);
   build_bwd16<H>(ob, m, threadIdx.x, blockDim.x);
+  load_level_tab(gr, s_lvl);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -372,7 +376,7 @@ __global__ __launch_bounds__(64 * kB16Waves) void k_inr_bwd_b16(PtyGrid gr, PtyM
     } else {
       float x0 = 0.5f, x1 = 0.5f;
       if (valid) point_coords(pts, p, x0, x1);
-      encode16(gr, x0, x1, g, table, feat);
+      encode16(gr, s_lvl, x0, x1, g, table, feat);
     }
     b16x8 zbr[H + 1], zbi[H + 1];
     float zr[8], zi[8];

@@ -142,7 +142,9 @@ __global__ __launch_bounds__(64 * kFwd16Waves, 4) void k_inr_fwd16(PtyGrid gr, P
                                                                 const float2* __restrict__ table, PtyPoints pts,
                                                                 float2* __restrict__ out) {
   __shared__ Lds16<H> s;
+  __shared__ uint4 s_lvl[kLevelTab];
   load16<H>(s, m, threadIdx.x, blockDim.x);
+  load_level_tab(gr, s_lvl);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(64 * kFwd16Waves, 4) void k_inr_fwd16(PtyGrid gr, P
     float x0 = 0.5f, x1 = 0.5f;
     if (valid) point_coords(pts, p, x0, x1);
     float feat[8];
-    encode16(gr, x0, x1, g, table, feat);
+    encode16(gr, s_lvl, x0, x1, g, table, feat);
     if (pts.features) store_features16(pts.features, tile, lane, feat);
     float zr[8], zi[8];
     {
@@ -260,7 +262,9 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
                                                          float* __restrict__ partial) {
   __shared__ Lds16<H> s;
   __shared__ Stage16<H> st[kW16];
+  __shared__ uint4 s_lvl[kLevelTab];
   load16<H>(s, m, threadIdx.x, blockDim.x);
+  load_level_tab(gr, s_lvl);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     } else {
       float x0 = 0.5f, x1 = 0.5f;
       if (valid) point_coords(pts, p, x0, x1);
-      encode16(gr, x0, x1, g, table, feat);
+      encode16(gr, s_lvl, x0, x1, g, table, feat);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) S.F[feat16(c, g) * PS + j] = feat[c];
