@@ -116,6 +116,28 @@ __device__ __forceinline__ uint32_t level_index_c(const LevelC& c, uint32_t cx, 
   return c.offset + (c.dense ? dl : hl);
 }
 
+// level_index_c split by a wave-uniform property of the inputs.  UNIT: every
+// input of the wavefront lies in [0, 1], so (validate_grid requires scale >= 0,
+// floor(scale + 0.5) + 1 <= res and res (res + 1) < 2 size on dense levels)
+// cx, cy <= res, cx + cy res < 2 size and one conditional subtraction is the
+// modulo.  Otherwise the full modulo of the 32-bit wrapped index.
+template <bool UNIT>
+__device__ __forceinline__ uint32_t level_index_u(const LevelC& c, uint32_t cx, uint32_t cy) {
+  uint32_t dl = cx + cy * c.res;
+  if constexpr (UNIT) dl = dl >= c.size ? dl - c.size : dl;
+  else dl %= c.size;
+  const uint32_t hl = (cx ^ (cy * 2654435761u)) & (c.size - 1u);
+  return c.offset + (c.dense ? dl : hl);
+}
+
+template <bool UNIT>
+__device__ __forceinline__ void corner_indices_u(const LevelC& c, const LevelPos& p, uint32_t idx[4]) {
+  idx[0] = level_index_u<UNIT>(c, p.gx, p.gy);
+  idx[1] = level_index_u<UNIT>(c, p.gx + 1u, p.gy);
+  idx[2] = level_index_u<UNIT>(c, p.gx, p.gy + 1u);
+  idx[3] = level_index_u<UNIT>(c, p.gx + 1u, p.gy + 1u);
+}
+
 __device__ __forceinline__ void corner_indices_c(const LevelC& c, const LevelPos& p, uint32_t idx[4]) {
   idx[0] = level_index_c(c, p.gx, p.gy);
   idx[1] = level_index_c(c, p.gx + 1u, p.gy);

@@ -128,12 +128,19 @@ int validate_grid(const PtyGrid* g) {
               "log2_hashmap_size %d outside 3..28", g->log2_hashmap_size);
   for (int l = 0; l < g->n_levels; ++l) {
     const bool dense = (g->dense_mask >> l) & 1u;
+    // inputs in [0, 1] must reach at most corner res on each axis (the
+    // encoders' dense-index modulo relies on it)
+    PTY_REQUIRE(g->scale[l] >= 0.f && floorf(g->scale[l] + 0.5f) + 1.f <= (float)g->res[l], PTY_EINVAL,
+                "level %d: scale %g needs res >= floor(scale + 0.5) + 1 (res %u)", l, (double)g->scale[l],
+                g->res[l]);
     if (!dense)
       PTY_REQUIRE(g->size[l] == (1u << g->log2_hashmap_size), PTY_EINVAL,
                   "hashed level %d must have size 2^log2T", l);
     else
-      PTY_REQUIRE((uint64_t)g->res[l] * g->res[l] <= g->size[l], PTY_EINVAL,
-                  "dense level %d: res^2 > size", l);
+      PTY_REQUIRE((uint64_t)g->res[l] * g->res[l] <= g->size[l] &&
+                      (uint64_t)g->res[l] * (g->res[l] + 1u) < 2ull * g->size[l] &&
+                      (uint64_t)g->res[l] * (g->res[l] + 1u) < (1ull << 32),
+                  PTY_EINVAL, "dense level %d: res^2 > size, res (res + 1) >= 2 size or >= 2^32", l);
   }
   return PTY_OK;
 }

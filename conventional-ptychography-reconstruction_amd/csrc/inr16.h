@@ -31,7 +31,7 @@ __device__ __forceinline__ void load_level_tab(const PtyGrid& gr, uint4* tab) {
   const int l = threadIdx.x;
   if (l < kLevelTab)
     tab[l] = l < gr.n_levels ? make_uint4(__float_as_uint(gr.scale[l]), gr.res[l], gr.size[l], gr.offset[l])
-                             : make_uint4(0u, 0u, 0u, 0u);
+                             : make_uint4(0u, 0u, 1u, 0u);   // scale 0, res 0, size 1: every corner is entry 0
 }
 
 __device__ __forceinline__ LevelC level_c_tab(const uint4* tab, uint32_t dense_mask, int l) {
@@ -45,34 +45,66 @@ __device__ __forceinline__ LevelC level_c_tab(const uint4* tab, uint32_t dense_m
   return c;
 }
 
+// Corner indices and fractions of the lane's levels 4 m + g (m < 4).
+template <bool UNIT>
+__device__ __forceinline__ void corners16(const PtyGrid& gr, const uint4* lvl, uint32_t dm, float x0, float x1, int g,
+                                          uint32_t (&idx)[4][4], float (&fx)[4], float (&fy)[4]) {
+  // Levels past n_levels read the padding entries of the table (scale 0,
+  // res 0, size 1, offset 0), whose corners are all entry 0: in bounds, and
+  // their features are discarded.  No per-level branch, so the four LDS reads
+  // issue together.
+  LevelC c[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c[m] = level_c_tab(lvl, dm, 4 * m + g);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const LevelPos p = level_pos_c(c[m], x0, x1);
+    corner_indices_u<UNIT>(c[m], p, idx[m]);
+    fx[m] = p.fx;
+    fy[m] = p.fy;
+  }
+}
+
 // The lane's 8 features feat[c] = feature feat16(c, g): the 16 corner indices
-// are formed first and the 16 gathers issued back to back.
+// are formed first and the 16 gathers issued back to back.  When every
+// coordinate of the wavefront lies in [0, 1] (the INR grids always do) the
+// dense-level modulo is one conditional subtraction; one uniform test per
+// tile picks that form.
 __device__ __forceinline__ void encode16(const PtyGrid& gr, const uint4* lvl, float x0, float x1, int g,
                                          const float2* __restrict__ table, float feat[8]) {
   uint32_t idx[4][4];
   float fx[4], fy[4];
   const uint32_t dm = __builtin_amdgcn_readfirstlane(gr.dense_mask);
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int l = 4 * m + g;
-    if (l < gr.n_levels) {
-      const LevelC c = level_c_tab(lvl, dm, l);
-      const LevelPos p = level_pos_c(c, x0, x1);
-      corner_indices_c(c, p, idx[m]);
-      fx[m] = p.fx;
-      fy[m] = p.fy;
-    } else {
-      idx[m][0] = idx[m][1] = idx[m][2] = idx[m][3] = 0u;   // in bounds; the result is discarded
-      fx[m] = fy[m] = 0.f;
-    }
-  }
+  const bool in_unit = x0 >= 0.f && x0 <= 1.f && x1 >= 0.f && x1 <= 1.f;
+  if (__builtin_amdgcn_ballot_w64(!in_unit) == 0)
+    corners16<true>(gr, lvl, dm, x0, x1, g, idx, fx, fy);
+  else
+    corners16<false>(gr, lvl, dm, x0, x1, g, idx, fx, fy);
+  // Gathers: below 2^29 entries the byte offset of an entry fits 32 bits, so
+  // the load takes the table base as a scalar and a 32-bit lane offset (no
+  // 64-bit address arithmetic per gather).
   float2 t[4][4];
+  const char* tb = reinterpret_cast<const char*>(table);
+  const bool off32 = gr.n_entries <= (1u << 29);
   if (gr.table_precision == PTY_TABLE_F16) {
     const __half2* t16 = reinterpret_cast<const __half2*>(table);
+    if (off32) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          t[m][c] = __half22float2(*reinterpret_cast<const __half2*>(tb + (idx[m][c] << 2)));
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) t[m][c] = __half22float2(t16[idx[m][c]]);
+    }
+  } else if (off32) {
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) t[m][c] = __half22float2(t16[idx[m][c]]);
+      for (int c = 0; c < 4; ++c) t[m][c] = *reinterpret_cast<const float2*>(tb + (idx[m][c] << 3));
   } else {
 #pragma unroll
     for (int m = 0; m < 4; ++m)

@@ -42,9 +42,14 @@ constexpr int kFwd16Waves = 4;     // waves per forward workgroup
 constexpr int WS36 = 36;           // LDS row stride of the weight matrices (floats)
 constexpr int PS = 16;             // LDS row stride of the per-point staging (16 points)
 
+// W0 is staged with each lane group's layer-0 K values contiguous: row o,
+// position 8 g + c holds W0[o][feat16(c, g)] (zero past n_in), so layer 0
+// reads them as two 16-byte loads per M block.
+__device__ __forceinline__ int w0pos(int f) { return 8 * ((f >> 1) & 3) + 2 * (f >> 3) + (f & 1); }
+
 template <int H>
 struct Lds16 {
-  float w0[32 * WS36];     // W0[o][f], f zero-padded to 32
+  float w0[32 * WS36];     // W0[o][feat16(c, g)] at column 8 g + c
   float ar[H][32 * WS36];  // Re A_k[o][i]
   float ai[H][32 * WS36];  // Im A_k[o][i]
   float b0[32];
@@ -58,7 +63,8 @@ template <int H>
 __device__ void load16(Lds16<H>& s, const PtyMlp& m, int tid, int nt) {
   for (int t = tid; t < 32 * 32; t += nt) {
     const int o = t >> 5, f = t & 31;
-    s.w0[o * WS36 + f] = f < m.n_in ? m.w0[o * m.n_in + f] : 0.f;
+    const int fw = feat16(f & 7, f >> 3);
+    s.w0[o * WS36 + f] = fw < m.n_in ? m.w0[o * m.n_in + fw] : 0.f;
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       const float2 a = ((const float2*)m.wh[k])[t];
@@ -80,6 +86,8 @@ __device__ void load16(Lds16<H>& s, const PtyMlp& m, int tid, int nt) {
   if (tid == 0) { s.bo[0] = m.bo[0]; s.bo[1] = m.bo[1]; }
 }
 
+__device__ __forceinline__ float4 lds4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+
 // u[mb] (units 16 mb + 4 g + v) = b0 + W0 h
 template <int H>
 __device__ __forceinline__ void layer0_16(const Lds16<H>& s, const float feat[8], int i, int g, v4f u[2]) {
@@ -88,9 +96,15 @@ __device__ __forceinline__ void layer0_16(const Lds16<H>& s, const float feat[8]
 #pragma unroll
     for (int v = 0; v < 4; ++v) u[mb][v] = s.b0[16 * mb + 4 * g + v];
 #pragma unroll
-  for (int c = 0; c < 8; ++c)
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) u[mb] = mfma4(s.w0[(16 * mb + i) * WS36 + feat16(c, g)], feat[c], u[mb]);
+    for (int mb = 0; mb < 2; ++mb) {
+      const float4 w = lds4(&s.w0[(16 * mb + i) * WS36 + 8 * g + 4 * h]);
+      u[mb] = mfma4(w.x, feat[4 * h], u[mb]);
+      u[mb] = mfma4(w.y, feat[4 * h + 1], u[mb]);
+      u[mb] = mfma4(w.z, feat[4 * h + 2], u[mb]);
+      u[mb] = mfma4(w.w, feat[4 * h + 3], u[mb]);
+    }
 }
 
 // (vr, vi)[mb] = A_k z + c_k, three real products per K step (Gauss):
@@ -103,17 +117,25 @@ __device__ __forceinline__ void hidden16(const Lds16<H>& s, int k, const float z
   v4f t1[2], t2[2], t3[2];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb) t1[mb] = t2[mb] = t3[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+  // K chunk st = 4 qd + e reads unit16(st, g) = 16 qd + 4 g + e: the four
+  // e of one row are contiguous (one 16-byte load, conflict-free at row
+  // stride 36)
 #pragma unroll
-  for (int st = 0; st < 8; ++st) {
-    const int u = unit16(st, g);
-    const float zs = zr[st] + zi[st];
+  for (int qd = 0; qd < 2; ++qd) {
+    float zs[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) zs[e] = zr[4 * qd + e] + zi[4 * qd + e];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-      const float a_r = s.ar[k][(16 * mb + i) * WS36 + u];
-      const float a_i = s.ai[k][(16 * mb + i) * WS36 + u];
-      t1[mb] = mfma4(a_r, zr[st], t1[mb]);
-      t2[mb] = mfma4(a_i, zi[st], t2[mb]);
-      t3[mb] = mfma4(a_r + a_i, zs, t3[mb]);
+      const float4 r4 = lds4(&s.ar[k][(16 * mb + i) * WS36 + 16 * qd + 4 * g]);
+      const float4 i4 = lds4(&s.ai[k][(16 * mb + i) * WS36 + 16 * qd + 4 * g]);
+      const float a_r[4] = {r4.x, r4.y, r4.z, r4.w}, a_i[4] = {i4.x, i4.y, i4.z, i4.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        t1[mb] = mfma4(a_r[e], zr[4 * qd + e], t1[mb]);
+        t2[mb] = mfma4(a_i[e], zi[4 * qd + e], t2[mb]);
+        t3[mb] = mfma4(a_r[e] + a_i[e], zs[e], t3[mb]);
+      }
     }
   }
 #pragma unroll
@@ -126,11 +148,12 @@ __device__ __forceinline__ void hidden16(const Lds16<H>& s, int k, const float z
 }
 
 __device__ __forceinline__ void act16(float w, const v4f vr[2], const v4f vi[2], float zr[8], float zi[8]) {
+  const float wl = -w * 1.4426950408889634f;   // exp(-w v) = 2^(-w log2(e) v)
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     float c, sn;
     cis(w * vr[s >> 2][s & 3], c, sn);
-    const float mag = exp_f(-w * vi[s >> 2][s & 3]);
+    const float mag = __builtin_amdgcn_exp2f(wl * vi[s >> 2][s & 3]);
     zr[s] = mag * c;
     zi[s] = mag * sn;
   }
@@ -138,7 +161,7 @@ __device__ __forceinline__ void act16(float w, const v4f vr[2], const v4f vi[2],
 
 // ============================================================== forward
 template <int H>
-__global__ __launch_bounds__(64 * kFwd16Waves, 4) void k_inr_fwd16(PtyGrid gr, PtyMlp m,
+__global__ __launch_bounds__(64 * kFwd16Waves, 5) void k_inr_fwd16(PtyGrid gr, PtyMlp m,
                                                                 const float2* __restrict__ table, PtyPoints pts,
                                                                 float2* __restrict__ out) {
   __shared__ Lds16<H> s;
@@ -195,7 +218,6 @@ struct Stage16 {
   float G[64 * PS];      // gradient staging
 };
 
-__device__ __forceinline__ float4 lds4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 __device__ __forceinline__ float sum4(const float4& a) { return (a.x + a.y) + (a.z + a.w); }
 
 // acc += A B over the 16 points: A rows (ra), B rows (rb) staged point-major;
@@ -442,7 +464,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
       for (int q = 0; q < 8; ++q) {
         const int u = unit16(q, g);
 #pragma unroll
-        for (int fb = 0; fb < 2; ++fb) gh[fb] = mfma4(s.w0[u * WS36 + 16 * fb + j], gu[q], gh[fb]);
+        for (int fb = 0; fb < 2; ++fb) gh[fb] = mfma4(s.w0[u * WS36 + w0pos(16 * fb + j)], gu[q], gh[fb]);
       }
       __builtin_amdgcn_s_setprio(0);
       wave_sync();

@@ -133,8 +133,10 @@ def hash_corners(coords_f32: np.ndarray, table: dict):
     pos = fma_f32(x, scale, 0.5) (one rounding, the form of tiny-cuda-nn's
     pos_fract), g = floor(pos), frac = pos - g (exact in f32).  Corner c has
     offsets (c&1, c>>1).
-    Dense index (cx + cy*res) % size, hashed index
-    ((cx * 1) ^ (cy * 2654435761)) mod 2^32 % size.
+    Corner coordinates are uint32 (tiny-cuda-nn's pos_grid is the uint32 of
+    (int)floor(pos)): dense index ((cx + cy*res) mod 2^32) % size, hashed
+    index ((cx * 1) ^ (cy * 2654435761)) mod 2^32 % size.  For inputs in
+    [0, 1] cx, cy >= 0 and the dense index is plainly (cx + cy*res) % size.
     """
     x = np.asarray(coords_f32, np.float32)
     N = x.shape[0]
@@ -152,7 +154,7 @@ def hash_corners(coords_f32: np.ndarray, table: dict):
             cx = gi[:, 0] + (c & 1)
             cy = gi[:, 1] + ((c >> 1) & 1)
             if table["dense"][l]:
-                local = (cx + cy * res) % size
+                local = (((cx & 0xFFFFFFFF) + (cy & 0xFFFFFFFF) * res) & 0xFFFFFFFF) % size
             else:
                 h = ((cx & 0xFFFFFFFF) ^ ((cy * HASH_PRIME_Y) & 0xFFFFFFFF)) & 0xFFFFFFFF
                 local = h % size

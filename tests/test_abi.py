@@ -34,6 +34,19 @@ def test_errors_are_reported_without_a_gpu(pkg):
     d.n_features = 3
     with pytest.raises(RuntimeError, match="n_features_per_level must be 2"):
         pkg._lib.call("pty_hashgrid_fwd", d, 1, 1, 1, 1, None)
+    # every corner of an input in [0, 1] must stay within res on each axis
+    d = spec.descriptor()
+    d.scale[3] = float(d.res[3])          # floor(res + 0.5) + 1 > res
+    with pytest.raises(RuntimeError, match="level 3: scale"):
+        pkg._lib.call("pty_hashgrid_fwd", d, 1, 1, 1, 1, None)
+    d = spec.descriptor()
+    d.scale[0] = -1.0
+    with pytest.raises(RuntimeError, match="level 0: scale"):
+        pkg._lib.call("pty_hashgrid_fwd", d, 1, 1, 1, 1, None)
+    d = spec.descriptor()
+    d.scale[0], d.res[0], d.size[0] = 0.0, 1, 1   # res (res + 1) = 2 = 2 size
+    with pytest.raises(RuntimeError, match="dense level 0"):
+        pkg._lib.call("pty_hashgrid_fwd", d, 1, 1, 1, 1, None)
     with pytest.raises(RuntimeError, match="prime factors"):
         pkg._lib.call("pty_upsample_fwd", 1, 67, 2, 1, 1, 1 << 30, None)   # 67 is prime > 64
     far = pkg.farfield.far_descriptor(64, 64, 256, 256, 10, 100, "FD_loss")

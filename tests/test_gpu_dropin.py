@@ -109,16 +109,18 @@ def test_complex_mlp_forward_and_complex_output(pkg, dev, hidden_layers):
         assert rel_err(mlp.net(h), ref.detach()) < 1e-5
 
 
-def test_forward_on_edited_coordinates(pkg, dev):
+@pytest.mark.parametrize("lo,hi", [(0.0, 1.0), (-0.25, 1.25)])
+def test_forward_on_edited_coordinates(pkg, dev, lo, hi):
     """A caller who edits xy_coords gets the INR on those coordinates (the
-    reference evaluates the encoder on the stored tensor, complex_inr.py:279)."""
+    reference evaluates the encoder on the stored tensor, complex_inr.py:279),
+    also outside [0, 1] (the encoder's general dense-index modulo)."""
     torch.manual_seed(5)
     model = pkg.ComplexINRModel2D(64, 64, 2, probe_width=16, probe_height=16, log2_hashmap_size=12,
                                   hidden_omega_0=1.0, trainable_omega0=True).to(dev)
     with torch.no_grad():
         model.xy_hash_encoder.params.uniform_(-1, 1)
         g = torch.Generator(device="cpu").manual_seed(6)
-        new = torch.rand(model.xy_coords.shape, generator=g).to(dev)
+        new = (lo + (hi - lo) * torch.rand(model.xy_coords.shape, generator=g)).to(dev)
         model.xy_coords.copy_(new)
     assert not model.coords_are_grid("object") and model.coords_are_grid("probe")
     obj, _ = model()

@@ -47,7 +47,8 @@ def test_hash_indices_bit_exact(pkg, dev, log2T):
     xb = boundary_coords(table, per_level=20, seed=log2T)
     parts = [rng.random((4000, 2)), orc.grid_coords(128, 128).numpy()[::7],
              np.stack([xb, rng.permutation(xb)], 1), np.stack([rng.random(xb.size), xb], 1),
-             np.array([[0.0, 0.0], [0.999999, 0.999999], [0.5, 0.25]])]
+             np.array([[0.0, 0.0], [0.999999, 0.999999], [0.5, 0.25]]),
+             rng.uniform(-0.3, 1.3, (2000, 2))]   # outside [0, 1]: uint32 corner wrap
     if log2T == 20:
         g = orc.grid_coords(1824, 1824).numpy().reshape(1824, 1824, 2)
         parts.append(g[::97].reshape(-1, 2))

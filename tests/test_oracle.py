@@ -96,7 +96,7 @@ def _brute_indices(x, table):
             cx, cy = g[0] + (c & 1), g[1] + ((c >> 1) & 1)
             res, size = int(table["res"][l]), int(table["size"][l])
             if res * res <= size:
-                idx = (cx + cy * res) % size
+                idx = ((cx % 2**32 + (cy % 2**32) * res) % 2**32) % size
             else:
                 idx = ((cx * 1) % 2**32 ^ (cy * 2654435761) % 2**32) % size
             row.append(int(table["offset"][l]) + idx)
@@ -150,6 +150,24 @@ def test_hash_indices_brute_force_at_cell_boundaries():
     idx, _ = orc.hash_corners(pts, table)
     for n in range(pts.shape[0]):
         assert idx[n].tolist() == _brute_indices(pts[n], table)
+
+
+def test_hash_indices_brute_force_outside_unit_square():
+    """Inputs outside [0, 1]: the corners are uint32 (negative cells wrap
+    modulo 2^32), which differs from a plain modulo when the level size does
+    not divide 2^32."""
+    table = orc.level_table(16, 14, 16, 1.5)
+    rng = np.random.default_rng(6)
+    pts = rng.uniform(-0.3, 1.3, (60, 2)).astype(np.float32)
+    idx, _ = orc.hash_corners(pts, table)
+    for n in range(pts.shape[0]):
+        assert idx[n].tolist() == _brute_indices(pts[n], table)
+    # level 1: scale 23, res 24, size 576 (dense); x = (-0.2, -0.2) lies in cell
+    # -5 on both axes: (-5 - 5*24) mod 2^32 = 2^32 - 125, and 2^32 = 256 mod 576
+    assert (table["scale"][1], table["res"][1], table["size"][1]) == (23.0, 24, 576)
+    i, _ = orc.hash_corners(np.array([[-0.2, -0.2]], np.float32), table)
+    assert i[0, 1, 0] == table["offset"][1] + 131
+    assert (-125) % 576 == 451   # the plain modulo would give another entry
 
 
 def test_hash_dense_edge_corner_wraps_into_next_row():
