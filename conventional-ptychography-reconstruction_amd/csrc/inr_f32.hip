@@ -213,10 +213,31 @@ __global__ __launch_bounds__(64 * kFwd16Waves, 5) void k_inr_fwd16(PtyGrid gr, P
 // ============================================================== backward
 template <int H>
 struct Stage16 {
-  float X[H][64 * PS];   // input of hidden layer k: rows 0..31 Re, 32..63 Im; column = point
-  float F[32 * PS];      // features, feature-major
+  // rows of 16 points, element (row, point) at st_elem(row, point)
+  float X[H][64 * PS];   // input of hidden layer k: rows 0..31 Re, 32..63 Im
+  float F[32 * PS];      // features, one row per feature
   float G[64 * PS];      // gradient staging
 };
+
+// Per-wave staging rows of 16 points.  Element (row r, point p) lives at
+//   16 P(r) + 4 ((p >> 2) ^ hq(P(r))) + (p & 3),  P(r) = r ^ ((r >> 2) & 3),
+//   hq(x) = ((x >> 1) & 3) ^ ((x >> 3) & 1)
+// (a permutation inside each aligned 4-row group and of the four point quads
+// of a row).  The 16-byte reads of rows 16 b + j (lane j) at quad g are then
+// conflict-free (as are those of row = lane), and the C-layout writes of rows
+// 4 g + c (lane group g) fall into distinct bank groups; unpermuted, both had
+// 4-way bank conflicts.  The quad a read returns holds points 4 g .. 4 g + 3
+// in order, so the sums are unchanged.
+static_assert(PS == 16, "the staging permutation assumes 16-point rows");
+__device__ __forceinline__ int st_hq(int x) { return ((x >> 1) & 3) ^ ((x >> 3) & 1); }
+__device__ __forceinline__ int st_elem(int r, int p) {
+  const int P = r ^ ((r >> 2) & 3);
+  return 16 * P + 4 * ((p >> 2) ^ st_hq(P)) + (p & 3);
+}
+__device__ __forceinline__ int st_quad(int r, int q) {
+  const int P = r ^ ((r >> 2) & 3);
+  return 16 * P + 4 * (q ^ st_hq(P));
+}
 
 __device__ __forceinline__ float sum4(const float4& a) { return (a.x + a.y) + (a.z + a.w); }
 
@@ -291,6 +312,16 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
   Stage16<H>& S = st[wave];
+  // Loop-invariant staging offsets of this lane (rows 16 apart differ by 256):
+  // wrow[e] = (row 4 g + e, point j), rrow = (row j, quad g), frow[e] =
+  // (row 8 (e >> 1) + 2 g + (e & 1), point j).
+  int wrow[4], frow[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    wrow[e] = st_elem(4 * g + e, j);
+    frow[e] = st_elem(8 * (e >> 1) + 2 * g + (e & 1), j);
+  }
+  const int rrow = st_quad(j, g);
 
   // P_k[mb][nb] (re, im): rows 16 mb + 4 g + v (out units), column 16 nb + j (in units)
   v4f Pr[H][2][2], Pi[H][2][2];
@@ -329,7 +360,7 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
       encode16(gr, s_lvl, x0, x1, g, table, feat);
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S.F[feat16(c, g) * PS + j] = feat[c];
+    for (int c = 0; c < 8; ++c) S.F[256 * (c >> 2) + frow[c & 3]] = feat[c];
     float zr[8], zi[8];
     // The two waves of a SIMD alternate MFMA blocks with transcendental
     // activation work; a wave raises its issue priority while it feeds the
@@ -347,8 +378,8 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     for (int k = 0; k < H; ++k) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        S.X[k][unit16(q, g) * PS + j] = zr[q];
-        S.X[k][(32 + unit16(q, g)) * PS + j] = zi[q];
+        S.X[k][256 * (q >> 2) + wrow[q & 3]] = zr[q];
+        S.X[k][512 + 256 * (q >> 2) + wrow[q & 3]] = zi[q];
       }
       v4f vr[2], vi[2];
       __builtin_amdgcn_s_setprio(2);
@@ -365,15 +396,15 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     for (int q = 0; q < 8; ++q) {
       const int u = unit16(q, g);
       // g_wo[u] += g_o conj(z_u); g_z = conj(wo_u) g_o
-      S.G[u * PS + j] = go.x * zr[q] + go.y * zi[q];
-      S.G[(32 + u) * PS + j] = go.y * zr[q] - go.x * zi[q];
+      S.G[256 * (q >> 2) + wrow[q & 3]] = go.x * zr[q] + go.y * zi[q];
+      S.G[512 + 256 * (q >> 2) + wrow[q & 3]] = go.y * zr[q] - go.x * zi[q];
       gzr[q] = s.wor[u] * go.x + s.woi[u] * go.y;
       gzi[q] = s.wor[u] * go.y - s.woi[u] * go.x;
     }
     wave_sync();
     // lane l sums staged row l (Re of unit l for l < 32, else Im of unit l - 32)
-    acc_wo += (sum4(lds4(&S.G[lane * PS])) + sum4(lds4(&S.G[lane * PS + 4]))) +
-              (sum4(lds4(&S.G[lane * PS + 8])) + sum4(lds4(&S.G[lane * PS + 12])));
+    acc_wo += (sum4(lds4(&S.G[st_quad(lane, 0)])) + sum4(lds4(&S.G[st_quad(lane, 1)]))) +
+              (sum4(lds4(&S.G[st_quad(lane, 2)])) + sum4(lds4(&S.G[st_quad(lane, 3)])));
     wave_sync();
     // ---------------- hidden layers, last to first
 #pragma unroll
@@ -385,23 +416,23 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
         gwr[q] = gzr[q] * zr[q] + gzi[q] * zi[q];
         gwi[q] = gzi[q] * zr[q] - gzr[q] * zi[q];
         const int u = unit16(q, g);
-        S.G[u * PS + j] = gwr[q];
-        S.G[(32 + u) * PS + j] = gwi[q];
+        S.G[256 * (q >> 2) + wrow[q & 3]] = gwr[q];
+        S.G[512 + 256 * (q >> 2) + wrow[q & 3]] = gwi[q];
       }
       wave_sync();
       // P += G X^H: Pr = Gr Xr^T + Gi Xi^T, Pi = Gi Xr^T - Gr Xi^T
       __builtin_amdgcn_s_setprio(2);
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
-        const float4 ar = lds4(&S.G[(16 * mb + j) * PS + 4 * g]);
-        const float4 ai = lds4(&S.G[(32 + 16 * mb + j) * PS + 4 * g]);
+        const float4 ar = lds4(&S.G[256 * mb + rrow]);
+        const float4 ai = lds4(&S.G[512 + 256 * mb + rrow]);
         qr[k][mb] += sum4(ar);
         qi[k][mb] += sum4(ai);
         const float4 nar = neg4(ar);
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-          const float4 xr = lds4(&S.X[k][(16 * nb + j) * PS + 4 * g]);
-          const float4 xi = lds4(&S.X[k][(32 + 16 * nb + j) * PS + 4 * g]);
+          const float4 xr = lds4(&S.X[k][256 * nb + rrow]);
+          const float4 xi = lds4(&S.X[k][512 + 256 * nb + rrow]);
           Pr[k][mb][nb] = mfma_pts(ar, xr, Pr[k][mb][nb]);
           Pr[k][mb][nb] = mfma_pts(ai, xi, Pr[k][mb][nb]);
           Pi[k][mb][nb] = mfma_pts(ai, xr, Pi[k][mb][nb]);
@@ -433,8 +464,8 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
         const float a1 = t1[q >> 2][q & 3], a2 = t2[q >> 2][q & 3], a3 = t3[q >> 2][q & 3];
         gzr[q] = w * ((a3 - a1) + a2);
         gzi[q] = -w * (a1 + a2);
-        zr[q] = S.X[k][unit16(q, g) * PS + j];
-        zi[q] = S.X[k][(32 + unit16(q, g)) * PS + j];
+        zr[q] = S.X[k][256 * (q >> 2) + wrow[q & 3]];
+        zi[q] = S.X[k][512 + 256 * (q >> 2) + wrow[q & 3]];
       }
       wave_sync();
     }
@@ -445,16 +476,16 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         gu[q] = gzi[q] * zr[q] - gzr[q] * zi[q];
-        S.G[unit16(q, g) * PS + j] = gu[q];
+        S.G[256 * (q >> 2) + wrow[q & 3]] = gu[q];
       }
       wave_sync();
       __builtin_amdgcn_s_setprio(2);
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
-        const float4 a = lds4(&S.G[(16 * mb + j) * PS + 4 * g]);
+        const float4 a = lds4(&S.G[256 * mb + rrow]);
         q0[mb] += sum4(a);
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) P0[mb][nb] = mfma_pts(a, lds4(&S.F[(16 * nb + j) * PS + 4 * g]), P0[mb][nb]);
+        for (int nb = 0; nb < 2; ++nb) P0[mb][nb] = mfma_pts(a, lds4(&S.F[256 * nb + rrow]), P0[mb][nb]);
       }
       // g_h (features 16 fb + 4 g + v) = W0^T Im(.)
       v4f gh[2];
