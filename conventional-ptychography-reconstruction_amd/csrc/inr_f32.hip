@@ -48,12 +48,13 @@ constexpr int PS = 16;             // LDS row stride of the per-point staging (1
 __device__ __forceinline__ int w0pos(int f) { return 8 * ((f >> 1) & 3) + 2 * (f >> 3) + (f & 1); }
 
 template <int H>
-struct Lds16 {
+struct __align__(16) Lds16 {   // read with 16-byte loads
   float w0[32 * WS36];     // W0[o][feat16(c, g)] at column 8 g + c
   float ar[H][32 * WS36];  // Re A_k[o][i]
   float ai[H][32 * WS36];  // Im A_k[o][i]
   float b0[32];
   float cr[H][32], ci[H][32];
+  float cs[H][32];         // c_r + c_i (the three-product form's third accumulator start)
   float om[H + 1];
   float wor[32], woi[32];
   float bo[2];
@@ -78,6 +79,7 @@ __device__ void load16(Lds16<H>& s, const PtyMlp& m, int tid, int nt) {
     for (int k = 0; k < H; ++k) {
       s.cr[k][t] = m.bh[k][2 * t];
       s.ci[k][t] = m.bh[k][2 * t + 1];
+      s.cs[k][t] = m.bh[k][2 * t] + m.bh[k][2 * t + 1];
     }
     s.wor[t] = m.wo[2 * t];
     s.woi[t] = m.wo[2 * t + 1];
@@ -109,14 +111,19 @@ __device__ __forceinline__ void layer0_16(const Lds16<H>& s, const float feat[8]
 
 // (vr, vi)[mb] = A_k z + c_k, three real products per K step (Gauss):
 // T1 = Ar zr, T2 = Ai zi, T3 = (Ar + Ai)(zr + zi); vr = c_r + T1 - T2,
-// vi = c_i + T3 - T1 - T2.  Ar + Ai is formed from the two LDS values the
-// four-product form reads anyway.
+// vi = c_i + T3 - T1 - T2.  The biases start the accumulators (t1 = c_r + T1,
+// t3 = c_r + c_i + T3), so vr = t1 - t2 and vi = t3 - t1 - t2.  Ar + Ai is
+// formed from the two LDS values the four-product form reads anyway.
 template <int H>
 __device__ __forceinline__ void hidden16(const Lds16<H>& s, int k, const float zr[8], const float zi[8], int i, int g,
                                          v4f vr[2], v4f vi[2]) {
   v4f t1[2], t2[2], t3[2];
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb) t1[mb] = t2[mb] = t3[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int mb = 0; mb < 2; ++mb) {
+    t1[mb] = *reinterpret_cast<const v4f*>(&s.cr[k][16 * mb + 4 * g]);
+    t3[mb] = *reinterpret_cast<const v4f*>(&s.cs[k][16 * mb + 4 * g]);
+    t2[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
   // K chunk st = 4 qd + e reads unit16(st, g) = 16 qd + 4 g + e: the four
   // e of one row are contiguous (one 16-byte load, conflict-free at row
   // stride 36)
@@ -142,17 +149,26 @@ __device__ __forceinline__ void hidden16(const Lds16<H>& s, int k, const float z
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      vr[mb][v] = s.cr[k][16 * mb + 4 * g + v] + (t1[mb][v] - t2[mb][v]);
-      vi[mb][v] = s.ci[k][16 * mb + 4 * g + v] + (t3[mb][v] - t1[mb][v] - t2[mb][v]);
+      vr[mb][v] = t1[mb][v] - t2[mb][v];
+      vi[mb][v] = t3[mb][v] - t1[mb][v] - t2[mb][v];
     }
+}
+
+// exp(i 2 pi r) for a phase r in revolutions: r - rint(r) is exact in f32, so
+// the only roundings are those of the scaled phase (as with the radian form).
+__device__ __forceinline__ void cis_rev(float r, float& c, float& s) {
+  r = r - rintf(r);
+  s = __builtin_amdgcn_sinf(r);
+  c = __builtin_amdgcn_cosf(r);
 }
 
 __device__ __forceinline__ void act16(float w, const v4f vr[2], const v4f vi[2], float zr[8], float zi[8]) {
   const float wl = -w * 1.4426950408889634f;   // exp(-w v) = 2^(-w log2(e) v)
+  const float wr = w * 0.15915494309189535f;   // w v in revolutions
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     float c, sn;
-    cis(w * vr[s >> 2][s & 3], c, sn);
+    cis_rev(wr * vr[s >> 2][s & 3], c, sn);
     const float mag = __builtin_amdgcn_exp2f(wl * vi[s >> 2][s & 3]);
     zr[s] = mag * c;
     zi[s] = mag * sn;
@@ -185,9 +201,10 @@ __global__ __launch_bounds__(64 * kFwd16Waves, 5) void k_inr_fwd16(PtyGrid gr, P
     float zr[8], zi[8];
     {
       v4f u[2];
+      const float w0r = s.om[0] * 0.15915494309189535f;
       layer0_16<H>(s, feat, j, g, u);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) cis(s.om[0] * u[q >> 2][q & 3], zr[q], zi[q]);
+      for (int q = 0; q < 8; ++q) cis_rev(w0r * u[q >> 2][q & 3], zr[q], zi[q]);
     }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
@@ -212,7 +229,7 @@ __global__ __launch_bounds__(64 * kFwd16Waves, 5) void k_inr_fwd16(PtyGrid gr, P
 
 // ============================================================== backward
 template <int H>
-struct Stage16 {
+struct __align__(16) Stage16 {
   // rows of 16 points, element (row, point) at st_elem(row, point)
   float X[H][64 * PS];   // input of hidden layer k: rows 0..31 Re, 32..63 Im
   float F[32 * PS];      // features, one row per feature
@@ -368,11 +385,12 @@ __global__ __launch_bounds__(64 * kW16) void k_inr_bwd16(PtyGrid gr, PtyMlp m, c
     // rather than the reverse.
     {
       v4f u[2];
+      const float w0r = s.om[0] * 0.15915494309189535f;
       __builtin_amdgcn_s_setprio(2);
       layer0_16<H>(s, feat, j, g, u);
       __builtin_amdgcn_s_setprio(0);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) cis(s.om[0] * u[q >> 2][q & 3], zr[q], zi[q]);
+      for (int q = 0; q < 8; ++q) cis_rev(w0r * u[q >> 2][q & 3], zr[q], zi[q]);
     }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
