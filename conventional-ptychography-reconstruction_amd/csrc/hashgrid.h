@@ -138,6 +138,28 @@ __device__ __forceinline__ void corner_indices_u(const LevelC& c, const LevelPos
   idx[3] = level_index_u<UNIT>(c, p.gx + 1u, p.gy + 1u);
 }
 
+// The same corners when every lane's level is known to be dense (DENSE) or
+// hashed (!DENSE): only that index form is computed.
+template <bool UNIT, bool DENSE>
+__device__ __forceinline__ uint32_t level_index_k(const LevelC& c, uint32_t cx, uint32_t cy) {
+  if constexpr (DENSE) {
+    uint32_t dl = cx + cy * c.res;
+    if constexpr (UNIT) dl = dl >= c.size ? dl - c.size : dl;
+    else dl %= c.size;
+    return c.offset + dl;
+  } else {
+    return c.offset + ((cx ^ (cy * 2654435761u)) & (c.size - 1u));
+  }
+}
+
+template <bool UNIT, bool DENSE>
+__device__ __forceinline__ void corner_indices_k(const LevelC& c, const LevelPos& p, uint32_t idx[4]) {
+  idx[0] = level_index_k<UNIT, DENSE>(c, p.gx, p.gy);
+  idx[1] = level_index_k<UNIT, DENSE>(c, p.gx + 1u, p.gy);
+  idx[2] = level_index_k<UNIT, DENSE>(c, p.gx, p.gy + 1u);
+  idx[3] = level_index_k<UNIT, DENSE>(c, p.gx + 1u, p.gy + 1u);
+}
+
 __device__ __forceinline__ void corner_indices_c(const LevelC& c, const LevelPos& p, uint32_t idx[4]) {
   idx[0] = level_index_c(c, p.gx, p.gy);
   idx[1] = level_index_c(c, p.gx + 1u, p.gy);

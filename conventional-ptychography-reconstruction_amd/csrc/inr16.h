@@ -56,10 +56,17 @@ __device__ __forceinline__ void corners16(const PtyGrid& gr, const uint4* lvl, u
   LevelC c[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) c[m] = level_c_tab(lvl, dm, 4 * m + g);
+  // Lane group g holds level 4 m + g: when the four levels of m are all dense
+  // or all hashed (a wave-uniform test of the dense mask; padding levels count
+  // as hashed) only that index form is computed, else both and a select.
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const LevelPos p = level_pos_c(c[m], x0, x1);
-    corner_indices_u<UNIT>(c[m], p, idx[m]);
+    const uint32_t kinds = (dm >> (4 * m)) & 0xFu;
+    if (kinds == 0u)
+      corner_indices_k<UNIT, false>(c[m], p, idx[m]);
+    else
+      corner_indices_u<UNIT>(c[m], p, idx[m]);
     fx[m] = p.fx;
     fy[m] = p.fy;
   }
