@@ -281,12 +281,15 @@ __global__ __launch_bounds__(256) void k_up2_rows_w(const float2* __restrict__ i
 
 // k_up2_rows for n = P Q, P a power of two (16 or 32), Q = 57 (n = 912,
 // 1824: the INR grids of canvases 1824 and 3648), on register DFTs.  One
-// wavefront per workgroup takes L = 64 / P lines per job (n = Q n1 + n2 =
+// wavefront per workgroup takes L lines (L P <= 64; one line at n = 1824, so
+// that the 1824-line pass launches 1824 wavefronts rather than 912: the
+// kernel's 183 VGPRs allow two per SIMD, and with two lines per wavefront it
+// ran at 0.7 per SIMD) per job (n = Q n1 + n2 =
 // m1 + Q m2, k = k1 + P k2):
 //   pass A : lane n2 < Q of each line loads x[Q n1 + n2] (n1 < P; coalesced
 //            across the lanes), P-point DFT over n1, times W_n^{n2 k1}, into
 //            the line's LDS slot at Q k1 + n2;
-//   pass B : lane (line, k1) takes the Q values at Q k1 .. Q k1 + Q - 1:
+//   pass B : lane (line, k1) < L P takes the Q values at Q k1 .. Q k1 + Q - 1:
 //            Q-point DFT (X[k1 + P k2] for k2 < Q), the half-sample phase of
 //            each frequency, inverse Q-point DFT, times W_n^{-m1 k1}, back to
 //            the same Q slots;
@@ -294,17 +297,17 @@ __global__ __launch_bounds__(256) void k_up2_rows_w(const float2* __restrict__ i
 //            shifted line at m1 + Q m2 (m2 < P), stored with the even samples
 //            (coalesced across the lanes).
 // The exchanges are wavefront-synchronous (no workgroup barrier); LDS is
-// L n float2 (29 KB).  Twiddles exp(-2 pi i k / m), k < m = 2n, come from a
+// L n float2 (14.6 KB at n = 1824, L = 1).  Twiddles exp(-2 pi i k / m), k < m = 2n, come from a
 // table built at compile time (double precision, rounded once).
 // Output and scales as k_up2_rows.
 __constant__ constexpr CTwPairs<3648> kTwUp3648 = make_ctw_pairs<3648>();
 __constant__ constexpr CTwPairs<1824> kTwUp1824 = make_ctw_pairs<1824>();
 
-template <int P, int Q, int MODE>
+template <int P, int Q, int L, int MODE>
 __global__ __launch_bounds__(64) void k_up2_rows_r(const float2* __restrict__ in, float2* __restrict__ out,
                                                    int rows, float scale) {
-  constexpr int n = P * Q, m = 2 * n, h = n / 2, L = 64 / P;
-  static_assert(P <= 64 && 64 % P == 0 && Q <= 64, "k_up2_rows_r: P divides 64, Q at most 64");
+  constexpr int n = P * Q, m = 2 * n, h = n / 2;
+  static_assert(P <= 64 && L * P <= 64 && Q <= 64, "k_up2_rows_r: L P lanes at most 64, Q at most 64");
   static_assert(m == 3648 || m == 1824, "k_up2_rows_r: twiddle tables for n = 912, 1824");
   const float2* tw = reinterpret_cast<const float2*>(m == 3648 ? kTwUp3648.v : kTwUp1824.v);
   __shared__ float2 buf[L * n];
@@ -334,7 +337,7 @@ __global__ __launch_bounds__(64) void k_up2_rows_r(const float2* __restrict__ in
   }
   wave_lds_sync();
   // pass B
-  {
+  if (lane < L * P) {
     const int k1 = lane % P;
     float2* slot = buf + (lane / P) * n + Q * k1;
     float2 x[Q];
@@ -551,9 +554,9 @@ static void launch_up2_rows(const float2* in, float2* out, const FftPlan& Pn, in
   }
   if (up_reg(n)) {
     if (n == 1824)
-      hipLaunchKernelGGL((k_up2_rows_r<32, 57, MODE>), dim3((rows + 1) / 2), dim3(64), 0, s, in, out, rows, scale);
+      hipLaunchKernelGGL((k_up2_rows_r<32, 57, 1, MODE>), dim3(rows), dim3(64), 0, s, in, out, rows, scale);
     else
-      hipLaunchKernelGGL((k_up2_rows_r<16, 57, MODE>), dim3((rows + 3) / 4), dim3(64), 0, s, in, out, rows, scale);
+      hipLaunchKernelGGL((k_up2_rows_r<16, 57, 2, MODE>), dim3((rows + 1) / 2), dim3(64), 0, s, in, out, rows, scale);
     return;
   }
   const size_t shm = sizeof(float2) * (size_t)((u.lds_tw ? 2 * n : 0) + 2 * u.lines * n);
