@@ -543,13 +543,14 @@ __global__ __launch_bounds__(256) void k_table_dense(PtyGrid g, PlanLayout L, Pl
 }
 
 // ------------------------------------------------------------------ hashed levels
-// blockIdx.x -> (level, chunk of 256 entries) over the hashed levels in order;
-// false when past the last one.
-__device__ __forceinline__ bool hashed_chunk(const PtyGrid& g, uint32_t bid, int& l, uint32_t& chunk) {
+// blockIdx.x -> (level, chunk of `per` entries) over the hashed levels in
+// order; false when past the last one.
+__device__ __forceinline__ bool hashed_chunk(const PtyGrid& g, uint32_t bid, int& l, uint32_t& chunk,
+                                             uint32_t per = 256) {
   chunk = bid;
   for (l = 0; l < g.n_levels; ++l) {
     if ((g.dense_mask >> l) & 1u) continue;
-    const uint32_t nch = (g.size[l] + 255) / 256;
+    const uint32_t nch = (g.size[l] + per - 1) / per;
     if (chunk < nch) return true;
     chunk -= nch;
   }
@@ -662,42 +663,80 @@ __global__ __launch_bounds__(256) void k_csr_fill(PtyGrid g, PlanLayout L, PlanP
   });
 }
 
-// Hashed-level table gradient over a chunk of 256 consecutive entries of one
-// level.  The chunk's CSR terms are one contiguous range; the workgroup walks
-// it in windows of kHashWin terms: every thread forms the product
-// w(fx_j, dx) * w(fy_i, dy) * g[l][i n0 + j] of one term (coalesced term
-// loads, neighbouring entries read neighbouring cells of the same rows) into
-// LDS, then each entry's thread adds its own terms from LDS in plan order.
-// The sum order per entry is fixed, so the result is deterministic.
+// Hashed-level table gradient over a chunk of 256 EPT consecutive entries of
+// one level, EPT adjacent entries per thread.  The kernel is a chain of
+// dependent memory round trips per workgroup (offsets, terms, gathers, Adam
+// state), so with few terms per entry the number of workgroup rounds sets its
+// time and two entries per thread (half the workgroups; the pair's Adam state
+// as 16-byte loads and stores) is faster; with many terms per entry the
+// entry-serial sums dominate and one entry per thread is (the launcher picks
+// EPT from the level's mean terms per entry).  The chunk's CSR terms are one
+// contiguous range; the workgroup walks it in windows of kHashWin terms:
+// every thread forms the product w(fx_j, dx) * w(fy_i, dy) * g[l][i n0 + j]
+// of one term (coalesced term loads, neighbouring entries read neighbouring
+// cells of the same rows) into LDS, then each entry's terms are added from
+// LDS in plan order by the entry's thread.  The sum order per entry is fixed,
+// so the result is deterministic and the same for either EPT.
 constexpr int kHashWin = 2048;
 constexpr int kHashPer = kHashWin / 256;   // terms per thread and window
 
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 nt_load4(const float2* q) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(q));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void nt_store4(float2* q, float4 v) {
+  __builtin_nontemporal_store(nt_f4{v.x, v.y, v.z, v.w}, reinterpret_cast<nt_f4*>(q));
+}
+
+template <int EPT>
 __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, PtyPoints pts, PlanPtrs P,
                                                       const float2* __restrict__ gfeat,
                                                       float2* __restrict__ dtable, TableAdam ad) {
-  __shared__ int s_off[257];
+  static_assert(EPT == 1 || EPT == 2, "one or two entries per thread");
+  constexpr int CH = 256 * EPT;
+  __shared__ int s_off[CH + 1];
   __shared__ float2 s_val[kHashWin];
   __shared__ AdamCoef s_coef;   // Adam's step coefficients, formed once per workgroup
   int l;
   uint32_t chunk;
-  if (!hashed_chunk(g, blockIdx.x, l, chunk)) return;   // uniform across the workgroup
+  if (!hashed_chunk(g, blockIdx.x, l, chunk, CH)) return;   // uniform across the workgroup
   if (ad.p && threadIdx.x == 0) s_coef = adam_coef(ad.hyper, ad.b1, ad.b2, ad.eps);
   const int tid = threadIdx.x;
-  const uint32_t e0 = chunk * 256;
-  const int ne = (int)min(256u, g.size[l] - e0);
+  const uint32_t e0 = chunk * CH;
+  // a hashed level's size is a power of two >= 8, so ne is a multiple of EPT
+  const int ne = (int)min((uint32_t)CH, g.size[l] - e0);
+  const bool mine = EPT * tid < ne;
+  // even for EPT 2: level offsets and sizes are multiples of 8
+  const uint32_t t0 = g.offset[l] + e0 + EPT * tid;
   const int* off = P.csr_o + L.off_co[l] + e0;
   const uint32_t* terms = P.csr_p + L.off_cp[l];
   const float* fxl = P.fx + (int64_t)l * L.n0;
   const float* fyl = P.fy + (int64_t)l * L.n1;
   const float2* gl = gfeat + (int64_t)l * pts.n;
   const int cap = 4 * L.n0 * L.n1;
-  EntryAdam st{};
-  if (ad.p && tid < ne) st = table_adam_load(ad, g.offset[l] + e0 + tid);
+  float2 sp[EPT], sm[EPT], sv[EPT];
+  if (ad.p && mine) {
+    if constexpr (EPT == 2) {
+      const float4 p4 = nt_load4(ad.p + t0), m4 = nt_load4(ad.m + t0), v4 = nt_load4(ad.v + t0);
+      sp[0] = make_float2(p4.x, p4.y); sp[1] = make_float2(p4.z, p4.w);
+      sm[0] = make_float2(m4.x, m4.y); sm[1] = make_float2(m4.z, m4.w);
+      sv[0] = make_float2(v4.x, v4.y); sv[1] = make_float2(v4.z, v4.w);
+    } else {
+      sp[0] = nt_load(ad.p + t0);
+      sm[0] = nt_load(ad.m + t0);
+      sv[0] = nt_load(ad.v + t0);
+    }
+  }
   for (int t = tid; t <= ne; t += 256) s_off[t] = min(off[t], cap);
   __syncthreads();
   const int k0 = s_off[0], k1 = s_off[ne];
-  const int my0 = tid < ne ? s_off[tid] : 0, my1 = tid < ne ? s_off[tid + 1] : 0;
-  float ax = 0.f, ay = 0.f;
+  int r[EPT + 1];   // the thread's entries' term ranges [r[q], r[q + 1])
+#pragma unroll
+  for (int q = 0; q <= EPT; ++q) r[q] = mine ? s_off[EPT * tid + q] : 0;
+  float2 acc[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) acc[q] = make_float2(0.f, 0.f);
   for (int w0 = k0; w0 < k1; w0 += kHashWin) {
     const int wn = min(kHashWin, k1 - w0);
     const int nu = (wn + 255) / 256;   // term slots in use, uniform across the workgroup
@@ -725,22 +764,41 @@ __global__ __launch_bounds__(256) void k_table_hashed(PtyGrid g, PlanLayout L, P
       if (u < nu && t < wn) s_val[t] = make_float2(wt[u] * v[u].x, wt[u] * v[u].y);
     }
     __syncthreads();
-    const int a = max(my0, w0), b = min(my1, w0 + wn);
-    for (int k = a; k < b; ++k) {
-      const float2 v = s_val[k - w0];
-      ax += v.x;
-      ay += v.y;
-    }
+#pragma unroll
+    for (int q = 0; q < EPT; ++q)
+      for (int k = max(r[q], w0), kend = min(r[q + 1], w0 + wn); k < kend; ++k) {
+        const float2 x = s_val[k - w0];
+        acc[q].x += x.x;
+        acc[q].y += x.y;
+      }
     __syncthreads();
   }
-  if (tid < ne && ad.p) {
-    table_adam_store(ad, s_coef, g.offset[l] + e0 + tid, st, make_float2(ax, ay));
-  } else if (tid < ne && (ax != 0.f || ay != 0.f)) {
-    const uint32_t t = g.offset[l] + e0 + tid;
-    float2 d = dtable[t];
-    d.x += ax;
-    d.y += ay;
-    dtable[t] = d;
+  if (!mine) return;
+  if (ad.p) {
+    const AdamCoef co = s_coef;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      adam_elem(sp[q].x, sm[q].x, sv[q].x, acc[q].x, co);
+      adam_elem(sp[q].y, sm[q].y, sv[q].y, acc[q].y, co);
+    }
+    if constexpr (EPT == 2) {
+      *reinterpret_cast<float4*>(ad.p + t0) = make_float4(sp[0].x, sp[0].y, sp[1].x, sp[1].y);
+      nt_store4(ad.m + t0, make_float4(sm[0].x, sm[0].y, sm[1].x, sm[1].y));
+      nt_store4(ad.v + t0, make_float4(sv[0].x, sv[0].y, sv[1].x, sv[1].y));
+    } else {
+      ad.p[t0] = sp[0];
+      nt_store(ad.m + t0, sm[0]);
+      nt_store(ad.v + t0, sv[0]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < EPT; ++q)
+      if (acc[q].x != 0.f || acc[q].y != 0.f) {
+        float2 d = dtable[t0 + q];
+        d.x += acc[q].x;
+        d.y += acc[q].y;
+        dtable[t0 + q] = d;
+      }
   }
 }
 
@@ -840,11 +898,21 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
                          gfeat, A);
     }
   }
+  // entries per thread of the hashed kernel: two while the hashed levels hold
+  // at most 8 terms per entry on average (4 per point and level), else one
+  int64_t hashed_entries = 0, n_hashed = 0;
+  for (int l = 0; l < g.n_levels; ++l)
+    if (!((g.dense_mask >> l) & 1u)) {
+      hashed_entries += g.size[l];
+      ++n_hashed;
+    }
+  const int64_t n_pts = (int64_t)L.n0 * L.n1;
+  const int ept = 4 * n_pts * n_hashed <= 8 * hashed_entries ? 2 : 1;
   uint32_t n_dense_entries = 0, hashed_chunks = 0, n_split_entries = 0;
   bool coarse = true;
   for (int l = 0; l < g.n_levels; ++l) {
     if ((g.dense_mask >> l) & 1u) n_dense_entries = g.offset[l] + g.size[l];
-    else hashed_chunks += (g.size[l] + 255) / 256;
+    else hashed_chunks += (g.size[l] + 256 * ept - 1) / (256 * ept);
     // leading dense levels with at least kDenseSplit rows of points per cell
     coarse = coarse && ((g.dense_mask >> l) & 1u) && L.n1 >= kDenseSplit * ((int)g.res[l] + 1);
     if (coarse) n_split_entries = g.offset[l] + g.size[l];
@@ -854,9 +922,14 @@ int launch_table_bwd(const PtyGrid& g, const PtyPoints& pts, const float2* gfeat
     hipLaunchKernelGGL(k_table_dense, dim3((lanes + 255) / 256), dim3(256), 0, stream, g, L, P, (const float2*)A,
                        n_dense_entries, n_split_entries, (float2*)dtable, ad);
   }
-  if (hashed_chunks > 0)
-    hipLaunchKernelGGL(k_table_hashed, dim3(hashed_chunks), dim3(256), 0, stream, g, L, pts, P, gfeat,
-                       (float2*)dtable, ad);
+  if (hashed_chunks > 0) {
+    if (ept == 2)
+      hipLaunchKernelGGL(k_table_hashed<2>, dim3(hashed_chunks), dim3(256), 0, stream, g, L, pts, P, gfeat,
+                         (float2*)dtable, ad);
+    else
+      hipLaunchKernelGGL(k_table_hashed<1>, dim3(hashed_chunks), dim3(256), 0, stream, g, L, pts, P, gfeat,
+                         (float2*)dtable, ad);
+  }
   return check_launch("table_bwd");
 }
 
