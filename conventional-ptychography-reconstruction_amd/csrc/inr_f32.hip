@@ -199,17 +199,23 @@ __global__ __launch_bounds__(64 * kFwd16Waves, 5) void k_inr_fwd16(PtyGrid gr, P
     encode16(gr, s_lvl, x0, x1, g, table, feat);
     if (pts.features) store_features16(pts.features, tile, lane, feat);
     float zr[8], zi[8];
+    // a wave raises its issue priority while it feeds the matrix core, so
+    // that the other waves' encoder and activation work fills in around it
     {
       v4f u[2];
       const float w0r = s.om[0] * 0.15915494309189535f;
+      __builtin_amdgcn_s_setprio(2);
       layer0_16<H>(s, feat, j, g, u);
+      __builtin_amdgcn_s_setprio(0);
 #pragma unroll
       for (int q = 0; q < 8; ++q) cis_rev(w0r * u[q >> 2][q & 3], zr[q], zi[q]);
     }
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       v4f vr[2], vi[2];
+      __builtin_amdgcn_s_setprio(2);
       hidden16<H>(s, k, zr, zi, j, g, vr, vi);
+      __builtin_amdgcn_s_setprio(0);
       act16(s.om[k + 1], vr, vi, zr, zi);
     }
     float orr = 0.f, oi = 0.f;
