@@ -31,9 +31,17 @@ COMMON_FLAGS = [
 ]
 
 
+# Per-file additions.  inr_bf16.hip: MFMA results in VGPRs (the compiler's
+# default put the forward's accumulators in AGPRs, 16 of them, which cost one
+# v_accvgpr_read per result element and a wave per SIMD of occupancy).
+FILE_FLAGS = {
+    "inr_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON_FLAGS, *FILE_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr}")
